@@ -1,0 +1,730 @@
+/*
+ * atus_oracle.c -- CPU restatement of the ATUS-2 split-step Fourier
+ * propagator and Sobolev-gradient minimiser.
+ *
+ * TEST INFRASTRUCTURE ONLY: this is the checker used by tests/, the smoke
+ * check and the CPU-baseline leg of bench.py.  The product never links it.
+ *
+ * Each function names the reference file:line whose semantics it restates.
+ * The FFT is this file's own (iterative radix-2, powers of two; a direct
+ * O(n^2) DFT for other lengths); it stands in for FFTW 3.3.8, which the
+ * reference links (cft_3d.cpp:41-42) and which is not installed here.
+ */
+#define _GNU_SOURCE
+#include "atus_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+/* ------------------------------------------------------------------ */
+/* grid helpers                                                        */
+/* ------------------------------------------------------------------ */
+
+long long orc_npts(const orc_grid *g) { return g->n[0] * g->n[1] * g->n[2]; }
+
+/* CRT_shared::Read_header m_ar / m_ar_k (CRT_shared.h:142-169) */
+double orc_ar(const orc_grid *g) {
+  double a = g->dx[0];
+  if (g->dim > 1) a *= g->dx[1];
+  if (g->dim > 2) a *= g->dx[2];
+  return a;
+}
+double orc_ar_k(const orc_grid *g) {
+  double a = g->dk[0];
+  if (g->dim > 1) a *= g->dk[1];
+  if (g->dim > 2) a *= g->dk[2];
+  return a;
+}
+
+static void idx3(const orc_grid *g, long long l, long long ijk[3]) {
+  long long nyz = g->n[1] * g->n[2];
+  ijk[0] = l / nyz;
+  ijk[1] = (l - ijk[0] * nyz) / g->n[2];
+  ijk[2] = l - ijk[0] * nyz - ijk[1] * g->n[2];
+}
+
+/* cft_3d::Get_x (cft_3d.cpp:128-138); cft_1d.cpp:85-90 for 1D */
+void orc_get_x(const orc_grid *g, long long l, double x[3]) {
+  long long ijk[3];
+  idx3(g, l, ijk);
+  for (int a = 0; a < 3; a++) {
+    long long shift = g->n[a] / 2;
+    x[a] = (a < g->dim) ? (double)(ijk[a] - shift) * g->dx[a] : 0.0;
+  }
+}
+
+/* cft_3d::Get_k with m_bfix == false (cft_3d.cpp:145-166) */
+void orc_get_k(const orc_grid *g, long long l, double k[3]) {
+  long long ijk[3];
+  idx3(g, l, ijk);
+  for (int a = 0; a < 3; a++) {
+    long long n = g->n[a], shift = n / 2;
+    long long t = (ijk[a] + shift) % n;
+    k[a] = (a < g->dim) ? g->dk[a] * (double)(t - shift) : 0.0;
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* FFT (stands in for fftw_execute, cft_3d.cpp:64,70)                   */
+/* ------------------------------------------------------------------ */
+
+static int is_pow2(long long n) { return n > 0 && (n & (n - 1)) == 0; }
+
+/* twiddle table w[j] = exp(-2 pi i j / n), j < n/2, rounded from long
+ * double so each entry is within half an ulp */
+static double *make_twiddles(long long n) {
+  long long h = n / 2 > 0 ? n / 2 : 1;
+  double *w = (double *)malloc(sizeof(double) * 2 * h);
+  for (long long j = 0; j < h; j++) {
+    long double a = -2.0L * 3.141592653589793238462643383279502884L *
+                    (long double)j / (long double)n;
+    w[2 * j] = (double)cosl(a);
+    w[2 * j + 1] = (double)sinl(a);
+  }
+  return w;
+}
+
+/* in-place radix-2 DIT on a contiguous complex vector of power-of-two n */
+static void fft_pow2(double *x, long long n, int sign, const double *w) {
+  /* bit reversal */
+  for (long long i = 1, j = 0; i < n; i++) {
+    long long bit = n >> 1;
+    for (; j & bit; bit >>= 1) j ^= bit;
+    j ^= bit;
+    if (i < j) {
+      double tr = x[2 * i], ti = x[2 * i + 1];
+      x[2 * i] = x[2 * j];
+      x[2 * i + 1] = x[2 * j + 1];
+      x[2 * j] = tr;
+      x[2 * j + 1] = ti;
+    }
+  }
+  for (long long len = 2; len <= n; len <<= 1) {
+    long long half = len >> 1, step = n / len;
+    for (long long i = 0; i < n; i += len) {
+      for (long long j = 0; j < half; j++) {
+        double wr = w[2 * j * step];
+        double wi = (sign < 0) ? w[2 * j * step + 1] : -w[2 * j * step + 1];
+        double *a = x + 2 * (i + j), *b = x + 2 * (i + j + half);
+        double br = b[0] * wr - b[1] * wi;
+        double bi = b[0] * wi + b[1] * wr;
+        b[0] = a[0] - br;
+        b[1] = a[1] - bi;
+        a[0] += br;
+        a[1] += bi;
+      }
+    }
+  }
+}
+
+/* direct DFT for lengths that are not powers of two (small n only) */
+static void dft_direct(double *x, long long n, int sign, double *tmp) {
+  for (long long k = 0; k < n; k++) {
+    long double sr = 0, si = 0;
+    for (long long j = 0; j < n; j++) {
+      long double a = (long double)sign * 2.0L *
+                      3.141592653589793238462643383279502884L *
+                      (long double)((j * k) % n) / (long double)n;
+      long double c = cosl(a), s = sinl(a);
+      sr += x[2 * j] * c - x[2 * j + 1] * s;
+      si += x[2 * j] * s + x[2 * j + 1] * c;
+    }
+    tmp[2 * k] = (double)sr;
+    tmp[2 * k + 1] = (double)si;
+  }
+  memcpy(x, tmp, sizeof(double) * 2 * n);
+}
+
+static void fft_line(double *x, long long n, int sign, const double *w,
+                     double *tmp) {
+  if (n == 1) return;
+  if (is_pow2(n))
+    fft_pow2(x, n, sign, w);
+  else
+    dft_direct(x, n, sign, tmp);
+}
+
+#define ORC_BLK 16
+
+/* transform along axis `a` of the [n0][n1][n2] array; pencils of the
+ * strided axes are gathered ORC_BLK at a time so every read touches whole
+ * cache lines */
+static void fft_axis(const orc_grid *g, double *data, int a, int sign) {
+  long long n = g->n[a];
+  if (n == 1) return;
+  long long inner = 1, outer = 1;
+  for (int b = a + 1; b < 3; b++) inner *= g->n[b];
+  for (int b = 0; b < a; b++) outer *= g->n[b];
+  double *w = make_twiddles(n);
+  if (inner == 1) {
+#pragma omp parallel
+    {
+      double *tmp = (double *)malloc(sizeof(double) * 2 * n);
+#pragma omp for schedule(static)
+      for (long long o = 0; o < outer; o++)
+        fft_line(data + 2 * o * n, n, sign, w, tmp);
+      free(tmp);
+    }
+  } else {
+    long long nblk = (inner + ORC_BLK - 1) / ORC_BLK;
+#pragma omp parallel
+    {
+      double *buf = (double *)malloc(sizeof(double) * 2 * n * ORC_BLK);
+      double *tmp = (double *)malloc(sizeof(double) * 2 * n);
+#pragma omp for schedule(static)
+      for (long long ob = 0; ob < outer * nblk; ob++) {
+        long long o = ob / nblk, c0 = (ob % nblk) * ORC_BLK;
+        long long cw = inner - c0 < ORC_BLK ? inner - c0 : ORC_BLK;
+        double *base = data + 2 * (o * n * inner + c0);
+        for (long long i = 0; i < n; i++)
+          for (long long c = 0; c < cw; c++) {
+            buf[2 * (c * n + i)] = base[2 * (i * inner + c)];
+            buf[2 * (c * n + i) + 1] = base[2 * (i * inner + c) + 1];
+          }
+        for (long long c = 0; c < cw; c++)
+          fft_line(buf + 2 * c * n, n, sign, w, tmp);
+        for (long long i = 0; i < n; i++)
+          for (long long c = 0; c < cw; c++) {
+            base[2 * (i * inner + c)] = buf[2 * (c * n + i)];
+            base[2 * (i * inner + c) + 1] = buf[2 * (c * n + i) + 1];
+          }
+      }
+      free(buf);
+      free(tmp);
+    }
+  }
+  free(w);
+}
+
+void orc_fft(const orc_grid *g, double *data, int sign) {
+  for (int a = 0; a < 3; a++) fft_axis(g, data, a, sign);
+}
+
+static void scale_all(double *data, long long npts, double fak) {
+#pragma omp parallel for schedule(static)
+  for (long long i = 0; i < 2 * npts; i++) data[i] *= fak;
+}
+
+/* the scale() factor of cft_Nd (cft_1d.cpp:193, cft_2d.cpp:451,
+ * cft_3d.cpp:629), same floating-point expression per dimension */
+static double ft_factor(const orc_grid *g, const double *s) {
+  switch (g->dim) {
+  case 1: return s[0] / sqrt(2.0 * M_PI);
+  case 2: return 0.5 * s[0] * s[1] / M_PI;
+  default: return s[0] * s[1] * s[2] / pow(2 * M_PI, 1.5);
+  }
+}
+
+/* cft_Nd::ft (cft_3d.cpp:58-74, m_bfix == false path) */
+void orc_ft(const orc_grid *g, double *data, int isign) {
+  if (isign != 1 && isign != -1) return;
+  orc_fft(g, data, isign);
+  scale_all(data, orc_npts(g), ft_factor(g, isign == -1 ? g->dx : g->dk));
+}
+
+/* ------------------------------------------------------------------ */
+/* real-time propagator                                                */
+/* ------------------------------------------------------------------ */
+
+/* CRT_Base::Init (CRT_Base.h:253-274): phi = -dt * k.scale(alpha).k, the
+ * table holds cos/sin(s * phi) with s = 0.5 (half) or 1 (full) */
+void orc_kinetic_table(const orc_grid *g, const double alpha[3], double dt,
+                       double s, double *tab) {
+  long long np = orc_npts(g);
+  const double mdt = -dt;
+#pragma omp parallel for schedule(static)
+  for (long long l = 0; l < np; l++) {
+    double k[3];
+    orc_get_k(g, l, k);
+    double ks = 0;
+    for (int a = 0; a < g->dim; a++) ks += (k[a] * alpha[a]) * k[a];
+    double phi = mdt * ks;
+    double ph = (s == 1.0) ? phi : s * phi;
+    tab[2 * l] = cos(ph);
+    tab[2 * l + 1] = sin(ph);
+  }
+}
+
+/* CRT_Base::Do_FT_Step_full / _half (CRT_Base.h:337-392) */
+void orc_ft_step(const orc_grid *g, int ncomp, double **psi,
+                 const double *tab) {
+  long long np = orc_npts(g);
+  for (int c = 0; c < ncomp; c++) orc_ft(g, psi[c], -1);
+  for (int c = 0; c < ncomp; c++) {
+    double *p = psi[c];
+#pragma omp parallel for schedule(static)
+    for (long long l = 0; l < np; l++) {
+      double re = p[2 * l], im = p[2 * l + 1];
+      p[2 * l] = re * tab[2 * l] - im * tab[2 * l + 1];
+      p[2 * l + 1] = im * tab[2 * l] + re * tab[2 * l + 1];
+    }
+  }
+  for (int c = 0; c < ncomp; c++) orc_ft(g, psi[c], 1);
+}
+
+/* CRT_Base::Do_NL_Step (CRT_Base.h:397-441): all phases from the
+ * pre-update densities, then psi_i *= exp(i phi_i) */
+void orc_nl_step(const orc_grid *g, int ncomp, double **psi, const double *gs,
+                 double dt, double **pot) {
+  long long np = orc_npts(g);
+  const double mdt = -dt;
+#pragma omp parallel for schedule(static)
+  for (long long l = 0; l < np; l++) {
+    double phi[4];
+    for (int i = 0; i < ncomp; i++) {
+      phi[i] = (pot && pot[i]) ? pot[i][l] : 0.0;
+      for (int j = 0; j < ncomp; j++)
+        phi[i] += gs[j + ncomp * i] *
+                  (psi[j][2 * l] * psi[j][2 * l] +
+                   psi[j][2 * l + 1] * psi[j][2 * l + 1]);
+      phi[i] *= mdt;
+    }
+    for (int i = 0; i < ncomp; i++) {
+      double s, c;
+      sincos(phi[i], &s, &c);
+      double re = psi[i][2 * l], im = psi[i][2 * l + 1];
+      psi[i][2 * l] = re * c - im * s;
+      psi[i][2 * l + 1] = im * c + re * s;
+    }
+  }
+}
+
+/* CRT_Base_IF::Do_NL_Step (CRT_Base_IF.h:381-419): gravity along the first
+ * axis and per-component detuning; the reference loop is serial, the result
+ * does not depend on the order */
+void orc_nl_if_step(const orc_grid *g, int ncomp, double **psi,
+                    const double *gs, double dt, double beta0,
+                    const double *deltaL) {
+  long long np = orc_npts(g);
+  const double mdt = -dt;
+#pragma omp parallel for schedule(static)
+  for (long long l = 0; l < np; l++) {
+    double phi[4], x[3];
+    orc_get_x(g, l, x);
+    for (int i = 0; i < ncomp; i++) {
+      phi[i] = 0;
+      for (int j = 0; j < ncomp; j++)
+        phi[i] += gs[j + ncomp * i] *
+                  (psi[j][2 * l] * psi[j][2 * l] +
+                   psi[j][2 * l + 1] * psi[j][2 * l + 1]);
+      phi[i] += beta0 * x[0] - deltaL[i];
+      phi[i] *= mdt;
+    }
+    for (int i = 0; i < ncomp; i++) {
+      double s, c;
+      sincos(phi[i], &s, &c);
+      double re = psi[i][2 * l], im = psi[i][2 * l + 1];
+      psi[i][2 * l] = re * c - im * s;
+      psi[i][2 * l + 1] = im * c + re * s;
+    }
+  }
+}
+
+/* Bragg_single::Do_Bragg_ad (bragg.cpp:186-304): per point, the analytic
+ * exponential of the 2x2 Hermitian H = [[V11, Om*eta],[Om*conj(eta), V22]]
+ * applied to (psi1, psi2).  Operation order follows the reference so that
+ * rounding matches closely. */
+void orc_bragg_ad(const orc_grid *g, double *psi1, double *psi2,
+                  const orc_bragg_params *p) {
+  long long np = orc_npts(g);
+  const double dt = p->dt;
+  const double t1 = p->t + 0.5 * dt;
+  const double F = p->F;
+#pragma omp parallel for schedule(static)
+  for (long long l = 0; l < np; l++) {
+    double x[3];
+    orc_get_x(g, l, x);
+    double *a = psi1 + 2 * l, *b = psi2 + 2 * l;
+    double n1 = a[0] * a[0] + a[1] * a[1];
+    double n2 = b[0] * b[0] + b[1] * b[1];
+    double V11 = p->gs[0] * n1 + p->gs[1] * n2 + p->beta0 * x[0];
+    double V22 = p->gs[2] * n1 + p->gs[3] * n2 - p->deltaL1 + p->beta0 * x[0];
+    double Om = F * p->amp0 *
+                cos(p->laser_k * x[0] -
+                    (p->laser_domh + p->chirp * t1 + p->chirp_rate0 * t1) * t1 +
+                    p->phase0 / 2);
+    double s, c;
+    if (Om == 0.0) {
+      sincos(-dt * V11, &s, &c);
+      double re = a[0];
+      a[0] = re * c - a[1] * s;
+      a[1] = re * s + a[1] * c;
+      sincos(-dt * V22, &s, &c);
+      re = b[0];
+      b[0] = re * c - b[1] * s;
+      b[1] = re * s + b[1] * c;
+      continue;
+    }
+    double etr, eti;
+    sincos(-0.5 * (p->laser_dk * x[0]), &eti, &etr);
+    double r = sqrt((V11 - V22) * (V11 - V22) + 4.0 * Om * Om);
+    double Ep = 0.5 * (V11 + V22 + r);
+    double Em = 0.5 * (V11 + V22 - r);
+    double gpr, gpi, gmr, gmi, nrm;
+    sincos(-dt * Ep, &s, &c);
+    nrm = 1.0 / fabs((V11 - Ep) * (V11 - Ep) + Om * Om);
+    gpr = nrm * c;
+    gpi = nrm * s;
+    sincos(-dt * Em, &s, &c);
+    nrm = 1.0 / fabs((V11 - Em) * (V11 - Em) + Om * Om);
+    gmr = nrm * c;
+    gmi = nrm * s;
+    double o2 = Om * Om;
+    double O11r = o2 * (gpr + gmr), O11i = o2 * (gpi + gmi);
+    double dp = V11 - Ep, dm = V11 - Em;
+    double O22r = dp * dp * gpr + dm * dm * gmr;
+    double O22i = dp * dp * gpi + dm * dm * gmi;
+    double Xr = -Om * (dp * gpr + dm * gmr);
+    double Xi = -Om * (dp * gpi + dm * gmi);
+    /* O12 = X * eta, O21 = X * conj(eta) */
+    double O12r = etr * Xr - eti * Xi, O12i = eti * Xr + etr * Xi;
+    double O21r = etr * Xr + eti * Xi, O21i = etr * Xi - eti * Xr;
+    double ar = a[0], ai = a[1], br = b[0], bi = b[1];
+    a[0] = (O11r * ar - O11i * ai) + (O12r * br - O12i * bi);
+    a[1] = (O11r * ai + O11i * ar) + (O12r * bi + O12i * br);
+    b[0] = (O21r * ar - O21i * ai) + (O22r * br - O22i * bi);
+    b[1] = (O21r * ai + O21i * ar) + (O22r * bi + O22i * br);
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* reductions                                                          */
+/* ------------------------------------------------------------------ */
+
+/* CRT_Base::Get_Particle_Number (CRT_Base.h:587-599) */
+double orc_particle_number(const orc_grid *g, const double *psi) {
+  long long np = orc_npts(g);
+  double s = 0;
+#pragma omp parallel for reduction(+ : s) schedule(static)
+  for (long long l = 0; l < np; l++)
+    s += psi[2 * l] * psi[2 * l] + psi[2 * l + 1] * psi[2 * l + 1];
+  return orc_ar(g) * s;
+}
+
+/* CRT_Base::Expval_Position (CRT_Base.h:481-525) */
+void orc_expval_x(const orc_grid *g, const double *psi, double out[3]) {
+  long long np = orc_npts(g);
+  double s0 = 0, s1 = 0, s2 = 0;
+#pragma omp parallel for reduction(+ : s0, s1, s2) schedule(static)
+  for (long long l = 0; l < np; l++) {
+    double x[3];
+    orc_get_x(g, l, x);
+    double d = psi[2 * l] * psi[2 * l] + psi[2 * l + 1] * psi[2 * l + 1];
+    s0 += x[0] * d;
+    s1 += x[1] * d;
+    s2 += x[2] * d;
+  }
+  double ar = orc_ar(g);
+  out[0] = ar * s0;
+  out[1] = g->dim > 1 ? ar * s1 : 0.0;
+  out[2] = g->dim > 2 ? ar * s2 : 0.0;
+}
+
+/* CRT_Base::Expval_Momentum (CRT_Base.h:533-580) */
+void orc_expval_k(const orc_grid *g, double *psi, double out[3]) {
+  long long np = orc_npts(g);
+  orc_ft(g, psi, -1);
+  double s0 = 0, s1 = 0, s2 = 0;
+#pragma omp parallel for reduction(+ : s0, s1, s2) schedule(static)
+  for (long long l = 0; l < np; l++) {
+    double k[3];
+    orc_get_k(g, l, k);
+    double d = psi[2 * l] * psi[2 * l] + psi[2 * l + 1] * psi[2 * l + 1];
+    s0 += k[0] * d;
+    s1 += k[1] * d;
+    s2 += k[2] * d;
+  }
+  orc_ft(g, psi, 1);
+  double ark = orc_ar_k(g);
+  out[0] = ark * s0;
+  out[1] = g->dim > 1 ? ark * s1 : 0.0;
+  out[2] = g->dim > 2 ? ark * s2 : 0.0;
+}
+
+/* CRT_Base_IF::compute_rabi_integrals (CRT_Base_IF.h:208-281), including
+ * the reference's slot counter, which advances only on a miss */
+void orc_rabi_integrals(const orc_grid *g, double *psi, const double *k0,
+                        int nk0, double threshold, double *out) {
+  long long np = orc_npts(g);
+  orc_ft(g, psi, -1);
+  double *acc = (double *)calloc((size_t)nk0, sizeof(double));
+#pragma omp parallel
+  {
+    double *loc = (double *)calloc((size_t)nk0, sizeof(double));
+#pragma omp for schedule(static)
+    for (long long l = 0; l < np; l++) {
+      double k[3];
+      orc_get_k(g, l, k);
+      int j = 0;
+      for (int q = 0; q < nk0; q++) {
+        double dd = 0;
+        for (int a = 0; a < g->dim; a++) {
+          double d = k0[3 * q + a] - k[a];
+          dd += d * d;
+        }
+        if (sqrt(dd) < threshold) {
+          loc[j] += psi[2 * l] * psi[2 * l] + psi[2 * l + 1] * psi[2 * l + 1];
+          continue;
+        }
+        j++;
+      }
+    }
+#pragma omp critical
+    for (int q = 0; q < nk0; q++) acc[q] += loc[q];
+    free(loc);
+  }
+  double ark = orc_ar_k(g);
+  for (int q = 0; q < nk0; q++) out[q] = ark * acc[q];
+  free(acc);
+  orc_ft(g, psi, 1);
+}
+
+/* Energy functional.  The reference defines no energy; this is
+ *   E = ar_k sum_i sum_k (sum_a alpha_a k_a^2) |psi_i^(k)|^2
+ *     + ar sum_i sum_x [ V_i |psi_i|^2 + 1/2 sum_j g_ij |psi_i|^2 |psi_j|^2 ]
+ * with psi^ = ft(-1) psi, i.e. the Hamiltonian whose kinetic propagator is
+ * the reference's exp(-i dt alpha k^2) (CRT_Base.h:253-274). */
+double orc_energy(const orc_grid *g, int ncomp, double **psi, const double *gs,
+                  const double alpha[3], double **pot) {
+  long long np = orc_npts(g);
+  double ekin = 0, epot = 0;
+  double *tmp = (double *)malloc(sizeof(double) * 2 * np);
+  for (int i = 0; i < ncomp; i++) {
+    memcpy(tmp, psi[i], sizeof(double) * 2 * np);
+    orc_ft(g, tmp, -1);
+    double s = 0;
+#pragma omp parallel for reduction(+ : s) schedule(static)
+    for (long long l = 0; l < np; l++) {
+      double k[3];
+      orc_get_k(g, l, k);
+      double ks = 0;
+      for (int a = 0; a < g->dim; a++) ks += alpha[a] * k[a] * k[a];
+      s += ks * (tmp[2 * l] * tmp[2 * l] + tmp[2 * l + 1] * tmp[2 * l + 1]);
+    }
+    ekin += s;
+  }
+  free(tmp);
+#pragma omp parallel for reduction(+ : epot) schedule(static)
+  for (long long l = 0; l < np; l++) {
+    double n[4];
+    for (int i = 0; i < ncomp; i++)
+      n[i] = psi[i][2 * l] * psi[i][2 * l] + psi[i][2 * l + 1] * psi[i][2 * l + 1];
+    for (int i = 0; i < ncomp; i++) {
+      double v = (pot && pot[i]) ? pot[i][l] : 0.0;
+      double nl = 0;
+      for (int j = 0; j < ncomp; j++) nl += gs[j + ncomp * i] * n[j];
+      epot += (v + 0.5 * nl) * n[i];
+    }
+  }
+  return orc_ar_k(g) * ekin + orc_ar(g) * epot;
+}
+
+/* ------------------------------------------------------------------ */
+/* sequence block                                                      */
+/* ------------------------------------------------------------------ */
+
+void orc_rt_set_dt(orc_rt *s, double dt) {
+  s->dt = dt;
+  orc_kinetic_table(&s->g, s->alpha, dt, 0.5, s->half_tab);
+  orc_kinetic_table(&s->g, s->alpha, dt, 1.0, s->full_tab);
+  s->tab_dt = dt;
+}
+
+static void v_step(orc_rt *s, int kind) {
+  if (kind == ORC_FREEPROP) {
+    orc_nl_step(&s->g, s->ncomp, s->psi, s->gs, s->dt, s->pot);
+  } else if (kind == ORC_FREEPROP_IF) {
+    orc_nl_if_step(&s->g, s->ncomp, s->psi, s->gs, s->dt, s->beta0, s->deltaL);
+  } else {
+    orc_bragg_params p;
+    p.dt = s->dt;
+    p.t = s->t;
+    /* Amplitude_at_time (CRT_Base_IF.h:171-198) at Get_t() + 0.5*Get_dt() */
+    p.F = s->amp ? s->amp(s->t + 0.5 * s->dt, s->amp_user) : 1.0;
+    p.amp0 = s->amp0;
+    p.laser_k = s->laser_k;
+    p.laser_domh = s->laser_domh;
+    p.laser_dk = s->laser_dk;
+    p.chirp = s->chirp;
+    p.chirp_rate0 = s->chirp_rate0;
+    p.phase0 = s->phase0;
+    p.beta0 = s->beta0;
+    p.deltaL1 = s->deltaL[1];
+    for (int q = 0; q < 4; q++) p.gs[q] = s->gs[q];
+    orc_bragg_ad(&s->g, s->psi[0], s->psi[1], &p);
+  }
+}
+
+/* CRT_Base::run_sequence hot loop (CRT_Base.h:776-786); the time
+ * bookkeeping is Do_FT_Step_half: t += 0.5*dt (:391), full: t += dt (:361) */
+void orc_rt_block(orc_rt *s, int kind, int Nk) {
+  orc_ft_step(&s->g, s->ncomp, s->psi, s->half_tab);
+  s->t += 0.5 * s->dt;
+  for (int j = 2; j <= Nk; j++) {
+    v_step(s, kind);
+    orc_ft_step(&s->g, s->ncomp, s->psi, s->full_tab);
+    s->t += s->dt;
+  }
+  v_step(s, kind);
+  orc_ft_step(&s->g, s->ncomp, s->psi, s->half_tab);
+  s->t += 0.5 * s->dt;
+}
+
+/* ------------------------------------------------------------------ */
+/* Sobolev gradient minimiser (CSOB_Base.h)                            */
+/* ------------------------------------------------------------------ */
+
+/* CSOB_Base::Init (CSOB_Base.h:229-248): op = 1/(1+alpha|k|^2),
+ * Lap = -alpha|k|^2 with isotropic alpha = Alpha_i[0] */
+void orc_sob_init_ops(orc_sob *s) {
+  long long np = orc_npts(&s->g);
+  for (int j = 0; j < s->nwf; j++) {
+    double al = s->alpha[j];
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < np; i++) {
+      double k[3];
+      orc_get_k(&s->g, i, k);
+      double phi = 0;
+      for (int a = 0; a < s->g.dim; a++) phi += (k[a] * al) * k[a];
+      s->op[j][i] = 1.0 / (1.0 + phi);
+      s->lapop[j][i] = -phi;
+    }
+  }
+}
+
+static double sob_pn(const orc_sob *s, int i) {
+  return orc_particle_number(&s->g, s->psi[i]);
+}
+
+/* CSOB_Base::Renormalize_All_Psi (CSOB_Base.h:523-543) */
+void orc_sob_renormalize(orc_sob *s) {
+  long long np = orc_npts(&s->g);
+  for (int i = 0; i < s->nwf; i++) s->N[i] = sob_pn(s, i);
+  for (int i = 0; i < s->nwf; i++) {
+    double fak = 1.0 / sqrt(s->N[i]);
+    scale_all(s->psi[i], np, fak);
+  }
+  for (int i = 0; i < s->nwf; i++) s->N[i] = 1;
+}
+
+/* out = ft(+1)[ mult * ft(-1)[in] ]; the reference copies into the field
+ * buffer, transforms in place and copies out (CSOB_Base.h:251-301) */
+static void sob_filter(const orc_sob *s, const double *in, const double *mult,
+                       double *out) {
+  long long np = orc_npts(&s->g);
+  if (out != in) memcpy(out, in, sizeof(double) * 2 * np);
+  orc_ft(&s->g, out, -1);
+#pragma omp parallel for schedule(static)
+  for (long long l = 0; l < np; l++) {
+    out[2 * l] *= mult[l];
+    out[2 * l + 1] *= mult[l];
+  }
+  orc_ft(&s->g, out, 1);
+}
+
+/* one body of the do-while loop of CSOB_Base::run (CSOB_Base.h:431-503) */
+void orc_sob_iter(orc_sob *s) {
+  long long np = orc_npts(&s->g);
+  int nw = s->nwf;
+  /* Compute_Sobolev_Psi (:277-301) */
+  for (int i = 0; i < nw; i++) sob_filter(s, s->psi[i], s->op[i], s->sob[i]);
+  /* Compute_Laplace (:251-274) */
+  for (int i = 0; i < nw; i++) sob_filter(s, s->psi[i], s->lapop[i], s->lap[i]);
+  /* Compute_Sobolev_Gradient (:304-337) */
+  for (int i = 0; i < nw; i++) {
+    double *in = s->grad[i];
+#pragma omp parallel for schedule(static)
+    for (long long l = 0; l < np; l++) {
+      double nl = 0;
+      for (int j = 0; j < nw; j++)
+        nl += s->gs[j + i * nw] * (s->psi[j][2 * l] * s->psi[j][2 * l] +
+                                   s->psi[j][2 * l + 1] * s->psi[j][2 * l + 1]);
+      double v = s->pot[i][l] + nl;
+      in[2 * l] = -s->lap[i][2 * l] + v * s->psi[i][2 * l];
+      in[2 * l + 1] = -s->lap[i][2 * l + 1] + v * s->psi[i][2 * l + 1];
+    }
+    sob_filter(s, in, s->op[i], in);
+  }
+  /* Project_Sobolev_Gradient (:340-367) */
+  for (int i = 0; i < nw; i++) {
+    double t1 = 0, t2 = 0;
+    const double *ps = s->psi[i], *sb = s->sob[i];
+    double *gr = s->grad[i];
+#pragma omp parallel for reduction(+ : t1, t2) schedule(static)
+    for (long long l = 0; l < np; l++) {
+      t1 += ps[2 * l] * gr[2 * l] + ps[2 * l + 1] * gr[2 * l + 1];
+      t2 += ps[2 * l] * sb[2 * l] + ps[2 * l + 1] * sb[2 * l + 1];
+    }
+    double fak = t1 / t2;
+#pragma omp parallel for schedule(static)
+    for (long long l = 0; l < np; l++) {
+      gr[2 * l] -= fak * sb[2 * l];
+      gr[2 * l + 1] -= fak * sb[2 * l + 1];
+    }
+  }
+  /* Compute_res (:370-397); the reference's m_res array has `dim`
+   * entries of which only the first no_wf are written: the unwritten ones
+   * are taken as zero here */
+  s->res_tot = 0;
+  for (int i = 0; i < nw; i++) {
+    const double *gr = s->grad[i];
+    double t = 0;
+#pragma omp parallel for reduction(+ : t) schedule(static)
+    for (long long l = 0; l < np; l++)
+      t += gr[2 * l] * gr[2 * l] + gr[2 * l + 1] * gr[2 * l + 1];
+    s->res[i] = t * orc_ar(&s->g);
+    s->res_tot += s->res[i] * s->res[i];
+  }
+  s->res_tot = sqrt(s->res_tot);
+  /* norm check and renormalisation after the gradient (:470-481) */
+  int brenorm = 0;
+  for (int i = 0; i < nw; i++) {
+    s->N[i] = sob_pn(s, i);
+    if (fabs(s->N[i] - 1) > 1e-5) brenorm = 1;
+  }
+  s->renormalised = brenorm;
+  if (brenorm) orc_sob_renormalize(s);
+  /* update (:483-492) */
+  for (int i = 0; i < nw; i++) {
+    double *ps = s->psi[i];
+    const double *gr = s->grad[i];
+    const double st = s->stepsize;
+#pragma omp parallel for schedule(static)
+    for (long long l = 0; l < np; l++) {
+      ps[2 * l] -= st * gr[2 * l];
+      ps[2 * l + 1] -= st * gr[2 * l + 1];
+    }
+  }
+}
+
+/* CSOB_Base::Compute_mu (CSOB_Base.h:400-424; never called by the
+ * reference, Get_Ar() taken as m_ar):
+ *   mu_i = ar * sum [ -Re(conj(psi_i) * Lap_i) + (V_i + sum_j g_ij |psi_j|^2) |psi_i|^2 ]
+ * with Lap_i = ft(+1)[-alpha |k|^2 ft(-1) psi_i] */
+void orc_sob_mu(orc_sob *s, double *mu) {
+  long long np = orc_npts(&s->g);
+  int nw = s->nwf;
+  for (int i = 0; i < nw; i++) sob_filter(s, s->psi[i], s->lapop[i], s->lap[i]);
+  for (int i = 0; i < nw; i++) {
+    double m = 0;
+    const double *ps = s->psi[i], *lp = s->lap[i];
+#pragma omp parallel for reduction(+ : m) schedule(static)
+    for (long long l = 0; l < np; l++) {
+      double nl = 0;
+      for (int j = 0; j < nw; j++)
+        nl += s->gs[j + i * nw] * (s->psi[j][2 * l] * s->psi[j][2 * l] +
+                                   s->psi[j][2 * l + 1] * s->psi[j][2 * l + 1]);
+      m += -(ps[2 * l] * lp[2 * l] + ps[2 * l + 1] * lp[2 * l + 1]) +
+           (s->pot[i][l] + nl) * (ps[2 * l] * ps[2 * l] + ps[2 * l + 1] * ps[2 * l + 1]);
+    }
+    mu[i] = orc_ar(&s->g) * m;
+  }
+}
