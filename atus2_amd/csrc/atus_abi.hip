@@ -1,0 +1,892 @@
+// atus_abi.hip -- the extern "C" boundary of libatus_hip.so (include/atus_hip.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/atus_hip.h"
+#include "kernels_misc.hpp"
+#include "launch.hpp"
+
+using namespace atus;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                          \
+  do {                                                                                         \
+    hipError_t e_ = (expr);                                                                    \
+    if (e_ != hipSuccess) return fail(ATUS_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define ATUS_TRY(expr)     \
+  do {                     \
+    int r_ = (expr);       \
+    if (r_ != ATUS_OK) return r_; \
+  } while (0)
+
+// the scale() factor of cft_Nd (cft_1d.cpp:193, cft_2d.cpp:451, cft_3d.cpp:629)
+double ft_factor(int dim, const double* s) {
+  switch (dim) {
+    case 1: return s[0] / sqrt(2.0 * M_PI);
+    case 2: return 0.5 * s[0] * s[1] / M_PI;
+    default: return s[0] * s[1] * s[2] / pow(2 * M_PI, 1.5);
+  }
+}
+
+unsigned red_blocks(long long n) {
+  long long b = (n + RED_THREADS - 1) / RED_THREADS;
+  if (b > RED_MAX_BLOCKS) b = RED_MAX_BLOCKS;
+  if (b < 1) b = 1;
+  return (unsigned)b;
+}
+
+unsigned ew_blocks(long long n) {
+  long long b = (n + 255) / 256;
+  if (b > 8192) b = 8192;
+  if (b < 1) b = 1;
+  return (unsigned)b;
+}
+
+}  // namespace
+
+struct atus_ctx {
+  int dim = 0;
+  int n[3] = {1, 1, 1};
+  double dx[3] = {0, 0, 0};
+  double dk[3] = {0, 0, 0};
+  int ncomp = 0;
+  long long npts = 0;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  size_t max_lds = 65536;
+
+  double2* psi = nullptr;   // ncomp * npts
+  double* pot = nullptr;    // ncomp * npts, or null
+  double2* scratch = nullptr;
+  std::map<int, double2*> tw;
+
+  double2* kin = nullptr;   // [2 fracs][3 axes][nmax]
+  int kin_nmax = 0;
+  double kin_dt = NAN;
+  bool kin_valid = false;
+  double alpha[3] = {0, 0, 0};
+  double gs[16] = {0};
+
+  double* red_part = nullptr;
+  double* red_out = nullptr;
+  double* red_host = nullptr;
+
+  bool sob_ready = false;
+  double sob_alpha[2] = {0, 0};
+  double sob_step = 0;
+  double2 *sob = nullptr, *lap = nullptr, *grad = nullptr;
+
+  bool timing = false;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms = 0;
+
+  // per-launch event pairs while profiling
+  bool prof = false;
+  std::vector<hipEvent_t> prof_ev;
+  std::vector<int> prof_fam;
+
+  Grid grid() const {
+    Grid g{};
+    g.dim = dim;
+    for (int a = 0; a < 3; ++a) {
+      g.n[a] = n[a];
+      g.dx[a] = dx[a];
+      g.dk[a] = dk[a];
+    }
+    g.npts = npts;
+    return g;
+  }
+  double ar() const {
+    double a = dx[0];
+    if (dim > 1) a *= dx[1];
+    if (dim > 2) a *= dx[2];
+    return a;
+  }
+  double ar_k() const {
+    double a = dk[0];
+    if (dim > 1) a *= dk[1];
+    if (dim > 2) a *= dk[2];
+    return a;
+  }
+  long long inner_of(int a) const {
+    long long v = 1;
+    for (int b = a + 1; b < dim; ++b) v *= n[b];
+    return v;
+  }
+  long long outer_of(int a) const {
+    long long v = 1;
+    for (int b = 0; b < a; ++b) v *= n[b];
+    return v;
+  }
+};
+
+namespace {
+
+int check_ctx(atus_ctx* c) {
+  if (!c) return fail(ATUS_ERR_ARG, "null context");
+  return hipSetDevice(c->device) == hipSuccess ? ATUS_OK : fail(ATUS_ERR_HIP, "hipSetDevice failed");
+}
+
+int check_comp(atus_ctx* c, int comp) {
+  if (comp < 0 || comp >= c->ncomp)
+    return fail(ATUS_ERR_ARG, "component " + std::to_string(comp) + " out of range [0," +
+                                  std::to_string(c->ncomp) + ")");
+  return ATUS_OK;
+}
+
+// brackets one launch with an event pair when the context is profiling
+struct ProfScope {
+  atus_ctx* c;
+  bool on;
+  ProfScope(atus_ctx* cc, int fam) : c(cc), on(cc->prof) {
+    if (!on) return;
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) {
+      on = false;
+      return;
+    }
+    (void)hipEventRecord(e, c->stream);
+    c->prof_ev.push_back(e);
+    c->prof_fam.push_back(fam);
+  }
+  ~ProfScope() {
+    if (!on) return;
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    (void)hipEventRecord(e, c->stream);
+    c->prof_ev.push_back(e);
+  }
+};
+
+int ensure_twiddles(atus_ctx* c, int N) {
+  if (c->tw.count(N)) return ATUS_OK;
+  std::vector<double2> h(N);
+  for (int j = 0; j < N; ++j) {
+    long double a = -2.0L * 3.141592653589793238462643383279502884L * (long double)j / (long double)N;
+    h[j] = make_double2((double)cosl(a), (double)sinl(a));
+  }
+  double2* d = nullptr;
+  if (hipMalloc(&d, sizeof(double2) * N) != hipSuccess) return fail(ATUS_ERR_NOMEM, "twiddle table allocation");
+  HIP_TRY(hipMemcpy(d, h.data(), sizeof(double2) * N, hipMemcpyHostToDevice));
+  c->tw[N] = d;
+  return ATUS_OK;
+}
+
+// per-axis kinetic phase tables for frac 0.5 (slot 0) and 1 (slot 1):
+//   E_a[j] = exp(i * s * phi_a(j)),  phi_a = -dt * (k_a alpha_a) k_a
+// (CRT_Base::Init, CRT_Base.h:253-274, factorised over the axes); axis 0
+// carries the ft(-1) * ft(+1) scale.
+int ensure_kin(atus_ctx* c, double dt) {
+  if (c->kin_valid && c->kin_dt == dt) return ATUS_OK;
+  const int nm = c->kin_nmax;
+  std::vector<double2> h((size_t)2 * 3 * nm, make_double2(1.0, 0.0));
+  const double scale = ft_factor(c->dim, c->dx) * ft_factor(c->dim, c->dk);
+  const double mdt = -dt;
+  for (int f = 0; f < 2; ++f) {
+    for (int a = 0; a < c->dim; ++a) {
+      const int N = c->n[a], sh = N / 2;
+      for (int j = 0; j < N; ++j) {
+        const double k = c->dk[a] * (double)(((j + sh) % N) - sh);
+        const double phi = mdt * ((k * c->alpha[a]) * k);
+        const double ph = (f == 0) ? 0.5 * phi : phi;
+        double2 e = make_double2(cos(ph), sin(ph));
+        if (a == 0) e = make_double2(e.x * scale, e.y * scale);
+        h[((size_t)f * 3 + a) * nm + j] = e;
+      }
+    }
+  }
+  HIP_TRY(hipMemcpyAsync(c->kin, h.data(), sizeof(double2) * h.size(), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->kin_dt = dt;
+  c->kin_valid = true;
+  return ATUS_OK;
+}
+
+KinTabs kin_tabs(atus_ctx* c, int f) {
+  KinTabs kt{};
+  const int nm = c->kin_nmax;
+  kt.e0 = c->kin + ((size_t)f * 3 + 0) * nm;
+  kt.e1 = c->dim > 1 ? c->kin + ((size_t)f * 3 + 1) * nm : nullptr;
+  kt.e2 = c->dim > 2 ? c->kin + ((size_t)f * 3 + 2) * nm : nullptr;
+  kt.n2 = c->dim > 2 ? c->n[2] : 1;
+  return kt;
+}
+
+int ensure_scratch(atus_ctx* c) {
+  if (c->scratch) return ATUS_OK;
+  if (hipMalloc(&c->scratch, sizeof(double2) * c->npts) != hipSuccess)
+    return fail(ATUS_ERR_NOMEM, "scratch allocation");
+  return ATUS_OK;
+}
+
+// one pass along axis a of `ncomp` components starting at src/dst
+int pass(atus_ctx* c, int a, int dir, const double2* src, double2* dst, int ncomp, double scale) {
+  const int N = c->n[a];
+  ATUS_TRY(ensure_twiddles(c, N));
+  const long long inner = c->inner_of(a), outer = c->outer_of(a);
+  ProfScope ps(c, inner == 1 ? ATUS_KFAM_PASS_CONTIG : ATUS_KFAM_PASS_STRIDED);
+  HIP_TRY(launch_pass(N, inner == 1, dir, src, dst, c->npts, ncomp, inner, outer, c->tw[N], scale, c->max_lds,
+                      c->stream));
+  return ATUS_OK;
+}
+
+// full forward (dir -1) or backward (+1) transform with the cft scale,
+// src -> dst (dst may equal src); axes from fastest to slowest
+int full_ft(atus_ctx* c, int dir, const double2* src, double2* dst, int ncomp) {
+  const double scale = ft_factor(c->dim, dir < 0 ? c->dx : c->dk);
+  const double2* s = src;
+  for (int a = c->dim - 1; a >= 0; --a) {
+    ATUS_TRY(pass(c, a, dir, s, dst, ncomp, a == 0 ? scale : 1.0));
+    s = dst;
+  }
+  return ATUS_OK;
+}
+
+// kinetic step on all components with table slot f (0 half, 1 full)
+int kinetic_unfused(atus_ctx* c, int f) {
+  // forward along the fast axes, fused phase along axis 0, inverse back
+  for (int a = c->dim - 1; a >= 1; --a) ATUS_TRY(pass(c, a, -1, c->psi, c->psi, c->ncomp, 1.0));
+  const int N0 = c->n[0];
+  ATUS_TRY(ensure_twiddles(c, N0));
+  VArgs va{};
+  {
+    ProfScope ps(c, ATUS_KFAM_KIN);
+    HIP_TRY(launch_kin(N0, c->dim == 1, V_NONE, 1, c->psi, c->npts, c->ncomp, c->inner_of(0), c->tw[N0],
+                       kin_tabs(c, f), va, c->dx[0], c->max_lds, c->stream));
+  }
+  for (int a = 1; a < c->dim; ++a) ATUS_TRY(pass(c, a, +1, c->psi, c->psi, c->ncomp, 1.0));
+  return ATUS_OK;
+}
+
+template <class Fn>
+int reduce(atus_ctx* c, const Fn& fn, long long n, double* out) {
+  constexpr int NV = Fn::NV;
+  const unsigned nb = red_blocks(n);
+  hipLaunchKernelGGL((k_reduce<Fn>), dim3(nb), dim3(RED_THREADS), 0, c->stream, fn, n, c->red_part);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL((k_reduce_final<NV>), dim3(1), dim3(RED_THREADS), 0, c->stream, c->red_part, (int)nb,
+                     c->red_out);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(c->red_host, c->red_out, sizeof(double) * NV, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  for (int v = 0; v < NV; ++v) out[v] = c->red_host[v];
+  return ATUS_OK;
+}
+
+int vkind_of(int kind) {
+  switch (kind) {
+    case ATUS_STEP_FREEPROP: return V_NL;
+    case ATUS_STEP_FREEPROP_IF: return V_NL_IF;
+    case ATUS_STEP_BRAGG_AD: return V_BRAGG;
+    default: return -1;
+  }
+}
+
+void fill_vargs(atus_ctx* c, VArgs& va, double dt, const atus_laser* L) {
+  va = VArgs{};
+  va.dt = dt;
+  for (int i = 0; i < 16; ++i) va.gs[i] = c->gs[i];
+  va.pot = c->pot;
+  va.pot_stride = c->npts;
+  if (L) {
+    va.beta0 = L->beta0;
+    for (int i = 0; i < 4; ++i) va.deltaL[i] = L->deltaL[i];
+    va.amp0 = L->amp0;
+    va.laser_k = L->laser_k;
+    va.laser_domh = L->laser_domh;
+    va.laser_dk = L->laser_dk;
+    va.chirp = L->chirp;
+    va.chirp_rate0 = L->chirp_rate0;
+    va.phase0 = L->phase0;
+  }
+  va.F = 1.0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* atus_last_error(void) { return g_err.c_str(); }
+
+const char* atus_version(void) { return "atus_hip 0.1 (gfx950, complex128)"; }
+
+int atus_device_info(char* name, int len, int* cu_count, double* hbm_gbps_peak) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  hipDeviceProp_t p;
+  HIP_TRY(hipGetDeviceProperties(&p, dev));
+  if (name && len > 0) {
+    snprintf(name, (size_t)len, "%s (%s)", p.name, p.gcnArchName);
+  }
+  if (cu_count) *cu_count = p.multiProcessorCount;
+  if (hbm_gbps_peak) *hbm_gbps_peak = 2.0 * (double)p.memoryClockRate * 1e3 * (p.memoryBusWidth / 8) / 1e9;
+  return ATUS_OK;
+}
+
+int atus_ctx_create(atus_ctx** out, int dim, const long long n[3], const double dx[3], const double dk[3],
+                    int ncomp) {
+  if (!out) return fail(ATUS_ERR_ARG, "null output pointer");
+  *out = nullptr;
+  if (dim < 1 || dim > 3) return fail(ATUS_ERR_ARG, "dim must be 1, 2 or 3");
+  if (ncomp < 1 || ncomp > 4) return fail(ATUS_ERR_ARG, "ncomp must be 1..4");
+  for (int a = 0; a < 3; ++a) {
+    if (a < dim) {
+      if (!supported_len((int)n[a]) || n[a] > 4096)
+        return fail(ATUS_ERR_UNSUPPORTED, "axis " + std::to_string(a) + " length " + std::to_string(n[a]) +
+                                              " is not a power of two in [2, 4096]");
+    } else if (n[a] != 1) {
+      return fail(ATUS_ERR_ARG, "unused axes must have length 1");
+    }
+  }
+  atus_ctx* c = new atus_ctx();
+  c->dim = dim;
+  c->ncomp = ncomp;
+  c->npts = 1;
+  for (int a = 0; a < 3; ++a) {
+    c->n[a] = (int)n[a];
+    c->dx[a] = dx[a];
+    c->dk[a] = dk[a];
+    c->npts *= n[a];
+    c->kin_nmax = std::max(c->kin_nmax, (int)n[a]);
+  }
+  auto bail = [&](int code, const std::string& m) {
+    atus_ctx_destroy(c);
+    return fail(code, m);
+  };
+  if (hipGetDevice(&c->device) != hipSuccess) return bail(ATUS_ERR_HIP, "no HIP device");
+  int lds = 0;
+  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) == hipSuccess && lds > 0)
+    c->max_lds = std::min<size_t>((size_t)lds, 65536);
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(ATUS_ERR_HIP, "stream creation");
+  if (hipMalloc(&c->psi, sizeof(double2) * c->npts * ncomp) != hipSuccess)
+    return bail(ATUS_ERR_NOMEM, "wavefunction allocation (" + std::to_string(sizeof(double2) * c->npts * ncomp) +
+                                    " bytes)");
+  if (hipMemset(c->psi, 0, sizeof(double2) * c->npts * ncomp) != hipSuccess) return bail(ATUS_ERR_HIP, "memset");
+  if (hipMalloc(&c->kin, sizeof(double2) * 2 * 3 * c->kin_nmax) != hipSuccess) return bail(ATUS_ERR_NOMEM, "tables");
+  if (hipMalloc(&c->red_part, sizeof(double) * RED_MAX_BLOCKS * RED_MAX_NV) != hipSuccess ||
+      hipMalloc(&c->red_out, sizeof(double) * RED_MAX_NV) != hipSuccess ||
+      hipHostMalloc(&c->red_host, sizeof(double) * RED_MAX_NV, hipHostMallocDefault) != hipSuccess)
+    return bail(ATUS_ERR_NOMEM, "reduction buffers");
+  if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
+    return bail(ATUS_ERR_HIP, "events");
+  for (int a = 0; a < dim; ++a)
+    if (ensure_twiddles(c, c->n[a]) != ATUS_OK) {
+      std::string m = g_err;
+      return bail(ATUS_ERR_NOMEM, m);
+    }
+  *out = c;
+  return ATUS_OK;
+}
+
+int atus_ctx_destroy(atus_ctx* c) {
+  if (!c) return ATUS_OK;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  (void)hipFree(c->psi);
+  (void)hipFree(c->pot);
+  (void)hipFree(c->scratch);
+  for (auto& kv : c->tw) (void)hipFree(kv.second);
+  (void)hipFree(c->kin);
+  (void)hipFree(c->red_part);
+  (void)hipFree(c->red_out);
+  if (c->red_host) (void)hipHostFree(c->red_host);
+  (void)hipFree(c->sob);
+  (void)hipFree(c->lap);
+  (void)hipFree(c->grad);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return ATUS_OK;
+}
+
+int atus_synchronize(atus_ctx* c) {
+  ATUS_TRY(check_ctx(c));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return ATUS_OK;
+}
+
+int atus_upload(atus_ctx* c, int comp, const double* host) {
+  ATUS_TRY(check_ctx(c));
+  ATUS_TRY(check_comp(c, comp));
+  if (!host) return fail(ATUS_ERR_ARG, "null host pointer");
+  HIP_TRY(hipMemcpyAsync(c->psi + comp * c->npts, host, sizeof(double2) * c->npts, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return ATUS_OK;
+}
+
+int atus_download(atus_ctx* c, int comp, double* host) {
+  ATUS_TRY(check_ctx(c));
+  ATUS_TRY(check_comp(c, comp));
+  if (!host) return fail(ATUS_ERR_ARG, "null host pointer");
+  HIP_TRY(hipMemcpyAsync(host, c->psi + comp * c->npts, sizeof(double2) * c->npts, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return ATUS_OK;
+}
+
+int atus_set_potential(atus_ctx* c, int comp, const double* host) {
+  ATUS_TRY(check_ctx(c));
+  if (!host) {
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    (void)hipFree(c->pot);
+    c->pot = nullptr;
+    return ATUS_OK;
+  }
+  ATUS_TRY(check_comp(c, comp));
+  if (!c->pot) {
+    if (hipMalloc(&c->pot, sizeof(double) * c->npts * c->ncomp) != hipSuccess)
+      return fail(ATUS_ERR_NOMEM, "potential allocation");
+    HIP_TRY(hipMemsetAsync(c->pot, 0, sizeof(double) * c->npts * c->ncomp, c->stream));
+  }
+  HIP_TRY(hipMemcpyAsync(c->pot + comp * c->npts, host, sizeof(double) * c->npts, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return ATUS_OK;
+}
+
+int atus_set_g(atus_ctx* c, const double* gs) {
+  ATUS_TRY(check_ctx(c));
+  if (!gs) return fail(ATUS_ERR_ARG, "null g");
+  for (int i = 0; i < 16; ++i) c->gs[i] = 0;
+  for (int i = 0; i < c->ncomp * c->ncomp; ++i) c->gs[i] = gs[i];
+  return ATUS_OK;
+}
+
+int atus_set_alpha(atus_ctx* c, const double alpha[3]) {
+  ATUS_TRY(check_ctx(c));
+  for (int a = 0; a < 3; ++a) c->alpha[a] = a < c->dim ? alpha[a] : 0.0;
+  c->kin_valid = false;
+  return ATUS_OK;
+}
+
+int atus_ft(atus_ctx* c, int comp, int isign) {
+  ATUS_TRY(check_ctx(c));
+  ATUS_TRY(check_comp(c, comp));
+  if (isign != 1 && isign != -1) return ATUS_OK;
+  double2* p = c->psi + comp * c->npts;
+  ATUS_TRY(full_ft(c, isign, p, p, 1));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return ATUS_OK;
+}
+
+int atus_kinetic(atus_ctx* c, double frac, double dt) {
+  ATUS_TRY(check_ctx(c));
+  if (frac != 0.5 && frac != 1.0) return fail(ATUS_ERR_ARG, "frac must be 0.5 or 1");
+  ATUS_TRY(ensure_kin(c, dt));
+  ATUS_TRY(kinetic_unfused(c, frac == 0.5 ? 0 : 1));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return ATUS_OK;
+}
+
+int atus_nl_step(atus_ctx* c, double dt) {
+  ATUS_TRY(check_ctx(c));
+  VArgs va;
+  fill_vargs(c, va, dt, nullptr);
+  HIP_TRY(launch_point(V_NL, c->ncomp, c->psi, c->grid(), va, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return ATUS_OK;
+}
+
+int atus_nl_if_step(atus_ctx* c, double dt, double beta0, const double* deltaL) {
+  ATUS_TRY(check_ctx(c));
+  atus_laser L{};
+  L.beta0 = beta0;
+  for (int i = 0; i < c->ncomp; ++i) L.deltaL[i] = deltaL ? deltaL[i] : 0.0;
+  VArgs va;
+  fill_vargs(c, va, dt, &L);
+  HIP_TRY(launch_point(V_NL_IF, c->ncomp, c->psi, c->grid(), va, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return ATUS_OK;
+}
+
+int atus_bragg_ad(atus_ctx* c, double dt, double t, double F, const atus_laser* laser) {
+  ATUS_TRY(check_ctx(c));
+  if (c->ncomp != 2) return fail(ATUS_ERR_STATE, "bragg_ad needs exactly two components");
+  if (!laser) return fail(ATUS_ERR_ARG, "null laser parameters");
+  VArgs va;
+  fill_vargs(c, va, dt, laser);
+  va.t1 = t + 0.5 * dt;
+  va.F = F;
+  HIP_TRY(launch_point(V_BRAGG, 2, c->psi, c->grid(), va, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return ATUS_OK;
+}
+
+int atus_set_momentum(atus_ctx* c, int comp, const double p[3]) {
+  ATUS_TRY(check_ctx(c));
+  ATUS_TRY(check_comp(c, comp));
+  hipLaunchKernelGGL(k_momentum, dim3(ew_blocks(c->npts)), dim3(256), 0, c->stream, c->psi + comp * c->npts, c->grid(),
+                     p[0], c->dim > 1 ? p[1] : 0.0, c->dim > 2 ? p[2] : 0.0);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return ATUS_OK;
+}
+
+int atus_run_block(atus_ctx* c, const atus_step_params* sp, int Nk, double* t) {
+  ATUS_TRY(check_ctx(c));
+  if (!sp || !t) return fail(ATUS_ERR_ARG, "null argument");
+  if (Nk < 1) return fail(ATUS_ERR_ARG, "Nk must be >= 1");
+  const int vk = vkind_of(sp->kind);
+  if (vk < 0) return fail(ATUS_ERR_ARG, "unknown step kind " + std::to_string(sp->kind));
+  if (vk == V_BRAGG && c->ncomp != 2) return fail(ATUS_ERR_STATE, "bragg_ad needs exactly two components");
+  const double dt = sp->dt;
+  ATUS_TRY(ensure_kin(c, dt));
+  VArgs base;
+  fill_vargs(c, base, dt, &sp->laser);
+  const bool fused = c->ncomp <= 2;
+  const int N0 = c->n[0], NL = c->n[c->dim - 1];
+  const long long inner0 = c->inner_of(0);
+  if (c->timing) HIP_TRY(hipEventRecord(c->ev0, c->stream));
+
+  double tt = *t;
+  // half kinetic step
+  if (fused && c->dim >= 2) {
+    ATUS_TRY(pass(c, c->dim - 1, -1, c->psi, c->psi, c->ncomp, 1.0));
+    if (c->dim == 3) ATUS_TRY(pass(c, 1, -1, c->psi, c->psi, c->ncomp, 1.0));
+    {
+      ProfScope ps(c, ATUS_KFAM_KIN);
+      HIP_TRY(launch_kin(N0, false, V_NONE, 1, c->psi, c->npts, c->ncomp, inner0, c->tw[N0], kin_tabs(c, 0), base,
+                         c->dx[0], c->max_lds, c->stream));
+    }
+    if (c->dim == 3) ATUS_TRY(pass(c, 1, +1, c->psi, c->psi, c->ncomp, 1.0));
+  } else {
+    ATUS_TRY(kinetic_unfused(c, 0));
+  }
+  tt += 0.5 * dt;
+  for (int m = 1; m <= Nk; ++m) {
+    VArgs va = base;
+    va.t1 = tt + 0.5 * dt;
+    va.F = sp->amp ? sp->amp(va.t1, sp->amp_user) : 1.0;
+    if (!std::isfinite(va.F)) return fail(ATUS_ERR_NUMERIC, "amplitude callback returned a non-finite value");
+    const int f = (m < Nk) ? 1 : 0;
+    if (!fused) {
+      {
+        ProfScope ps(c, ATUS_KFAM_POINT);
+        HIP_TRY(launch_point(vk, c->ncomp, c->psi, c->grid(), va, c->stream));
+      }
+      ATUS_TRY(kinetic_unfused(c, f));
+    } else if (c->dim == 1) {
+      ProfScope ps(c, ATUS_KFAM_KIN);
+      HIP_TRY(launch_kin(N0, true, vk, c->ncomp, c->psi, c->npts, c->ncomp, 1, c->tw[N0], kin_tabs(c, f), va,
+                         c->dx[0], c->max_lds, c->stream));
+    } else {
+      const long long outer = c->npts / NL;
+      const long long ppx = c->dim == 3 ? c->n[1] : 1;
+      {
+        ProfScope ps(c, ATUS_KFAM_VFUSE);
+        HIP_TRY(launch_vfuse(NL, vk, c->ncomp, c->psi, c->npts, outer, c->tw[NL], va, ppx, c->n[0], c->dx[0],
+                             c->max_lds, c->stream));
+      }
+      if (c->dim == 3) ATUS_TRY(pass(c, 1, -1, c->psi, c->psi, c->ncomp, 1.0));
+      {
+        ProfScope ps(c, ATUS_KFAM_KIN);
+        HIP_TRY(launch_kin(N0, false, V_NONE, 1, c->psi, c->npts, c->ncomp, inner0, c->tw[N0], kin_tabs(c, f), va,
+                           c->dx[0], c->max_lds, c->stream));
+      }
+      if (c->dim == 3) ATUS_TRY(pass(c, 1, +1, c->psi, c->psi, c->ncomp, 1.0));
+    }
+    tt += (m < Nk) ? dt : 0.5 * dt;
+  }
+  if (fused && c->dim >= 2) ATUS_TRY(pass(c, c->dim - 1, +1, c->psi, c->psi, c->ncomp, 1.0));
+  if (c->timing) {
+    HIP_TRY(hipEventRecord(c->ev1, c->stream));
+    HIP_TRY(hipEventSynchronize(c->ev1));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+  } else {
+    HIP_TRY(hipStreamSynchronize(c->stream));
+  }
+  *t = tt;
+  return ATUS_OK;
+}
+
+int atus_particle_number(atus_ctx* c, int comp, double* out) {
+  ATUS_TRY(check_ctx(c));
+  ATUS_TRY(check_comp(c, comp));
+  if (!out) return fail(ATUS_ERR_ARG, "null output");
+  RedNorm fn{c->psi + comp * c->npts};
+  double s;
+  ATUS_TRY(reduce(c, fn, c->npts, &s));
+  *out = c->ar() * s;
+  if (!std::isfinite(*out)) return fail(ATUS_ERR_NUMERIC, "particle number is not finite");
+  return ATUS_OK;
+}
+
+int atus_expval_x(atus_ctx* c, int comp, double out[3]) {
+  ATUS_TRY(check_ctx(c));
+  ATUS_TRY(check_comp(c, comp));
+  RedX fn{c->psi + comp * c->npts, c->grid()};
+  double s[3];
+  ATUS_TRY(reduce(c, fn, c->npts, s));
+  for (int a = 0; a < 3; ++a) out[a] = a < c->dim ? c->ar() * s[a] : 0.0;
+  return ATUS_OK;
+}
+
+int atus_expval_k(atus_ctx* c, int comp, double out[3]) {
+  ATUS_TRY(check_ctx(c));
+  ATUS_TRY(check_comp(c, comp));
+  ATUS_TRY(ensure_scratch(c));
+  ATUS_TRY(full_ft(c, -1, c->psi + comp * c->npts, c->scratch, 1));
+  RedK fn{c->scratch, c->grid()};
+  double s[3];
+  ATUS_TRY(reduce(c, fn, c->npts, s));
+  for (int a = 0; a < 3; ++a) out[a] = a < c->dim ? c->ar_k() * s[a] : 0.0;
+  return ATUS_OK;
+}
+
+int atus_rabi_integrals(atus_ctx* c, int comp, const double* k0, int n, double threshold, double* out) {
+  ATUS_TRY(check_ctx(c));
+  ATUS_TRY(check_comp(c, comp));
+  if (n < 1 || n > RED_MAX_NV || !k0 || !out) return fail(ATUS_ERR_ARG, "need 1..8 momentum points");
+  ATUS_TRY(ensure_scratch(c));
+  ATUS_TRY(full_ft(c, -1, c->psi + comp * c->npts, c->scratch, 1));
+  RedRabi fn{};
+  fn.psik = c->scratch;
+  fn.gr = c->grid();
+  fn.nk0 = n;
+  for (int q = 0; q < n; ++q)
+    for (int a = 0; a < 3; ++a) fn.k0[q][a] = k0[3 * q + a];
+  fn.thr = threshold;
+  double s[RED_MAX_NV];
+  ATUS_TRY(reduce(c, fn, c->npts, s));
+  for (int q = 0; q < n; ++q) out[q] = c->ar_k() * s[q];
+  return ATUS_OK;
+}
+
+int atus_energy(atus_ctx* c, double* out) {
+  ATUS_TRY(check_ctx(c));
+  if (!out) return fail(ATUS_ERR_ARG, "null output");
+  ATUS_TRY(ensure_scratch(c));
+  double ekin = 0;
+  for (int i = 0; i < c->ncomp; ++i) {
+    ATUS_TRY(full_ft(c, -1, c->psi + i * c->npts, c->scratch, 1));
+    RedEkin fn{};
+    fn.psik = c->scratch;
+    fn.gr = c->grid();
+    for (int a = 0; a < 3; ++a) fn.alpha[a] = c->alpha[a];
+    double s;
+    ATUS_TRY(reduce(c, fn, c->npts, &s));
+    ekin += s;
+  }
+  RedEpot fp{};
+  fp.psi = c->psi;
+  fp.pot = c->pot;
+  fp.npts = c->npts;
+  fp.nc = c->ncomp;
+  for (int i = 0; i < 16; ++i) fp.gs[i] = c->gs[i];
+  double ep;
+  ATUS_TRY(reduce(c, fp, c->npts, &ep));
+  *out = c->ar_k() * ekin + c->ar() * ep;
+  return ATUS_OK;
+}
+
+// ------------------------------------------------------------------ Sobolev
+
+namespace {
+
+// out_sob = F^-1[op F psi], out_lap = F^-1[-alpha k^2 F psi] (optional)
+int sob_filter(atus_ctx* c, int comp, const double2* src, double2* out_sob, double2* out_lap) {
+  const long long off = comp * c->npts;
+  SobArgs sa{};
+  sa.alpha = c->sob_alpha[comp];
+  sa.dim = c->dim;
+  for (int a = 0; a < 3; ++a) {
+    sa.dk[a] = c->dk[a];
+    sa.n[a] = c->n[a];
+  }
+  sa.scale = ft_factor(c->dim, c->dx) * ft_factor(c->dim, c->dk);
+  const double2* s = src + off;
+  double2* w = out_sob + off;
+  for (int a = c->dim - 1; a >= 1; --a) {
+    ATUS_TRY(pass(c, a, -1, s, w, 1, 1.0));
+    s = w;
+  }
+  const int N0 = c->n[0];
+  ATUS_TRY(ensure_twiddles(c, N0));
+  HIP_TRY(launch_sobf(N0, c->dim == 1, s, w, out_lap ? out_lap + off : nullptr, c->npts, 1, c->inner_of(0),
+                      c->tw[N0], sa, c->max_lds, c->stream));
+  for (int a = 1; a < c->dim; ++a) {
+    ATUS_TRY(pass(c, a, +1, w, w, 1, 1.0));
+    if (out_lap) ATUS_TRY(pass(c, a, +1, out_lap + off, out_lap + off, 1, 1.0));
+  }
+  return ATUS_OK;
+}
+
+struct RedMu {
+  static constexpr int NV = 1;
+  const double2* psi;
+  const double2* lap;
+  const double* pot;
+  long long npts;
+  int nw, comp;
+  double g0, g1;
+  __device__ void operator()(long long l, double (&acc)[1]) const {
+    double nl = 0.0;
+    for (int j = 0; j < nw; ++j) {
+      const double2 a = psi[j * npts + l];
+      nl += (j == 0 ? g0 : g1) * (a.x * a.x + a.y * a.y);
+    }
+    const double2 p = psi[comp * npts + l], lp = lap[comp * npts + l];
+    acc[0] += -(p.x * lp.x + p.y * lp.y) + (pot[comp * npts + l] + nl) * (p.x * p.x + p.y * p.y);
+  }
+};
+
+}  // namespace
+
+int atus_sob_setup(atus_ctx* c, const double* alpha_iso, double stepsize) {
+  ATUS_TRY(check_ctx(c));
+  if (c->ncomp > 2) return fail(ATUS_ERR_UNSUPPORTED, "Sobolev minimiser supports 1 or 2 wavefunctions");
+  if (!alpha_iso) return fail(ATUS_ERR_ARG, "null alpha");
+  if (!c->pot) return fail(ATUS_ERR_STATE, "set the potential before atus_sob_setup");
+  for (int i = 0; i < c->ncomp; ++i) c->sob_alpha[i] = alpha_iso[i];
+  c->sob_step = stepsize;
+  const size_t bytes = sizeof(double2) * c->npts * c->ncomp;
+  if (!c->sob && hipMalloc(&c->sob, bytes) != hipSuccess) return fail(ATUS_ERR_NOMEM, "Sobolev buffers");
+  if (!c->lap && hipMalloc(&c->lap, bytes) != hipSuccess) return fail(ATUS_ERR_NOMEM, "Sobolev buffers");
+  if (!c->grad && hipMalloc(&c->grad, bytes) != hipSuccess) return fail(ATUS_ERR_NOMEM, "Sobolev buffers");
+  c->sob_ready = true;
+  return ATUS_OK;
+}
+
+int atus_sob_renormalize(atus_ctx* c) {
+  ATUS_TRY(check_ctx(c));
+  std::vector<double> N(c->ncomp);
+  for (int i = 0; i < c->ncomp; ++i) ATUS_TRY(atus_particle_number(c, i, &N[i]));
+  for (int i = 0; i < c->ncomp; ++i) {
+    hipLaunchKernelGGL(k_scale, dim3(ew_blocks(c->npts)), dim3(256), 0, c->stream, c->psi + i * c->npts, c->npts,
+                       1.0 / sqrt(N[i]));
+    HIP_TRY(hipGetLastError());
+  }
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return ATUS_OK;
+}
+
+int atus_sob_iter(atus_ctx* c, double* res_tot, int* renormalised) {
+  ATUS_TRY(check_ctx(c));
+  if (!c->sob_ready) return fail(ATUS_ERR_STATE, "call atus_sob_setup first");
+  const int nw = c->ncomp;
+  const long long np = c->npts;
+  // Compute_Sobolev_Psi and Compute_Laplace share the forward transform
+  for (int i = 0; i < nw; ++i) ATUS_TRY(sob_filter(c, i, c->psi, c->sob, c->lap));
+  // Compute_Sobolev_Gradient
+  for (int i = 0; i < nw; ++i) {
+    hipLaunchKernelGGL(k_sob_grad_in, dim3(ew_blocks(np)), dim3(256), 0, c->stream, c->psi, c->lap, c->pot, c->grad,
+                       np, nw, i, c->gs[0 + i * nw], nw > 1 ? c->gs[1 + i * nw] : 0.0);
+    HIP_TRY(hipGetLastError());
+    ATUS_TRY(sob_filter(c, i, c->grad, c->grad, nullptr));
+  }
+  // Project_Sobolev_Gradient + Compute_res + particle numbers
+  double res_sq = 0;
+  std::vector<double> N(nw);
+  for (int i = 0; i < nw; ++i) {
+    RedProj fp{c->psi + i * np, c->grad + i * np, c->sob + i * np};
+    double pr[2];
+    ATUS_TRY(reduce(c, fp, np, pr));
+    const double fak = pr[0] / pr[1];
+    const unsigned nb = red_blocks(np);
+    hipLaunchKernelGGL(k_sob_project_res, dim3(nb), dim3(RED_THREADS), 0, c->stream, c->grad + i * np,
+                       c->sob + i * np, c->psi + i * np, np, fak, c->red_part);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL((k_reduce_final<2>), dim3(1), dim3(RED_THREADS), 0, c->stream, c->red_part, (int)nb, c->red_out);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(c->red_host, c->red_out, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    const double res_i = c->red_host[0] * c->ar();
+    N[i] = c->ar() * c->red_host[1];
+    res_sq += res_i * res_i;
+  }
+  bool brenorm = false;
+  for (int i = 0; i < nw; ++i)
+    if (fabs(N[i] - 1) > 1e-5) brenorm = true;
+  for (int i = 0; i < nw; ++i) {
+    const double f = brenorm ? 1.0 / sqrt(N[i]) : 1.0;
+    hipLaunchKernelGGL(k_sob_update, dim3(ew_blocks(np)), dim3(256), 0, c->stream, c->psi + i * np, c->grad + i * np,
+                       np, f, c->sob_step);
+    HIP_TRY(hipGetLastError());
+  }
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  const double rt = sqrt(res_sq);
+  if (res_tot) *res_tot = rt;
+  if (renormalised) *renormalised = brenorm ? 1 : 0;
+  if (!std::isfinite(rt)) return fail(ATUS_ERR_NUMERIC, "Sobolev residual is not finite");
+  return ATUS_OK;
+}
+
+int atus_sob_mu(atus_ctx* c, double* mu) {
+  ATUS_TRY(check_ctx(c));
+  if (!c->sob_ready) return fail(ATUS_ERR_STATE, "call atus_sob_setup first");
+  if (!mu) return fail(ATUS_ERR_ARG, "null output");
+  const int nw = c->ncomp;
+  for (int i = 0; i < nw; ++i) ATUS_TRY(sob_filter(c, i, c->psi, c->sob, c->lap));
+  for (int i = 0; i < nw; ++i) {
+    RedMu fn{c->psi, c->lap, c->pot, c->npts, nw, i, c->gs[0 + i * nw], nw > 1 ? c->gs[1 + i * nw] : 0.0};
+    double s;
+    ATUS_TRY(reduce(c, fn, c->npts, &s));
+    mu[i] = c->ar() * s;
+  }
+  return ATUS_OK;
+}
+
+int atus_run_block_profiled(atus_ctx* c, const atus_step_params* sp, int Nk, double* t, double* ms, int* count) {
+  ATUS_TRY(check_ctx(c));
+  if (!ms || !count) return fail(ATUS_ERR_ARG, "null output");
+  for (int f = 0; f < ATUS_NKFAM; ++f) {
+    ms[f] = 0;
+    count[f] = 0;
+  }
+  c->prof = true;
+  c->prof_ev.clear();
+  c->prof_fam.clear();
+  int rc = atus_run_block(c, sp, Nk, t);
+  c->prof = false;
+  if (rc == ATUS_OK) {
+    if (hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(ATUS_ERR_HIP, "synchronize after profiled block");
+  }
+  for (size_t i = 0; rc == ATUS_OK && i + 1 < c->prof_ev.size() && i / 2 < c->prof_fam.size(); i += 2) {
+    float e = 0;
+    if (hipEventElapsedTime(&e, c->prof_ev[i], c->prof_ev[i + 1]) != hipSuccess) {
+      rc = fail(ATUS_ERR_HIP, "event timing");
+      break;
+    }
+    const int f = c->prof_fam[i / 2];
+    ms[f] += e;
+    count[f] += 1;
+  }
+  for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
+  c->prof_ev.clear();
+  c->prof_fam.clear();
+  return rc;
+}
+
+int atus_set_timing(atus_ctx* c, int enable) {
+  ATUS_TRY(check_ctx(c));
+  c->timing = enable != 0;
+  return ATUS_OK;
+}
+
+int atus_last_block_ms(atus_ctx* c, double* ms) {
+  ATUS_TRY(check_ctx(c));
+  if (!ms) return fail(ATUS_ERR_ARG, "null output");
+  *ms = c->timing ? c->last_ms : 0.0;
+  return ATUS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,48 @@
+// k_sob.hip -- Sobolev filter passes along axis 0 and the pointwise V launcher.
+#include "launch.hpp"
+
+namespace atus {
+
+template <int N>
+static hipError_t sob_n(bool contig, const double2* src, double2* o1, double2* o2, long long cs, int ncomp,
+                        long long inner, const double2* tw, const SobArgs& sa, size_t max_lds, hipStream_t s) {
+  PassCfg c = make_pass_cfg(N, contig ? 1 : inner, 1, ncomp, max_lds);
+  if (contig)
+    hipLaunchKernelGGL((k_sobf<N, true>), c.grid, c.block, c.lds, s, src, o1, o2, cs, c.g, tw, sa);
+  else
+    hipLaunchKernelGGL((k_sobf<N, false>), c.grid, c.block, c.lds, s, src, o1, o2, cs, c.g, tw, sa);
+  return hipGetLastError();
+}
+
+hipError_t launch_sobf(int N, bool contig, const double2* src, double2* out_sob, double2* out_lap,
+                       long long comp_stride, int ncomp, long long inner, const double2* tw, const SobArgs& sa,
+                       size_t max_lds, hipStream_t s) {
+  switch (N) {
+#define ATUS_CASE(n) \
+  case n: return sob_n<n>(contig, src, out_sob, out_lap, comp_stride, ncomp, inner, tw, sa, max_lds, s);
+    ATUS_CASE(2) ATUS_CASE(4) ATUS_CASE(8) ATUS_CASE(16) ATUS_CASE(32) ATUS_CASE(64) ATUS_CASE(128)
+    ATUS_CASE(256) ATUS_CASE(512) ATUS_CASE(1024) ATUS_CASE(2048) ATUS_CASE(4096)
+#undef ATUS_CASE
+    default: return hipErrorInvalidValue;
+  }
+}
+
+static unsigned point_blocks(long long n) {
+  long long b = (n + 255) / 256;
+  if (b > 8192) b = 8192;
+  if (b < 1) b = 1;
+  return (unsigned)b;
+}
+
+hipError_t launch_point(int vk, int nc, double2* psi, const Grid& gr, const VArgs& va, hipStream_t s) {
+  dim3 grid(point_blocks(gr.npts)), block(256);
+#define ATUS_PT(V, C) \
+  if (vk == V && nc == C) { hipLaunchKernelGGL((k_point<V, C>), grid, block, 0, s, psi, gr, va); return hipGetLastError(); }
+  ATUS_PT(V_NL, 1) ATUS_PT(V_NL, 2) ATUS_PT(V_NL, 3) ATUS_PT(V_NL, 4)
+  ATUS_PT(V_NL_IF, 1) ATUS_PT(V_NL_IF, 2) ATUS_PT(V_NL_IF, 3) ATUS_PT(V_NL_IF, 4)
+  ATUS_PT(V_BRAGG, 2)
+#undef ATUS_PT
+  return hipErrorInvalidValue;
+}
+
+}  // namespace atus

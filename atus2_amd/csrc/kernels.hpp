@@ -1,0 +1,631 @@
+// kernels.hpp -- the HIP kernels of the split-step / Sobolev path.
+//
+// Data: complex128 (double2), row-major [n0][n1][n2], axis 2 fastest;
+// components are npts apart in one allocation.  A transform along axis a
+// sees the array as [outer][N][inner] (inner = product of the faster axes).
+//
+// Kernels (all launched with blockDim.x = T * P, P = N / R threads per pencil):
+//   k_pass   : one unnormalised DFT direction along one axis, times a scale
+//   k_kin    : forward DFT along axis 0, multiply by the kinetic phase, inverse
+//              DFT along axis 0 (the whole k-space step of a 1D grid; the
+//              innermost step of 2D/3D grids); optionally preceded by the
+//              pointwise potential / Bragg step (1D only, all components
+//              in one thread)
+//   k_vfuse  : inverse DFT along the last (contiguous) axis, pointwise
+//              potential / Bragg step, forward DFT along the same axis
+//   k_sobf   : forward DFT along axis 0, multiply by 1/(1+alpha k^2) and
+//              -alpha k^2, two inverse DFTs (Sobolev psi and Laplacian)
+//   k_point  : standalone pointwise steps
+//   k_reduce : deterministic two-stage reductions
+#pragma once
+#include "fft_core.hpp"
+
+namespace atus {
+
+enum VKind { V_NONE = 0, V_NL = 1, V_NL_IF = 2, V_BRAGG = 3 };
+
+struct Grid {
+  int dim;
+  int n[3];
+  double dx[3];
+  double dk[3];
+  long long npts;
+};
+
+// geometry of one axis pass
+struct AxisGeo {
+  long long inner;      // element stride of the transformed axis
+  long long outer;      // number of [N][inner] slabs
+  long long col_tiles;  // strided: inner / T tiles per slab
+  long long ntiles;     // gridDim.x
+  int T;                // pencils per workgroup
+};
+
+// parameters of the pointwise step V (CRT_Base::Do_NL_Step,
+// CRT_Base_IF::Do_NL_Step, Bragg_single::Do_Bragg_ad)
+struct VArgs {
+  double dt;
+  double gs[16];            // row-major ncomp x ncomp
+  const double* pot;        // NULL: potential not initialised
+  long long pot_stride;     // npts
+  double beta0;
+  double deltaL[4];
+  // Bragg
+  double t1;                // Get_t() + 0.5*dt
+  double F;                 // Amplitude_at_time()
+  double amp0, laser_k, laser_domh, laser_dk, chirp, chirp_rate0, phase0;
+};
+
+// blockIdx.x -> tile so that the workgroups one XCD runs (round-robin
+// dispatch: XCD = blockIdx.x % 8) cover one contiguous range of tiles
+__device__ __forceinline__ long long xcd_tile(long long b, long long nb) {
+  const long long per = nb >> 3, rem = nb & 7;
+  const long long xcd = b & 7, idx = b >> 3;
+  return xcd * per + (xcd < rem ? xcd : rem) + idx;
+}
+
+template <int N, bool CONTIG>
+struct PencilMap {
+  int t;
+  int p;
+  bool valid;
+  long long base;    // element offset of pencil element 0
+  long long stride;  // element stride along the pencil
+  long long pid;     // pencil id: CONTIG -> pencil index; strided -> o*inner + col
+  __device__ __forceinline__ PencilMap(const AxisGeo& g) {
+    constexpr int P = Plan<N>::P;
+    const long long tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int tid = threadIdx.x;
+    if constexpr (CONTIG) {
+      p = tid / P;
+      t = tid % P;
+      pid = tile * g.T + p;
+      valid = pid < g.outer;
+      base = pid * (long long)N;
+      stride = 1;
+    } else {
+      p = tid % g.T;
+      t = tid / g.T;
+      const long long o = tile / g.col_tiles;
+      const long long col = (tile % g.col_tiles) * g.T + p;
+      valid = col < g.inner;
+      base = o * (long long)N * g.inner + col;
+      stride = g.inner;
+      pid = o * g.inner + col;
+    }
+  }
+  __device__ __forceinline__ long long elem(int j) const { return base + (long long)j * stride; }
+};
+
+template <int N>
+__device__ __forceinline__ void load_pencil(double2 (&v)[Plan<N>::R], const double2* src,
+                                            long long base, long long stride, int t, bool valid) {
+  constexpr int R = Plan<N>::R, P = Plan<N>::P;
+#pragma unroll
+  for (int i = 0; i < R; ++i)
+    v[i] = valid ? src[base + (long long)(t + P * i) * stride] : make_double2(0.0, 0.0);
+}
+
+template <int N>
+__device__ __forceinline__ void store_pencil(const double2 (&v)[Plan<N>::R], double2* dst,
+                                             long long base, long long stride, int t, bool valid) {
+  constexpr int R = Plan<N>::R, P = Plan<N>::P;
+  if (!valid) return;
+#pragma unroll
+  for (int i = 0; i < R; ++i) dst[base + (long long)(t + P * i) * stride] = v[i];
+}
+
+// ------------------------------------------------------------ pointwise V
+
+// CRT_Base::Do_NL_Step (CRT_Base.h:397-441) / CRT_Base_IF::Do_NL_Step
+// (CRT_Base_IF.h:381-419) on one point of NC components.  `extra[i]` is the
+// potential V_i (NL, added first as in CRT_Base.h:416) or beta0*x0 - DeltaL_i
+// (NL_IF, added last as in CRT_Base_IF.h:405).
+template <int NC, bool EXTRA_FIRST>
+__device__ __forceinline__ void nl_point(double2 (&a)[NC], const double (&extra)[NC], const VArgs& p) {
+  double phi[NC];
+  double n[NC];
+#pragma unroll
+  for (int j = 0; j < NC; ++j) n[j] = a[j].x * a[j].x + a[j].y * a[j].y;
+#pragma unroll
+  for (int i = 0; i < NC; ++i) {
+    double ph = EXTRA_FIRST ? extra[i] : 0.0;
+#pragma unroll
+    for (int j = 0; j < NC; ++j) ph += p.gs[j + NC * i] * n[j];
+    if (!EXTRA_FIRST) ph += extra[i];
+    phi[i] = ph * (-p.dt);
+  }
+#pragma unroll
+  for (int i = 0; i < NC; ++i) {
+    double s, c;
+    sincos(phi[i], &s, &c);
+    const double re = a[i].x;
+    a[i].x = re * c - a[i].y * s;
+    a[i].y = a[i].y * c + re * s;
+  }
+}
+
+// Rabi frequency Omega(x0) and eta(x0) of Do_Bragg_ad (bragg.cpp:224,240)
+__device__ __forceinline__ void bragg_field(double x0, const VArgs& p, double& Om, double2& eta) {
+  const double t1 = p.t1;
+  Om = p.F * p.amp0 *
+       cos(p.laser_k * x0 - (p.laser_domh + p.chirp * t1 + p.chirp_rate0 * t1) * t1 + p.phase0 / 2);
+  double s, c;
+  sincos(-0.5 * (p.laser_dk * x0), &s, &c);
+  eta = make_double2(c, s);
+}
+
+// per-point analytic 2x2 exponential of Do_Bragg_ad (bragg.cpp:226-301),
+// same operation order as the reference
+__device__ __forceinline__ void bragg_point(double2& a, double2& b, double bx, double Om, double2 eta,
+                                            const VArgs& p) {
+  const double dt = p.dt;
+  const double n1 = a.x * a.x + a.y * a.y;
+  const double n2 = b.x * b.x + b.y * b.y;
+  const double V11 = p.gs[0] * n1 + p.gs[1] * n2 + bx;
+  const double V22 = p.gs[2] * n1 + p.gs[3] * n2 - p.deltaL[1] + bx;
+  double s, c;
+  if (Om == 0.0) {
+    sincos(-dt * V11, &s, &c);
+    double re = a.x;
+    a.x = re * c - a.y * s;
+    a.y = re * s + a.y * c;
+    sincos(-dt * V22, &s, &c);
+    re = b.x;
+    b.x = re * c - b.y * s;
+    b.y = re * s + b.y * c;
+    return;
+  }
+  const double r = sqrt((V11 - V22) * (V11 - V22) + 4.0 * Om * Om);
+  const double Ep = 0.5 * (V11 + V22 + r);
+  const double Em = 0.5 * (V11 + V22 - r);
+  sincos(-dt * Ep, &s, &c);
+  double nrm = 1.0 / fabs((V11 - Ep) * (V11 - Ep) + Om * Om);
+  const double gpr = nrm * c, gpi = nrm * s;
+  sincos(-dt * Em, &s, &c);
+  nrm = 1.0 / fabs((V11 - Em) * (V11 - Em) + Om * Om);
+  const double gmr = nrm * c, gmi = nrm * s;
+  const double o2 = Om * Om;
+  const double O11r = o2 * (gpr + gmr), O11i = o2 * (gpi + gmi);
+  const double dp = V11 - Ep, dm = V11 - Em;
+  const double O22r = dp * dp * gpr + dm * dm * gmr;
+  const double O22i = dp * dp * gpi + dm * dm * gmi;
+  const double Xr = -Om * (dp * gpr + dm * gmr);
+  const double Xi = -Om * (dp * gpi + dm * gmi);
+  const double O12r = eta.x * Xr - eta.y * Xi, O12i = eta.y * Xr + eta.x * Xi;
+  const double O21r = eta.x * Xr + eta.y * Xi, O21i = eta.x * Xi - eta.y * Xr;
+  const double ar = a.x, ai = a.y, br = b.x, bi = b.y;
+  a.x = (O11r * ar - O11i * ai) + (O12r * br - O12i * bi);
+  a.y = (O11r * ai + O11i * ar) + (O12r * bi + O12i * br);
+  b.x = (O21r * ar - O21i * ai) + (O22r * br - O22i * bi);
+  b.y = (O21r * ai + O21i * ar) + (O22r * bi + O22i * br);
+}
+
+// V step on the registers of NC component pencils at one element.
+// x0: first-axis coordinate of the element; l: linear point index.
+template <int VK, int NC>
+__device__ __forceinline__ void v_point(double2 (&a)[NC], double x0, long long l, const VArgs& p,
+                                        double Om, double2 eta) {
+  if constexpr (VK == V_NL) {
+    double extra[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) extra[c] = p.pot ? p.pot[c * p.pot_stride + l] : 0.0;
+    nl_point<NC, true>(a, extra, p);
+  } else if constexpr (VK == V_NL_IF) {
+    double extra[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) extra[c] = p.beta0 * x0 - p.deltaL[c];
+    nl_point<NC, false>(a, extra, p);
+  } else if constexpr (VK == V_BRAGG) {
+    static_assert(NC == 2, "Bragg coupling needs two components");
+    bragg_point(a[0], a[1], p.beta0 * x0, Om, eta, p);
+  }
+}
+
+// ------------------------------------------------------------ k_pass
+
+template <int N, bool CONTIG, int DIR>
+__global__ __launch_bounds__(256) void k_pass(const double2* src, double2* dst,
+                                              long long comp_stride, AxisGeo g, const double2* __restrict__ tw,
+                                              double scale) {
+  extern __shared__ double2 lds_all[];
+  constexpr int R = Plan<N>::R;
+  PencilMap<N, CONTIG> m(g);
+  double2* lds = lds_all + m.p * N;
+  const long long coff = (long long)blockIdx.y * comp_stride;
+  double2 v[R];
+  load_pencil<N>(v, src + coff, m.base, m.stride, m.t, m.valid);
+  fft_pencil<N, DIR, CONTIG && (Plan<N>::P <= 64)>(v, m.t, lds, tw);
+  if (scale != 1.0) {
+#pragma unroll
+    for (int i = 0; i < R; ++i) v[i] = cscale(v[i], scale);
+  }
+  store_pencil<N>(v, dst + coff, m.base, m.stride, m.t, m.valid);
+}
+
+// ------------------------------------------------------------ k_kin
+// Kinetic step along axis 0: forward DFT, multiply by
+//   E0[j] * E1[i1] * E2[i2]  (per-axis phase tables, E0 includes the
+//   ft(-1)*ft(+1) scale), inverse DFT.
+// NC > 1 or VK != V_NONE: 1D grids only (the pencil is the whole axis 0 run
+// and all components are held by the same thread).
+struct KinTabs {
+  const double2* e0;  // n0
+  const double2* e1;  // n1 (NULL in 1D)
+  const double2* e2;  // n2 (NULL in 1D/2D)
+  int n2;
+};
+
+template <int N, bool CONTIG, int VK, int NC>
+__global__ __launch_bounds__(256) void k_kin(double2* __restrict__ psi, long long comp_stride, AxisGeo g,
+                                             const double2* __restrict__ tw, KinTabs kt, VArgs va, double x0_dx) {
+  extern __shared__ double2 lds_all[];
+  constexpr int R = Plan<N>::R, P = Plan<N>::P;
+  constexpr bool WAVE = CONTIG && (P <= 64);
+  PencilMap<N, CONTIG> m(g);
+  double2* lds = lds_all + m.p * N;
+  // pencil constant of the phase (axes 1, 2)
+  double2 cp = make_double2(1.0, 0.0);
+  if constexpr (!CONTIG) {
+    const long long col = m.pid % g.inner;
+    if (kt.e1) {
+      const int i1 = (int)(col / kt.n2), i2 = (int)(col % kt.n2);
+      cp = kt.e1[i1];
+      if (kt.e2) cp = cmul(cp, kt.e2[i2]);
+    }
+  }
+  double2 v[NC][R];
+  const long long c0 = (NC == 1) ? (long long)blockIdx.y * comp_stride : 0;
+#pragma unroll
+  for (int c = 0; c < NC; ++c) load_pencil<N>(v[c], psi + c0 + c * comp_stride, m.base, m.stride, m.t, m.valid);
+  if constexpr (VK != V_NONE) {
+    // 1D: the pencil is axis 0 itself, x0 = (j - n0/2) dx
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      const int j = m.t + P * i;
+      const double x0 = (double)(j - N / 2) * x0_dx;
+      double2 a[NC];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) a[c] = v[c][i];
+      double Om = 0.0;
+      double2 eta = make_double2(1.0, 0.0);
+      if constexpr (VK == V_BRAGG) bragg_field(x0, va, Om, eta);
+      if (m.valid) v_point<VK, NC>(a, x0, m.base + j, va, Om, eta);
+#pragma unroll
+      for (int c = 0; c < NC; ++c) v[c][i] = a[c];
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    fft_pencil<N, -1, WAVE>(v[c], m.t, lds, tw);
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      double2 e = kt.e0[m.t + P * i];
+      if constexpr (!CONTIG) e = cmul(e, cp);
+      v[c][i] = cmul(v[c][i], e);
+    }
+    fft_pencil<N, +1, WAVE>(v[c], m.t, lds, tw);
+    store_pencil<N>(v[c], psi + c0 + c * comp_stride, m.base, m.stride, m.t, m.valid);
+  }
+}
+
+// ------------------------------------------------------------ k_vfuse
+// inverse DFT along the last axis, V step, forward DFT along the last axis.
+// The pencil (contiguous) has fixed coordinates on the other axes, so
+// x0 and the Bragg field are per-pencil constants.
+template <int N, int VK, int NC>
+__global__ __launch_bounds__(256) void k_vfuse(double2* __restrict__ psi, long long comp_stride, AxisGeo g,
+                                               const double2* __restrict__ tw, VArgs va, long long pencils_per_x0,
+                                               int n0, double dx0) {
+  extern __shared__ double2 lds_all[];
+  constexpr int R = Plan<N>::R, P = Plan<N>::P;
+  constexpr bool WAVE = P <= 64;
+  PencilMap<N, true> m(g);
+  double2* lds = lds_all + m.p * N;
+  const int i0 = (int)(m.pid / pencils_per_x0);
+  const double x0 = (double)(i0 - n0 / 2) * dx0;
+  double Om = 0.0;
+  double2 eta = make_double2(1.0, 0.0);
+  if constexpr (VK == V_BRAGG) bragg_field(x0, va, Om, eta);
+  double2 v[NC][R];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    load_pencil<N>(v[c], psi + c * comp_stride, m.base, 1, m.t, m.valid);
+    fft_pencil<N, +1, WAVE>(v[c], m.t, lds, tw);
+  }
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    double2 a[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) a[c] = v[c][i];
+    if (m.valid) v_point<VK, NC>(a, x0, m.base + m.t + P * i, va, Om, eta);
+#pragma unroll
+    for (int c = 0; c < NC; ++c) v[c][i] = a[c];
+  }
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    fft_pencil<N, -1, WAVE>(v[c], m.t, lds, tw);
+    store_pencil<N>(v[c], psi + c * comp_stride, m.base, 1, m.t, m.valid);
+  }
+}
+
+// ------------------------------------------------------------ k_sobf
+// Sobolev filters along axis 0 (CSOB_Base.h:251-301 fused): from the
+// transform F of psi along the other axes, writes
+//   out_sob = inverse0( op(k) * forward0(F) ) * scale,  op = 1/(1+alpha|k|^2)
+//   out_lap = inverse0( -alpha|k|^2 * forward0(F) ) * scale  (if non-NULL)
+// k is the native-order wave vector (cft_3d.cpp:145-166).
+struct SobArgs {
+  double alpha;
+  double dk[3];
+  int n[3];
+  int dim;
+  double scale;
+};
+
+__device__ __forceinline__ double kval(int i, int n, double dk) {
+  const int s = n / 2;
+  return dk * (double)(((i + s) % n) - s);
+}
+
+template <int N, bool CONTIG>
+__global__ __launch_bounds__(256) void k_sobf(const double2* src, double2* out_sob, double2* out_lap, long long comp_stride, AxisGeo g,
+                                              const double2* __restrict__ tw, SobArgs sa) {
+  extern __shared__ double2 lds_all[];
+  constexpr int R = Plan<N>::R, P = Plan<N>::P;
+  constexpr bool WAVE = CONTIG && (P <= 64);
+  PencilMap<N, CONTIG> m(g);
+  double2* lds = lds_all + m.p * N;
+  const long long coff = (long long)blockIdx.y * comp_stride;
+  double k12 = 0.0;  // sum over axes 1, 2 of k^2 (k.scale(alpha).k with isotropic alpha)
+  if constexpr (!CONTIG) {
+    const long long col = m.pid % g.inner;
+    if (sa.dim >= 2) {
+      const int i1 = (sa.dim == 3) ? (int)(col / sa.n[2]) : (int)col;
+      const double k1 = kval(i1, sa.n[1], sa.dk[1]);
+      k12 += (k1 * sa.alpha) * k1;
+    }
+    if (sa.dim == 3) {
+      const int i2 = (int)(col % sa.n[2]);
+      const double k2 = kval(i2, sa.n[2], sa.dk[2]);
+      k12 += (k2 * sa.alpha) * k2;
+    }
+  }
+  double2 v[R];
+  load_pencil<N>(v, src + coff, m.base, m.stride, m.t, m.valid);
+  fft_pencil<N, -1, WAVE>(v, m.t, lds, tw);
+  double2 w[R];
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    const double k0 = kval(m.t + P * i, N, sa.dk[0]);
+    const double phi = (k0 * sa.alpha) * k0 + k12;
+    w[i] = cscale(v[i], -phi * sa.scale);
+    v[i] = cscale(v[i], sa.scale / (1.0 + phi));
+  }
+  fft_pencil<N, +1, WAVE>(v, m.t, lds, tw);
+  store_pencil<N>(v, out_sob + coff, m.base, m.stride, m.t, m.valid);
+  if (out_lap) {
+    fft_pencil<N, +1, WAVE>(w, m.t, lds, tw);
+    store_pencil<N>(w, out_lap + coff, m.base, m.stride, m.t, m.valid);
+  }
+}
+
+// ------------------------------------------------------------ pointwise
+
+__device__ __forceinline__ double x0_of(long long l, const Grid& gr) {
+  const long long inner = gr.npts / gr.n[0];
+  const int i0 = (int)(l / inner);
+  return (double)(i0 - gr.n[0] / 2) * gr.dx[0];
+}
+
+template <int VK, int NC>
+__global__ __launch_bounds__(256) void k_point(double2* __restrict__ psi, Grid gr, VArgs va) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long l = (long long)blockIdx.x * blockDim.x + threadIdx.x; l < gr.npts; l += stride) {
+    double2 a[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) a[c] = psi[c * gr.npts + l];
+    const double x0 = x0_of(l, gr);
+    double Om = 0.0;
+    double2 eta = make_double2(1.0, 0.0);
+    if constexpr (VK == V_BRAGG) bragg_field(x0, va, Om, eta);
+    v_point<VK, NC>(a, x0, l, va, Om, eta);
+#pragma unroll
+    for (int c = 0; c < NC; ++c) psi[c * gr.npts + l] = a[c];
+  }
+}
+
+// ------------------------------------------------------------ reductions
+
+constexpr int RED_THREADS = 256;
+constexpr int RED_MAX_BLOCKS = 1024;
+constexpr int RED_MAX_NV = 8;
+
+template <int NV>
+__device__ __forceinline__ void block_reduce_store(double (&acc)[NV], double* out) {
+  __shared__ double sh[RED_THREADS / 64][NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    double x = acc[v];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    acc[v] = x;
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) sh[wave][v] = acc[v];
+  }
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    double s = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += sh[w][threadIdx.x];
+    out[threadIdx.x] = s;
+  }
+}
+
+template <class Fn>
+__global__ __launch_bounds__(RED_THREADS) void k_reduce(Fn fn, long long n, double* __restrict__ partials) {
+  constexpr int NV = Fn::NV;
+  double acc[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long l = (long long)blockIdx.x * blockDim.x + threadIdx.x; l < n; l += stride) fn(l, acc);
+  block_reduce_store<NV>(acc, partials + (long long)blockIdx.x * NV);
+}
+
+template <int NV>
+__global__ __launch_bounds__(RED_THREADS) void k_reduce_final(const double* __restrict__ partials, int nblocks,
+                                                              double* __restrict__ out) {
+  double acc[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+  for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc[v] += partials[(long long)b * NV + v];
+  }
+  block_reduce_store<NV>(acc, out);
+}
+
+// sum |psi|^2
+struct RedNorm {
+  static constexpr int NV = 1;
+  const double2* psi;
+  __device__ void operator()(long long l, double (&acc)[1]) const {
+    const double2 a = psi[l];
+    acc[0] += a.x * a.x + a.y * a.y;
+  }
+};
+
+// sum x_a |psi|^2 (CRT_Base::Expval_Position)
+struct RedX {
+  static constexpr int NV = 3;
+  const double2* psi;
+  Grid gr;
+  __device__ void operator()(long long l, double (&acc)[3]) const {
+    const double2 a = psi[l];
+    const double d = a.x * a.x + a.y * a.y;
+    const long long n12 = (long long)gr.n[1] * gr.n[2];
+    const int i0 = (int)(l / n12);
+    const int i1 = (int)((l - i0 * n12) / gr.n[2]);
+    const int i2 = (int)(l - i0 * n12 - (long long)i1 * gr.n[2]);
+    acc[0] += ((double)(i0 - gr.n[0] / 2) * gr.dx[0]) * d;
+    acc[1] += ((double)(i1 - gr.n[1] / 2) * gr.dx[1]) * d;
+    acc[2] += ((double)(i2 - gr.n[2] / 2) * gr.dx[2]) * d;
+  }
+};
+
+__device__ __forceinline__ void k_of(long long l, const Grid& gr, double (&k)[3]) {
+  const long long n12 = (long long)gr.n[1] * gr.n[2];
+  const int i0 = (int)(l / n12);
+  const int i1 = (int)((l - i0 * n12) / gr.n[2]);
+  const int i2 = (int)(l - i0 * n12 - (long long)i1 * gr.n[2]);
+  k[0] = kval(i0, gr.n[0], gr.dk[0]);
+  k[1] = gr.dim > 1 ? kval(i1, gr.n[1], gr.dk[1]) : 0.0;
+  k[2] = gr.dim > 2 ? kval(i2, gr.n[2], gr.dk[2]) : 0.0;
+}
+
+// sum k_a |psi^|^2 (CRT_Base::Expval_Momentum) on a transformed buffer
+struct RedK {
+  static constexpr int NV = 3;
+  const double2* psik;
+  Grid gr;
+  __device__ void operator()(long long l, double (&acc)[3]) const {
+    const double2 a = psik[l];
+    const double d = a.x * a.x + a.y * a.y;
+    double k[3];
+    k_of(l, gr, k);
+    acc[0] += k[0] * d;
+    acc[1] += k[1] * d;
+    acc[2] += k[2] * d;
+  }
+};
+
+// Rabi momentum bins (CRT_Base_IF::compute_rabi_integrals, CRT_Base_IF.h:
+// 240-256), including the slot counter that only advances on a miss
+struct RedRabi {
+  static constexpr int NV = RED_MAX_NV;
+  const double2* psik;
+  Grid gr;
+  int nk0;
+  double k0[RED_MAX_NV][3];
+  double thr;
+  __device__ void operator()(long long l, double (&acc)[RED_MAX_NV]) const {
+    const double2 a = psik[l];
+    const double d = a.x * a.x + a.y * a.y;
+    double k[3];
+    k_of(l, gr, k);
+    int j = 0;
+    for (int q = 0; q < nk0; ++q) {
+      double dd = 0.0;
+      for (int ax = 0; ax < gr.dim; ++ax) {
+        const double e = k0[q][ax] - k[ax];
+        dd += e * e;
+      }
+      if (sqrt(dd) < thr) {
+#pragma unroll
+        for (int s = 0; s < RED_MAX_NV; ++s)
+          if (s == j) acc[s] += d;
+        continue;
+      }
+      ++j;
+    }
+  }
+};
+
+// kinetic energy density sum_a alpha_a k_a^2 |psi^|^2 on a transformed buffer
+struct RedEkin {
+  static constexpr int NV = 1;
+  const double2* psik;
+  Grid gr;
+  double alpha[3];
+  __device__ void operator()(long long l, double (&acc)[1]) const {
+    const double2 a = psik[l];
+    double k[3];
+    k_of(l, gr, k);
+    double ks = 0.0;
+    for (int ax = 0; ax < gr.dim; ++ax) ks += alpha[ax] * k[ax] * k[ax];
+    acc[0] += ks * (a.x * a.x + a.y * a.y);
+  }
+};
+
+// potential + interaction energy sum_i (V_i + 1/2 sum_j g_ij n_j) n_i
+struct RedEpot {
+  static constexpr int NV = 1;
+  const double2* psi;
+  const double* pot;
+  long long npts;
+  int nc;
+  double gs[16];
+  __device__ void operator()(long long l, double (&acc)[1]) const {
+    double n[4];
+    for (int i = 0; i < nc; ++i) {
+      const double2 a = psi[i * npts + l];
+      n[i] = a.x * a.x + a.y * a.y;
+    }
+    double e = 0.0;
+    for (int i = 0; i < nc; ++i) {
+      const double v = pot ? pot[i * npts + l] : 0.0;
+      double nl = 0.0;
+      for (int j = 0; j < nc; ++j) nl += gs[j + nc * i] * n[j];
+      e += (v + 0.5 * nl) * n[i];
+    }
+    acc[0] += e;
+  }
+};
+
+// <psi, g> and <psi, psi_sob> (real parts, CSOB_Base.h:350-356)
+struct RedProj {
+  static constexpr int NV = 2;
+  const double2* psi;
+  const double2* grad;
+  const double2* sob;
+  __device__ void operator()(long long l, double (&acc)[2]) const {
+    const double2 p = psi[l], g = grad[l], s = sob[l];
+    acc[0] += p.x * g.x + p.y * g.y;
+    acc[1] += p.x * s.x + p.y * s.y;
+  }
+};
+
+}  // namespace atus

@@ -1,0 +1,108 @@
+// kernels_misc.hpp -- non-template kernels, launched only by atus_abi.hip
+// (kept out of kernels.hpp so each has one definition).
+#pragma once
+#include "kernels.hpp"
+
+namespace atus {
+
+// psi *= exp(i p.x)  (CRT_Base::Setup_Momentum, CRT_Base.h:449-473)
+__global__ __launch_bounds__(256) void k_momentum(double2* __restrict__ psi, Grid gr, double p0, double p1,
+                                                  double p2) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  const long long n12 = (long long)gr.n[1] * gr.n[2];
+  for (long long l = (long long)blockIdx.x * blockDim.x + threadIdx.x; l < gr.npts; l += stride) {
+    const int i0 = (int)(l / n12);
+    const int i1 = (int)((l - i0 * n12) / gr.n[2]);
+    const int i2 = (int)(l - i0 * n12 - (long long)i1 * gr.n[2]);
+    double px = p0 * ((double)(i0 - gr.n[0] / 2) * gr.dx[0]);
+    if (gr.dim > 1) px += p1 * ((double)(i1 - gr.n[1] / 2) * gr.dx[1]);
+    if (gr.dim > 2) px += p2 * ((double)(i2 - gr.n[2] / 2) * gr.dx[2]);
+    double s, c;
+    sincos(px, &s, &c);
+    const double2 a = psi[l];
+    psi[l] = make_double2(a.x * c - a.y * s, a.x * s + a.y * c);
+  }
+}
+
+// multiply a component by a complex table per point (kinetic step of the
+// unfused path)
+__global__ __launch_bounds__(256) void k_mul_phase(double2* __restrict__ psi, long long npts, KinTabs kt, int n1,
+                                                   int n2, int dim) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  const long long n12 = (long long)n1 * n2;
+  for (long long l = (long long)blockIdx.x * blockDim.x + threadIdx.x; l < npts; l += stride) {
+    const int i0 = (int)(l / n12);
+    const int i1 = (int)((l - i0 * n12) / n2);
+    const int i2 = (int)(l - i0 * n12 - (long long)i1 * n2);
+    double2 e = kt.e0[i0];
+    if (dim > 1) e = cmul(e, kt.e1[i1]);
+    if (dim > 2) e = cmul(e, kt.e2[i2]);
+    psi[blockIdx.y * npts + l] = cmul(psi[blockIdx.y * npts + l], e);
+  }
+}
+
+// ------------------------------------------------------------ Sobolev pointwise
+
+// L2 gradient -Lap psi + (V + sum_j g_ij |psi_j|^2) psi  (CSOB_Base.h:322-327)
+__global__ __launch_bounds__(256) void k_sob_grad_in(const double2* __restrict__ psi, const double2* __restrict__ lap,
+                                                     const double* __restrict__ pot, double2* __restrict__ out,
+                                                     long long npts, int nw, int comp, double g0, double g1) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  const double g[2] = {g0, g1};
+  for (long long l = (long long)blockIdx.x * blockDim.x + threadIdx.x; l < npts; l += stride) {
+    double nl = 0.0;
+    for (int j = 0; j < nw; ++j) {
+      const double2 a = psi[j * npts + l];
+      nl += g[j] * (a.x * a.x + a.y * a.y);
+    }
+    const double v = pot[comp * npts + l] + nl;
+    const double2 p = psi[comp * npts + l];
+    const double2 lp = lap[comp * npts + l];
+    out[comp * npts + l] = make_double2(-lp.x + v * p.x, -lp.y + v * p.y);
+  }
+}
+
+// g -= fak * psi_sob (CSOB_Base.h:360-365), then sum |g|^2 and |psi|^2
+__global__ __launch_bounds__(RED_THREADS) void k_sob_project_res(double2* __restrict__ grad,
+                                                                 const double2* __restrict__ sob,
+                                                                 const double2* __restrict__ psi, long long n,
+                                                                 double fak, double* __restrict__ partials) {
+  double acc[2] = {0.0, 0.0};
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long l = (long long)blockIdx.x * blockDim.x + threadIdx.x; l < n; l += stride) {
+    double2 g = grad[l];
+    const double2 s = sob[l];
+    g.x -= fak * s.x;
+    g.y -= fak * s.y;
+    grad[l] = g;
+    const double2 p = psi[l];
+    acc[0] += g.x * g.x + g.y * g.y;
+    acc[1] += p.x * p.x + p.y * p.y;
+  }
+  block_reduce_store<2>(acc, partials + (long long)blockIdx.x * 2);
+}
+
+// psi <- f * psi - step * g  (Renormalize_All_Psi then the update,
+// CSOB_Base.h:481-492; f = 1 when no renormalisation is due)
+__global__ __launch_bounds__(256) void k_sob_update(double2* __restrict__ psi, const double2* __restrict__ grad,
+                                                    long long n, double f, double step) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long l = (long long)blockIdx.x * blockDim.x + threadIdx.x; l < n; l += stride) {
+    double2 p = psi[l];
+    const double2 g = grad[l];
+    if (f != 1.0) {
+      p.x *= f;
+      p.y *= f;
+    }
+    p.x -= step * g.x;
+    p.y -= step * g.y;
+    psi[l] = p;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_scale(double2* __restrict__ psi, long long n, double f) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long l = (long long)blockIdx.x * blockDim.x + threadIdx.x; l < n; l += stride) psi[l] = cscale(psi[l], f);
+}
+
+}  // namespace atus

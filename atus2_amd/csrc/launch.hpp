@@ -1,0 +1,48 @@
+// launch.hpp -- host-side launchers of the kernel families (one translation
+// unit each, so the template instantiations compile in parallel).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+
+namespace atus {
+
+// workgroup shape of one axis pass: T pencils of P = N/min(16,N) threads
+struct PassCfg {
+  dim3 grid;
+  dim3 block;
+  size_t lds;
+  AxisGeo g;
+};
+
+inline int threads_per_pencil(int N) { return N / (N < 16 ? N : 16); }
+
+// contiguous pencils if inner == 1; the pass covers `outer` slabs of
+// [N][inner]; ycount = gridDim.y (components)
+PassCfg make_pass_cfg(int N, long long inner, long long outer, int ycount, size_t max_lds, int max_threads = 256);
+
+bool supported_len(int N);  // power of two in [2, 4096]
+
+hipError_t launch_pass(int N, bool contig, int dir, const double2* src, double2* dst, long long comp_stride,
+                       int ncomp, long long inner, long long outer, const double2* tw, double scale,
+                       size_t max_lds, hipStream_t s);
+
+// kinetic step along axis 0; vk/nc select the fused V prologue (1D only)
+hipError_t launch_kin(int N, bool contig, int vk, int nc, double2* psi, long long comp_stride, int ncomp,
+                      long long inner, const double2* tw, const KinTabs& kt, const VArgs& va, double dx0,
+                      size_t max_lds, hipStream_t s);
+
+// inverse / V / forward along the last (contiguous) axis of a 2D/3D grid
+hipError_t launch_vfuse(int N, int vk, int nc, double2* psi, long long comp_stride, long long outer,
+                        const double2* tw, const VArgs& va, long long pencils_per_x0, int n0, double dx0,
+                        size_t max_lds, hipStream_t s);
+
+// Sobolev filters along axis 0
+hipError_t launch_sobf(int N, bool contig, const double2* src, double2* out_sob, double2* out_lap,
+                       long long comp_stride, int ncomp, long long inner, const double2* tw, const SobArgs& sa,
+                       size_t max_lds, hipStream_t s);
+
+// pointwise V step on all points (unfused path)
+hipError_t launch_point(int vk, int nc, double2* psi, const Grid& gr, const VArgs& va, hipStream_t s);
+
+}  // namespace atus

@@ -1,0 +1,314 @@
+"""ctypes binding of libatus_hip.so (include/atus_hip.h).
+
+This is the Python face of the C-ABI: it loads the in-tree library and
+raises if it is missing -- there is no CPU fallback.  Fields are numpy
+complex128 arrays in the reference's fftw_complex layout.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libatus_hip.so")
+
+STEP_FREEPROP, STEP_FREEPROP_IF, STEP_BRAGG_AD = 0, 1, 2
+
+EXPORTED = [
+    "atus_last_error", "atus_version", "atus_device_info", "atus_ctx_create", "atus_ctx_destroy",
+    "atus_synchronize", "atus_upload", "atus_download", "atus_set_potential", "atus_set_g",
+    "atus_set_alpha", "atus_ft", "atus_kinetic", "atus_nl_step", "atus_nl_if_step", "atus_bragg_ad",
+    "atus_set_momentum", "atus_run_block", "atus_particle_number", "atus_expval_x", "atus_expval_k",
+    "atus_rabi_integrals", "atus_energy", "atus_sob_setup", "atus_sob_renormalize", "atus_sob_iter",
+    "atus_sob_mu", "atus_set_timing", "atus_last_block_ms", "atus_run_block_profiled",
+]
+
+KFAM = ["pass_contig", "pass_strided", "kin", "vfuse", "point"]
+
+
+class AtusError(RuntimeError):
+    pass
+
+
+class Laser(C.Structure):
+    _fields_ = [("amp0", C.c_double), ("laser_k", C.c_double), ("laser_domh", C.c_double),
+                ("laser_dk", C.c_double), ("chirp", C.c_double), ("chirp_rate0", C.c_double),
+                ("phase0", C.c_double), ("beta0", C.c_double), ("deltaL", C.c_double * 4)]
+
+
+AMP_FN = C.CFUNCTYPE(C.c_double, C.c_double, C.c_void_p)
+
+
+class StepParams(C.Structure):
+    _fields_ = [("kind", C.c_int), ("dt", C.c_double), ("laser", Laser), ("amp", AMP_FN),
+                ("amp_user", C.c_void_p)]
+
+
+_lib = None
+
+
+def load():
+    """Load libatus_hip.so; raises AtusError if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise AtusError(f"{LIB_PATH} is missing: build it with `make -C atus2_amd/csrc` "
+                        "(or __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    P, D, I = C.c_void_p, C.c_double, C.c_int
+    pctx = C.POINTER(C.c_void_p)
+    sig = {
+        "atus_last_error": (C.c_char_p, []),
+        "atus_version": (C.c_char_p, []),
+        "atus_device_info": (I, [C.c_char_p, I, C.POINTER(I), C.POINTER(D)]),
+        "atus_ctx_create": (I, [pctx, I, C.POINTER(C.c_longlong), C.POINTER(D), C.POINTER(D), I]),
+        "atus_ctx_destroy": (I, [P]),
+        "atus_synchronize": (I, [P]),
+        "atus_upload": (I, [P, I, P]),
+        "atus_download": (I, [P, I, P]),
+        "atus_set_potential": (I, [P, I, P]),
+        "atus_set_g": (I, [P, P]),
+        "atus_set_alpha": (I, [P, C.POINTER(D)]),
+        "atus_ft": (I, [P, I, I]),
+        "atus_kinetic": (I, [P, D, D]),
+        "atus_nl_step": (I, [P, D]),
+        "atus_nl_if_step": (I, [P, D, D, P]),
+        "atus_bragg_ad": (I, [P, D, D, D, C.POINTER(Laser)]),
+        "atus_set_momentum": (I, [P, I, C.POINTER(D)]),
+        "atus_run_block": (I, [P, C.POINTER(StepParams), I, C.POINTER(D)]),
+        "atus_particle_number": (I, [P, I, C.POINTER(D)]),
+        "atus_expval_x": (I, [P, I, C.POINTER(D)]),
+        "atus_expval_k": (I, [P, I, C.POINTER(D)]),
+        "atus_rabi_integrals": (I, [P, I, P, I, D, P]),
+        "atus_energy": (I, [P, C.POINTER(D)]),
+        "atus_sob_setup": (I, [P, P, D]),
+        "atus_sob_renormalize": (I, [P]),
+        "atus_sob_iter": (I, [P, C.POINTER(D), C.POINTER(I)]),
+        "atus_sob_mu": (I, [P, P]),
+        "atus_set_timing": (I, [P, I]),
+        "atus_last_block_ms": (I, [P, C.POINTER(D)]),
+        "atus_run_block_profiled": (I, [P, C.POINTER(StepParams), I, C.POINTER(D), C.POINTER(D),
+                                        C.POINTER(I)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        msg = _lib.atus_last_error().decode(errors="replace")
+        raise AtusError(f"atus call failed ({rc}): {msg}")
+
+
+def _dp(a: np.ndarray):
+    assert a.flags.c_contiguous
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def laser(**kw) -> Laser:
+    L = Laser()
+    dl = kw.pop("deltaL", (0.0,) * 4)
+    for k, v in kw.items():
+        setattr(L, k, float(v))
+    for i, v in enumerate(dl):
+        L.deltaL[i] = float(v)
+    return L
+
+
+class Context:
+    """One grid with `ncomp` device-resident complex128 components."""
+
+    def __init__(self, shape, dx, dk, ncomp=1):
+        L = load()
+        shape = tuple(int(s) for s in shape)
+        self.shape = shape
+        self.dim = len(shape)
+        self.ncomp = int(ncomp)
+        n = (C.c_longlong * 3)(*(list(shape) + [1] * (3 - self.dim)))
+        dxa = (C.c_double * 3)(*(list(dx)[:self.dim] + [0.0] * (3 - self.dim)))
+        dka = (C.c_double * 3)(*(list(dk)[:self.dim] + [0.0] * (3 - self.dim)))
+        h = C.c_void_p()
+        _check(L.atus_ctx_create(C.byref(h), self.dim, n, dxa, dka, self.ncomp))
+        self._h = h
+        self._amp_keep = None
+
+    @classmethod
+    def from_grid(cls, g, ncomp=1):
+        """From an oracle-style Grid structure (dim, n, dx, dk)."""
+        shape = tuple(int(g.n[a]) for a in range(g.dim))
+        return cls(shape, [g.dx[a] for a in range(g.dim)], [g.dk[a] for a in range(g.dim)], ncomp)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.atus_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # state
+    def upload(self, comp, psi):
+        a = np.ascontiguousarray(psi, dtype=np.complex128)
+        if a.shape != self.shape:
+            raise ValueError(f"shape {a.shape} != grid {self.shape}")
+        _check(_lib.atus_upload(self._h, comp, _dp(a)))
+
+    def download(self, comp):
+        a = np.empty(self.shape, np.complex128)
+        _check(_lib.atus_download(self._h, comp, _dp(a)))
+        return a
+
+    def set_potential(self, comp, V):
+        if V is None:
+            _check(_lib.atus_set_potential(self._h, 0, None))
+            return
+        a = np.ascontiguousarray(V, dtype=np.float64)
+        if a.shape != self.shape:
+            raise ValueError("potential shape mismatch")
+        _check(_lib.atus_set_potential(self._h, comp, _dp(a)))
+
+    def set_g(self, gs):
+        a = np.ascontiguousarray(np.ravel(gs), dtype=np.float64)
+        _check(_lib.atus_set_g(self._h, _dp(a)))
+
+    def set_alpha(self, alpha):
+        a = (C.c_double * 3)(*(list(alpha) + [0.0] * (3 - len(alpha))))
+        _check(_lib.atus_set_alpha(self._h, a))
+
+    # steps
+    def ft(self, comp, isign):
+        _check(_lib.atus_ft(self._h, comp, isign))
+
+    def kinetic(self, frac, dt):
+        _check(_lib.atus_kinetic(self._h, frac, dt))
+
+    def nl_step(self, dt):
+        _check(_lib.atus_nl_step(self._h, dt))
+
+    def nl_if_step(self, dt, beta0, deltaL):
+        a = np.ascontiguousarray(deltaL, dtype=np.float64)
+        _check(_lib.atus_nl_if_step(self._h, dt, beta0, _dp(a)))
+
+    def bragg_ad(self, dt, t, F, las: Laser):
+        _check(_lib.atus_bragg_ad(self._h, dt, t, F, C.byref(las)))
+
+    def set_momentum(self, comp, p):
+        a = (C.c_double * 3)(*(list(p) + [0.0] * (3 - len(p))))
+        _check(_lib.atus_set_momentum(self._h, comp, a))
+
+    def run_block(self, kind, dt, Nk, t, las: Laser | None = None, amp=None):
+        sp = StepParams()
+        sp.kind = kind
+        sp.dt = dt
+        if las is not None:
+            sp.laser = las
+        if amp is not None:
+            cb = AMP_FN(lambda tt, u: float(amp(tt)))
+            self._amp_keep = cb
+            sp.amp = cb
+        tt = C.c_double(t)
+        _check(_lib.atus_run_block(self._h, C.byref(sp), int(Nk), C.byref(tt)))
+        return tt.value
+
+    def run_block_profiled(self, kind, dt, Nk, t, las: Laser | None = None, amp=None):
+        """Runs one block like run_block; returns (t, {family: (ms_total, launches)})."""
+        sp = StepParams()
+        sp.kind = kind
+        sp.dt = dt
+        if las is not None:
+            sp.laser = las
+        if amp is not None:
+            cb = AMP_FN(lambda tt, u: float(amp(tt)))
+            self._amp_keep = cb
+            sp.amp = cb
+        tt = C.c_double(t)
+        ms = (C.c_double * len(KFAM))()
+        cnt = (C.c_int * len(KFAM))()
+        _check(_lib.atus_run_block_profiled(self._h, C.byref(sp), int(Nk), C.byref(tt), ms, cnt))
+        return tt.value, {KFAM[i]: (ms[i], cnt[i]) for i in range(len(KFAM))}
+
+    # reductions
+    def particle_number(self, comp=0):
+        out = C.c_double()
+        _check(_lib.atus_particle_number(self._h, comp, C.byref(out)))
+        return out.value
+
+    def expval_x(self, comp=0):
+        out = (C.c_double * 3)()
+        _check(_lib.atus_expval_x(self._h, comp, out))
+        return np.array(out[:self.dim])
+
+    def expval_k(self, comp=0):
+        out = (C.c_double * 3)()
+        _check(_lib.atus_expval_k(self._h, comp, out))
+        return np.array(out[:self.dim])
+
+    def rabi_integrals(self, comp, k0list, threshold):
+        k0 = np.zeros((len(k0list), 3))
+        for i, k in enumerate(k0list):
+            k0[i, :len(k)] = k
+        k0 = np.ascontiguousarray(k0)
+        out = np.zeros(len(k0list))
+        _check(_lib.atus_rabi_integrals(self._h, comp, _dp(k0), len(k0list), threshold, _dp(out)))
+        return out
+
+    def energy(self):
+        out = C.c_double()
+        _check(_lib.atus_energy(self._h, C.byref(out)))
+        return out.value
+
+    # Sobolev
+    def sob_setup(self, alpha_iso, stepsize):
+        a = np.ascontiguousarray(alpha_iso, dtype=np.float64)
+        _check(_lib.atus_sob_setup(self._h, _dp(a), stepsize))
+
+    def sob_renormalize(self):
+        _check(_lib.atus_sob_renormalize(self._h))
+
+    def sob_iter(self):
+        r = C.c_double()
+        rn = C.c_int()
+        _check(_lib.atus_sob_iter(self._h, C.byref(r), C.byref(rn)))
+        return r.value, bool(rn.value)
+
+    def sob_mu(self):
+        out = np.zeros(self.ncomp)
+        _check(_lib.atus_sob_mu(self._h, _dp(out)))
+        return out
+
+    # timing
+    def set_timing(self, on=True):
+        _check(_lib.atus_set_timing(self._h, 1 if on else 0))
+
+    def last_block_ms(self):
+        out = C.c_double()
+        _check(_lib.atus_last_block_ms(self._h, C.byref(out)))
+        return out.value
+
+    def synchronize(self):
+        _check(_lib.atus_synchronize(self._h))
+
+
+def device_info():
+    L = load()
+    name = C.create_string_buffer(256)
+    cu = C.c_int()
+    bw = C.c_double()
+    _check(L.atus_device_info(name, 256, C.byref(cu), C.byref(bw)))
+    return name.value.decode(), cu.value, bw.value

@@ -1,0 +1,403 @@
+// CRT_Base_IF.h -- interferometry layer over CRT_Base (the reference's
+// CRT_Base_IF<T,dim,no_int_states>, CRT_Base_IF.h:48-946): laser
+// parameters, Amplitude_at_time (AMP_T), the gravity/detuning free step,
+// Rabi momentum-bin integrals, chirp / phase scans and their text outputs.
+//
+// The GSL diagonalisers registered as "bragg" and "raman"
+// (CRT_Base_IF.h:427-632, 3-state couplings) are not part of this build:
+// selecting them throws.  phase[0] starts at 0 (the reference leaves it
+// uninitialised unless chirp_mode == 1).
+#ifndef ATUS_AMD_CRT_BASE_IF_H
+#define ATUS_AMD_CRT_BASE_IF_H
+
+#include <list>
+
+#include "CRT_Base.h"
+
+template <class T, int dim, int no_int_states>
+class CRT_Base_IF : public CRT_Base<T, dim, no_int_states> {
+ public:
+  explicit CRT_Base_IF(ParameterHandler*);
+  virtual ~CRT_Base_IF() = default;
+  void run_sequence();
+
+ protected:
+  using CRT_Base<T, dim, no_int_states>::m_header;
+  using CRT_Base<T, dim, no_int_states>::m_params;
+  using CRT_Base<T, dim, no_int_states>::m_fields;
+  using CRT_Base<T, dim, no_int_states>::m_custom_fct;
+  using CRT_Base<T, dim, no_int_states>::m_set;
+  using CRT_shared::m_no_of_pts;
+
+  CPoint<dim> beta;
+  std::array<double, no_int_states> DeltaL;
+  std::array<double, 2> Amp, laser_k, laser_dk, laser_domh, phase, chirp_rate;
+  double chirp;
+  bool amp_is_t;
+
+  static void Do_NL_Step_Wrapper(void*, sequence_item&);
+  static void Numerical_Bragg_Wrapper(void*, sequence_item&);
+  static void Numerical_Raman_Wrapper(void*, sequence_item&);
+  void Do_NL_Step();
+
+  void UpdateParams();
+  void Output_rabi_freq_list(std::string, const long long);
+  void Output_chirps_list(std::string);
+
+  virtual bool run_custom_sequence(const sequence_item&) override = 0;
+
+  double m_rabi_threshold;
+  std::vector<CPoint<dim>> m_rabi_momentum_list;
+  std::list<std::list<double>> m_rabi_freq_list;
+  std::list<std::list<double>> m_chirps_list;
+
+  void compute_rabi_integrals();
+  double Amplitude_at_time();
+
+  // AMP_T evaluator
+  atus_host::Expr m_amp_expr;
+  double m_amp_time = 0.0;
+  double amplitude_at(double t1) override;
+  bool has_amplitude() const override { return amp_is_t; }
+  void fill_step_params(atus_step_params& sp, const sequence_item& seq) override;
+  atus_laser laser_params() const;
+};
+
+template <class T, int dim, int no_int_states>
+CRT_Base_IF<T, dim, no_int_states>::CRT_Base_IF(ParameterHandler* params) : CRT_Base<T, dim, no_int_states>(params) {
+  this->m_map_stepfcts["freeprop"] = &Do_NL_Step_Wrapper;
+  this->m_map_stepfcts["bragg"] = &Numerical_Bragg_Wrapper;
+  this->m_map_stepfcts["raman"] = &Numerical_Raman_Wrapper;
+  this->m_device_steps.erase(&CRT_Base<T, dim, no_int_states>::Do_NL_Step_Wrapper);
+  this->m_device_steps[&Do_NL_Step_Wrapper] = ATUS_STEP_FREEPROP_IF;
+  phase.fill(0.0);
+  chirp_rate.fill(0.0);
+  Amp.fill(0.0);
+  laser_k.fill(0.0);
+  laser_dk.fill(0.0);
+  laser_domh.fill(0.0);
+  amp_is_t = false;
+  UpdateParams();
+}
+
+// CRT_Base_IF::UpdateParams (CRT_Base_IF.h:145-168)
+template <class T, int dim, int no_int_states>
+void CRT_Base_IF<T, dim, no_int_states>::UpdateParams() {
+  for (int i = 0; i < dim; i++) beta[i] = m_params->Get_VConstant("Beta", i);
+  for (int i = 0; i < no_int_states; i++) DeltaL[i] = m_params->Get_VConstant("Delta_L", i);
+  Amp[0] = m_params->Get_VConstant("Amp_1", 0);
+  Amp[1] = m_params->Get_VConstant("Amp_1", 1);
+  laser_k[0] = m_params->Get_Constant("laser_k");
+  laser_dk[0] = m_params->Get_Constant("laser_dk");
+  laser_domh[0] = m_params->Get_Constant("laser_domh");
+  m_rabi_threshold = m_params->Get_Constant("rabi_threshold");
+  chirp = m_params->Get_Constant("chirp");
+}
+
+template <class T, int dim, int no_int_states>
+double CRT_Base_IF<T, dim, no_int_states>::amplitude_at(double t1) {
+  m_amp_time = t1;
+  return m_amp_expr.Eval();
+}
+
+// CRT_Base_IF::Amplitude_at_time (CRT_Base_IF.h:171-198)
+template <class T, int dim, int no_int_states>
+double CRT_Base_IF<T, dim, no_int_states>::Amplitude_at_time() {
+  if (!amp_is_t) return 1;
+  return amplitude_at(this->Get_t() + 0.5 * this->Get_dt());
+}
+
+template <class T, int dim, int no_int_states>
+atus_laser CRT_Base_IF<T, dim, no_int_states>::laser_params() const {
+  atus_laser L{};
+  L.amp0 = Amp[0];
+  L.laser_k = laser_k[0];
+  L.laser_domh = laser_domh[0];
+  L.laser_dk = laser_dk[0];
+  L.chirp = chirp;
+  L.chirp_rate0 = chirp_rate[0];
+  L.phase0 = phase[0];
+  L.beta0 = beta[0];
+  for (int i = 0; i < no_int_states && i < 4; ++i) L.deltaL[i] = DeltaL[i];
+  return L;
+}
+
+template <class T, int dim, int no_int_states>
+void CRT_Base_IF<T, dim, no_int_states>::fill_step_params(atus_step_params& sp, const sequence_item&) {
+  sp.laser = laser_params();
+  sp.amp = nullptr;
+  sp.amp_user = nullptr;
+}
+
+template <class T, int dim, int no_int_states>
+void CRT_Base_IF<T, dim, no_int_states>::Do_NL_Step_Wrapper(void* ptr, sequence_item&) {
+  static_cast<CRT_Base_IF*>(ptr)->Do_NL_Step();
+}
+template <class T, int dim, int no_int_states>
+void CRT_Base_IF<T, dim, no_int_states>::Numerical_Bragg_Wrapper(void*, sequence_item&) {
+  throw std::string("the numerical Bragg step (GSL eigensolver) is not part of this build");
+}
+template <class T, int dim, int no_int_states>
+void CRT_Base_IF<T, dim, no_int_states>::Numerical_Raman_Wrapper(void*, sequence_item&) {
+  throw std::string("the numerical Raman step (GSL eigensolver) is not part of this build");
+}
+
+// CRT_Base_IF::Do_NL_Step (CRT_Base_IF.h:381-419)
+template <class T, int dim, int no_int_states>
+void CRT_Base_IF<T, dim, no_int_states>::Do_NL_Step() {
+  double dl[4] = {0, 0, 0, 0};
+  for (int i = 0; i < no_int_states && i < 4; ++i) dl[i] = DeltaL[i];
+  atus_host::check(atus_nl_if_step(m_set->device(), m_header.dt, beta[0], dl), "atus_nl_if_step");
+}
+
+// CRT_Base_IF::compute_rabi_integrals (CRT_Base_IF.h:208-281); the device
+// transforms a copy, so psi is left as it was
+template <class T, int dim, int no_int_states>
+void CRT_Base_IF<T, dim, no_int_states>::compute_rabi_integrals() {
+  const int n = (int)m_rabi_momentum_list.size();
+  if (n == 0) {
+    std::cerr << "WARNING: Rabi momentum list is empty. Cannot compute rabi frquencies." << std::endl;
+    return;
+  }
+  std::vector<double> k0(3 * (size_t)n, 0.0), out(n);
+  for (int q = 0; q < n; ++q)
+    for (int a = 0; a < dim; ++a) k0[3 * q + a] = m_rabi_momentum_list[q][a];
+  atus_host::check(atus_rabi_integrals(m_set->device_readonly(), 0, k0.data(), n, m_rabi_threshold, out.data()),
+                   "atus_rabi_integrals");
+  m_rabi_freq_list.push_back(std::list<double>(out.begin(), out.end()));
+}
+
+// CRT_Base_IF::Output_rabi_freq_list (CRT_Base_IF.h:289-314)
+template <class T, int dim, int no_int_states>
+void CRT_Base_IF<T, dim, no_int_states>::Output_rabi_freq_list(std::string filename, const long long Nk) {
+  std::ofstream txtfile(filename);
+  double t = 0;
+  txtfile << "# time \t";
+  for (auto& i : m_rabi_momentum_list) txtfile << i << "\t";
+  txtfile << std::endl;
+  for (auto& i : m_rabi_freq_list) {
+    txtfile << t << "\t";
+    for (auto j : i) txtfile << j << "\t";
+    txtfile << std::endl;
+    t += m_header.dt * double(Nk);
+  }
+}
+
+// CRT_Base_IF::Output_chirps_list (CRT_Base_IF.h:320-342)
+template <class T, int dim, int no_int_states>
+void CRT_Base_IF<T, dim, no_int_states>::Output_chirps_list(std::string filename) {
+  std::ofstream txtfile(filename);
+  txtfile << "# Step \t Chirp \t ";
+  for (auto& i : m_rabi_momentum_list) txtfile << i << "\t";
+  txtfile << std::endl;
+  for (auto& i : m_chirps_list) {
+    for (auto j : i) {
+      txtfile.precision(8);
+      txtfile << j << "\t";
+    }
+    txtfile << std::endl;
+  }
+}
+
+// CRT_Base_IF::run_sequence (CRT_Base_IF.h:639-946)
+template <class T, int dim, int no_int_states>
+void CRT_Base_IF<T, dim, no_int_states>::run_sequence() {
+  StepFunction step_fct = nullptr, half_step_fct = nullptr, full_step_fct = nullptr;
+  char filename[1024];
+  std::cout << "FYI: Found " << m_params->m_sequence.size() << " sequences." << std::endl;
+  const int nrm = (int)m_rabi_momentum_list.size();
+  if (nrm == 0) std::cerr << "WARNING: Rabi momentum list is empty. Cannot compute rabi frquencies." << std::endl;
+  try {
+    half_step_fct = this->m_map_stepfcts.at("half_step");
+    full_step_fct = this->m_map_stepfcts.at("full_step");
+  } catch (const std::out_of_range&) {
+    throw std::string("Critical Error: Invalid fct ptr to ft_half_step or ft_full_step");
+  }
+  this->sync_params();
+  int seq_counter = 1;
+  for (auto seq : m_params->m_sequence) {
+    if (run_custom_sequence(seq)) {
+      seq_counter++;
+      continue;
+    }
+    if (seq.name == "set_momentum") {
+      std::vector<std::string> vec;
+      {
+        std::string cur;
+        for (char ch : seq.content) {
+          if (ch == ',') {
+            vec.push_back(cur);
+            cur.clear();
+          } else
+            cur += ch;
+        }
+        vec.push_back(cur);
+      }
+      if (!((int)vec.size() > dim)) throw std::string("set_momentum needs more than dim comma-separated values");
+      if (!(seq.comp <= dim)) throw std::string("set_momentum: component index out of range");
+      CPoint<dim> P;
+      for (int i = 0; i < dim; i++) P[i] = std::stod(vec[i]);
+      this->Setup_Momentum(P, seq.comp);
+      std::cout << "FYI: started new sequence " << seq.name << "\n";
+      std::cout << "FYI: momentum set for component " << seq.comp << "\n";
+      continue;
+    }
+    double max_duration = 0;
+    for (double d : seq.duration)
+      if (d > max_duration) max_duration = d;
+    int subN = int(max_duration / seq.dt);
+    int Nk = seq.Nk;
+    int Na = subN / seq.Nk;
+    std::cout << "FYI: started new sequence " << seq.name << "\n";
+    std::cout << "FYI: sequence no : " << seq_counter << "\n";
+    std::cout << "FYI: duration    : " << max_duration << "\n";
+    std::cout << "FYI: dt          : " << seq.dt << "\n";
+    std::cout << "FYI: Na          : " << Na << "\n";
+    std::cout << "FYI: Nk          : " << Nk << "\n";
+    std::cout << "FYI: Na*Nk*dt    : " << double(Na * Nk) * seq.dt << "\n";
+    try {
+      std::string amp = m_params->Get_simulation("AMP_T");
+      std::cout << "FYI: Amp is      : " << amp << "\n";
+      amp_is_t = true;
+      m_params->Setup_muParser(m_amp_expr);
+      m_amp_expr.SetExpr(amp);
+      m_amp_expr.DefineVar("t", &m_amp_time);
+    } catch (std::string&) {
+      std::cout << "FYI: Amp is      : 1" << std::endl;
+      amp_is_t = false;
+    }
+    if (double(Na * Nk) * seq.dt != max_duration) std::cout << "FYI: double(Na*Nk)*seq.dt != max_duration\n";
+    if (this->Get_dt() != seq.dt) this->Set_dt(seq.dt);
+    auto it = this->m_map_stepfcts.find(seq.name);
+    if (it == this->m_map_stepfcts.end()) throw std::string("Critical Error: Invalid squence name " + seq.name);
+    step_fct = it->second;
+    if (seq.name == "freeprop") seq.no_of_chirps = 1;
+    const double backup_t = m_header.t;
+    double backup_end_t = m_header.t;
+    for (int k = 0; k < no_int_states; k++) {
+      snprintf(filename, sizeof(filename), "Seq_%d_%d.bin", seq_counter, k + 1);
+      std::remove(filename);
+    }
+    m_chirps_list.clear();
+    const int nch = seq.no_of_chirps > 0 ? seq.no_of_chirps : 1;
+    std::vector<double> dw(std::max(nch, 2), 0.0), dphi(std::max(nch, 2), 0.0);
+    if (seq.no_of_chirps > 1) {
+      for (int k = 0; k < no_int_states; k++) {
+        snprintf(filename, sizeof(filename), "%.3f_%d.bin", this->Get_t(), k + 1);
+        this->Save_Phi(filename, k);
+      }
+      dw[0] = seq.chirp_min;
+      dw[1] = seq.chirp_max;
+    } else {
+      dw[0] = 0;
+    }
+    const bool fused = this->device_block_possible(step_fct);
+    for (int s = 0; s < nch; ++s) {
+      m_rabi_freq_list.clear();
+      chirp_rate[0] = dw[s];
+      if (seq.chirp_mode == 1) phase[0] = s * 2.0 * M_PI / (1.0 * seq.no_of_chirps);
+      for (int i = 1; i <= Na; i++) {
+        if (fused) {
+          this->run_device_block(step_fct, seq, Nk);
+        } else {
+          (*half_step_fct)(this, seq);
+          for (int j = 2; j <= Nk; j++) {
+            (*step_fct)(this, seq);
+            (*full_step_fct)(this, seq);
+          }
+          (*step_fct)(this, seq);
+          (*half_step_fct)(this, seq);
+        }
+        std::cout << "t = " << std::to_string(m_header.t) << std::endl;
+        if (seq.output_freq == freq::each && s == 0)
+          for (int k = 0; k < no_int_states; k++) {
+            snprintf(filename, sizeof(filename), "%.3f_%d.bin", this->Get_t(), k + 1);
+            this->Save_Phi(filename, k);
+          }
+        if (seq.output_freq == freq::packed && s == 0)
+          for (int k = 0; k < no_int_states; k++) {
+            snprintf(filename, sizeof(filename), "Seq_%d_%d.bin", seq_counter, k + 1);
+            this->Append_Phi(filename, k);
+          }
+        if (seq.compute_pn_freq == freq::each)
+          for (int c = 0; c < no_int_states; c++)
+            std::cout << "N[" << c << "] = " << this->Get_Particle_Number(c) << std::endl;
+        if (seq.rabi_output_freq == freq::each) compute_rabi_integrals();
+        if (seq.custom_freq == freq::each && m_custom_fct != nullptr) (*m_custom_fct)(this, seq);
+      }
+      if (seq.output_freq == freq::last && s == 0)
+        for (int k = 0; k < no_int_states; k++) {
+          snprintf(filename, sizeof(filename), "%.3f_%d.bin", this->Get_t(), k + 1);
+          this->Save_Phi(filename, k);
+        }
+      if (seq.no_of_chirps > 1 && s == 0) {
+        backup_end_t = this->Get_t();
+        for (int k = 0; k < no_int_states; k++) {
+          snprintf(filename, sizeof(filename), "%.3f_%d.bin", this->Get_t(), k + 1);
+          this->Save_Phi(filename, k);
+        }
+      }
+      if (seq.compute_pn_freq == freq::last)
+        for (int c = 0; c < no_int_states; c++)
+          std::cout << "N[" << c << "] = " << this->Get_Particle_Number(c) << std::endl;
+      if (seq.custom_freq == freq::last && m_custom_fct != nullptr) (*m_custom_fct)(this, seq);
+      if (seq.rabi_output_freq == freq::last) {
+        compute_rabi_integrals();
+        std::cout << "Rabi output for timestep ";
+        for (auto& i : m_rabi_freq_list) {
+          std::cout << this->Get_t() << "\n";
+          for (auto j : i) std::cout << j << "\t";
+          std::cout << std::endl;
+        }
+      }
+      if (seq.rabi_output_freq == freq::each) {
+        snprintf(filename, sizeof(filename), "Rabi_%d_%d.txt", seq_counter, s);
+        Output_rabi_freq_list(filename, seq.Nk);
+      }
+      if (seq.no_of_chirps > 1) {
+        if (nrm > 0) {
+          compute_rabi_integrals();
+          std::list<double> tmp = m_rabi_freq_list.back();
+          if (seq.chirp_mode == 1) {
+            tmp.push_front(phase[0]);
+          } else {
+            dphi[s] = tmp.front();
+            tmp.push_front(chirp_rate[0]);
+          }
+          tmp.push_front(s);
+          m_chirps_list.push_back(tmp);
+        }
+        if (s < seq.no_of_chirps - 1) {
+          for (int k = 0; k < no_int_states; k++) {
+            snprintf(filename, sizeof(filename), "%.3f_%d.bin", backup_t, k + 1);
+            std::ifstream file1(filename, std::ifstream::binary);
+            file1.seekg(sizeof(generic_header), std::ifstream::beg);
+            file1.read((char*)m_fields[k]->Getp2In(), sizeof(fftw_complex) * m_no_of_pts);
+          }
+          m_header.t = backup_t;
+        }
+        if (s > 0 && s < seq.no_of_chirps - 1) {
+          dw[s + 1] = (dw[s] + dw[s - 1]) / 2;
+          if (dphi[s] - dphi[s - 1] < 0) {
+            dw[s] = dw[s - 1];
+            dphi[s] = dphi[s - 1];
+          }
+        }
+      }
+    }
+    if (seq.no_of_chirps > 1) {
+      snprintf(filename, sizeof(filename), "Chirp_%d.txt", seq_counter);
+      Output_chirps_list(filename);
+      for (int k = 0; k < no_int_states; k++) {
+        snprintf(filename, sizeof(filename), "%.3f_%d.bin", backup_end_t, k + 1);
+        std::ifstream file1(filename, std::ifstream::binary);
+        file1.seekg(sizeof(generic_header), std::ifstream::beg);
+        file1.read((char*)m_fields[k]->Getp2In(), sizeof(fftw_complex) * m_no_of_pts);
+      }
+      m_header.t = backup_end_t;
+    }
+    seq_counter++;
+  }
+}
+
+#endif

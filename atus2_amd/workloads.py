@@ -1,0 +1,77 @@
+"""Synthetic workloads of the benchmark configurations (initial states,
+potentials and step parameters), built on the host with numpy.
+
+Grids follow the reference's header conventions: dx = L/n, dk = 2 pi/L,
+x_i = (i - n/2) dx (gen_psi_0.cpp:60-88, 199-220).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+
+def axis_coords(n: int, L: float):
+    dx = L / n
+    return (np.arange(n) - n // 2) * dx
+
+
+def grid_spacings(shape, L):
+    Ls = [L] * len(shape) if np.isscalar(L) else list(L)
+    dx = [Ls[a] / shape[a] for a in range(len(shape))]
+    dk = [2 * math.pi / Ls[a] for a in range(len(shape))]
+    return dx, dk
+
+
+def particle_number(psi, dx):
+    return float(np.prod(dx) * np.sum(psi.real ** 2 + psi.imag ** 2))
+
+
+@dataclass
+class BraggWorkload:
+    """3D two-component Bragg pulse: 256^3 on [-32,32]^3, psi1 a Gaussian of
+    widths (6,4,4) normalised to 1, psi2 = 0; g11=g22=100, g12=95;
+    Omega(t) = 5 exp(-((t-0.5)/0.5)^2/2) cos(4 x - 8 t) (the coupling runs
+    along the first axis, as in Do_Bragg_ad); dt = 2e-4, Nk = 100; the Rabi
+    populations of component 0 at k = 0, +-8 are taken after every block."""
+    n: int = 256
+    L: float = 64.0
+    widths: tuple = (6.0, 4.0, 4.0)
+    gs: tuple = (100.0, 95.0, 95.0, 100.0)
+    alpha: tuple = (0.5, 0.5, 0.5)
+    amp0: float = 5.0
+    laser_k: float = 4.0
+    laser_domh: float = 8.0
+    pulse_t0: float = 0.5
+    pulse_sigma: float = 0.5
+    dt: float = 2e-4
+    Nk: int = 100
+    rabi_threshold: float = 2.0
+    shape: tuple = field(init=False)
+
+    def __post_init__(self):
+        self.shape = (self.n, self.n, self.n)
+
+    @property
+    def spacings(self):
+        return grid_spacings(self.shape, self.L)
+
+    def initial_state(self):
+        x = axis_coords(self.n, self.L)
+        wx, wy, wz = self.widths
+        gx = np.exp(-x ** 2 / (2 * wx ** 2))
+        gy = np.exp(-x ** 2 / (2 * wy ** 2))
+        gz = np.exp(-x ** 2 / (2 * wz ** 2))
+        psi1 = (gx[:, None, None] * gy[None, :, None] * gz[None, None, :]).astype(np.complex128)
+        dx, _ = self.spacings
+        psi1 /= math.sqrt(particle_number(psi1, dx))
+        psi2 = np.zeros_like(psi1)
+        return psi1, psi2
+
+    def amp(self, t):
+        return math.exp(-0.5 * ((t - self.pulse_t0) / self.pulse_sigma) ** 2)
+
+    def rabi_k0(self):
+        k = 2 * self.laser_k
+        return [(0.0, 0.0, 0.0), (k, 0.0, 0.0), (-k, 0.0, 0.0)]

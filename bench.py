@@ -1,0 +1,158 @@
+#!/usr/bin/env python3
+"""Benchmark: 3D two-component Bragg pulse, 256^3, complex128, on one MI355X.
+
+A step is one full Strang step of both components (one Bragg 2x2 step plus
+one kinetic step); the block structure is the reference's hot loop (Nk = 100
+steps per block, Rabi populations of component 0 after every block).
+Prints one JSON line with the throughput in Gpt-steps/s (10^9 grid points
+advanced by one step per second), the roofline of the dominant kernel
+(HIP-event timing on the context's stream) and a CPU baseline (the oracle,
+on a bounded sample of the same workload).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def gpu_run(args, wl):
+    from atus2_amd import hip
+    shape = wl.shape
+    dx, dk = wl.spacings
+    psi1, psi2 = wl.initial_state()
+    las = hip.laser(amp0=wl.amp0, laser_k=wl.laser_k, laser_domh=wl.laser_domh, deltaL=(0.0, 0.0))
+    ctx = hip.Context(shape, dx, dk, 2)
+    ctx.upload(0, psi1)
+    ctx.upload(1, psi2)
+    ctx.set_g(wl.gs)
+    ctx.set_alpha(wl.alpha)
+    npts = float(math.prod(shape))
+    N0 = ctx.particle_number(0) + ctx.particle_number(1)
+    t = 0.0
+    log(f"[bench] warmup {args.warmup} steps")
+    left = args.warmup
+    while left > 0:
+        nk = min(wl.Nk, left)
+        t = ctx.run_block(hip.STEP_BRAGG_AD, wl.dt, nk, t, las, wl.amp)
+        left -= nk
+    ctx.synchronize()
+    log(f"[bench] timing {args.steps} steps")
+    k0 = wl.rabi_k0()
+    rabi = None
+    t0 = time.perf_counter()
+    left = args.steps
+    while left > 0:
+        nk = min(wl.Nk, left)
+        t = ctx.run_block(hip.STEP_BRAGG_AD, wl.dt, nk, t, las, wl.amp)
+        rabi = ctx.rabi_integrals(0, k0, wl.rabi_threshold)
+        left -= nk
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    N1 = ctx.particle_number(0) + ctx.particle_number(1)
+    nsteps_done = args.warmup + args.steps
+    log(f"[bench] {elapsed:.3f} s for {args.steps} steps; norm drift {abs(N1 - N0):.2e} over {nsteps_done} steps")
+
+    # per-kernel device times of one block (HIP events on the context stream)
+    t, prof = ctx.run_block_profiled(hip.STEP_BRAGG_AD, wl.dt, wl.Nk, t, las, wl.amp)
+    fams = {k: v for k, v in prof.items() if v[1] > 0}
+    dom = max(fams, key=lambda k: fams[k][0])
+    avg_ms = fams[dom][0] / fams[dom][1]
+    # every pass family reads and writes each point of both components once
+    bytes_per_launch = 2 * npts * 32.0
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    kernels = {k: {"avg_us": 1e3 * v[0] / v[1], "launches": v[1],
+                   "GBps": bytes_per_launch / (v[0] / v[1] * 1e-3) / 1e9} for k, v in fams.items()}
+    ctx.close()
+    return {
+        "elapsed": elapsed,
+        "norm_drift_per_step": abs(N1 - N0) / nsteps_done,
+        "rabi": [float(v) for v in rabi],
+        "dominant": dom,
+        "dom_avg_ms": avg_ms,
+        "achieved": achieved,
+        "bytes_per_launch": bytes_per_launch,
+        "kernels": kernels,
+    }
+
+
+def cpu_baseline(wl, steps):
+    """The oracle on the same 256^3 Bragg workload, `steps` steps."""
+    from oracle import oracle as O
+    g = O.make_grid(wl.shape, L=wl.L)
+    psi1, psi2 = wl.initial_state()
+    S = O.RTSolver(g, [psi1, psi2], list(wl.gs), list(wl.alpha), dt=wl.dt, amp=wl.amp,
+                   beta0=0.0, deltaL=[0.0, 0.0], amp0=wl.amp0, laser_k=wl.laser_k,
+                   laser_domh=wl.laser_domh)
+    del psi1, psi2
+    t0 = time.perf_counter()
+    S.block(O.BRAGG_AD, steps)
+    el = time.perf_counter() - t0
+    npts = float(math.prod(wl.shape))
+    return {"value": npts * steps / el / 1e9, "unit": "Gpt-steps/s", "cores": O.threads(), "kind": "port",
+            "sample": f"{steps} Strang steps (one block) of the 256^3 two-component Bragg workload, "
+                      f"{el:.1f} s, radix-2 OpenMP FFT oracle"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(sharding is described in DESIGN.md); rerun with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    from atus2_amd.workloads import BraggWorkload
+    wl = BraggWorkload(n=args.n)
+    r = gpu_run(args, wl)
+    npts = float(math.prod(wl.shape))
+    value = npts * args.steps / r["elapsed"] / 1e9
+    cpu = None
+    if not args.no_cpu_baseline:
+        log(f"[bench] CPU baseline: oracle, {args.cpu_steps} steps")
+        cpu = cpu_baseline(wl, args.cpu_steps)
+        log(f"[bench] CPU baseline {cpu['value']:.4f} Gpt-steps/s on {cpu['cores']} threads")
+    out = {
+        "metric": "Gpt-steps/s + %HBM roof, 256³ 2-comp Bragg FP64; ground-state time warm vs cold",
+        "value": value,
+        "unit": "Gpt-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * r["elapsed"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"3D two-component Bragg pulse, {wl.n}^3 on [-32,32]^3, complex128, "
+                               f"dt=2e-4, Nk=100, Rabi populations per block",
+                   "grid": list(wl.shape), "components": 2, "Nk": wl.Nk},
+        "roofline": {"bound": "hbm", "achieved": r["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": r["achieved"] / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": r["dominant"], "avg_us": 1e3 * r["dom_avg_ms"],
+                     "algorithmic_bytes_per_launch": r["bytes_per_launch"]},
+        "cpu_baseline": cpu,
+        "kernels": r["kernels"],
+        "checks": {"norm_drift_per_step": r["norm_drift_per_step"], "rabi_last_block": r["rabi"]},
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

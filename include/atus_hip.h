@@ -1,0 +1,169 @@
+/*
+ * atus_hip.h -- C-ABI of libatus_hip.so, the MI355X (gfx950) implementation
+ * of the ATUS-2 split-step Fourier propagator and Sobolev-gradient
+ * minimiser.
+ *
+ * Every entry point returns ATUS_OK (0) or a negative status; the message of
+ * the last failure on the calling thread is returned by atus_last_error().
+ * No C++ exception crosses this boundary.  Pointers named `host` are host
+ * memory in the fftw_complex layout of the reference: interleaved (re, im)
+ * doubles, row-major [nx][ny][nz], z fastest (cft_base.h:209).
+ *
+ * A context owns one grid, its components (device resident, complex128),
+ * optional potentials and one HIP stream.  Calls on one context must come
+ * from one thread at a time (as the reference's solver objects, which are
+ * not thread-safe).  Grid axes must be powers of two between 2 and 4096
+ * (1 for unused axes).
+ *
+ * Each declaration names the reference interface it replaces.
+ */
+#ifndef ATUS_HIP_H
+#define ATUS_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ATUS_OK 0
+#define ATUS_ERR_ARG (-1)          /* invalid argument (sizes, comp index, ...) */
+#define ATUS_ERR_HIP (-2)          /* HIP runtime failure (message has the HIP error) */
+#define ATUS_ERR_UNSUPPORTED (-3)  /* grid shape outside what the kernels cover */
+#define ATUS_ERR_NOMEM (-4)        /* device allocation failed */
+#define ATUS_ERR_STATE (-5)        /* call not valid in the context's state */
+#define ATUS_ERR_NUMERIC (-6)      /* a reduction returned NaN or Inf */
+
+typedef struct atus_ctx atus_ctx;
+
+/* message of the last failing call on this thread ("" if none) */
+const char *atus_last_error(void);
+/* library version string and the device it runs on */
+const char *atus_version(void);
+int atus_device_info(char *name, int len, int *cu_count, double *hbm_gbps_peak);
+
+/* ---- context: replaces cft_base<dim> construction + CRT_shared::Read_header
+ * (cft_base.h:47-88, CRT_shared.h:131-172).  n[a] = nDimX/Y/Z, dx/dk as in
+ * generic_header (my_structs.h:60-65).  ncomp = no_int_states (1..4). */
+int atus_ctx_create(atus_ctx **out, int dim, const long long n[3], const double dx[3], const double dk[3],
+                    int ncomp);
+int atus_ctx_destroy(atus_ctx *ctx);
+int atus_synchronize(atus_ctx *ctx);
+
+/* ---- state: replaces the host buffers behind cft_base::Getp2In()
+ * (cft_base.h:175) and CRT_Base::LoadFiles / Save_Phi (CRT_Base.h:207-240,
+ * 607-618): copies of component `comp` between host and device. */
+int atus_upload(atus_ctx *ctx, int comp, const double *host);
+int atus_download(atus_ctx *ctx, int comp, double *host);
+/* CRT_Base::Init_Potential + Setup_Potential (CRT_Base.h:277-298):
+ * npts doubles for component comp; host == NULL removes all potentials
+ * (m_potenial_initialized == false) */
+int atus_set_potential(atus_ctx *ctx, int comp, const double *host);
+/* m_gs (CRT_Base.h:114, read at :162-169): ncomp*ncomp, row-major */
+int atus_set_g(atus_ctx *ctx, const double *gs);
+/* m_alpha (CRT_Base.h:89, read at :172-173) */
+int atus_set_alpha(atus_ctx *ctx, const double alpha[3]);
+
+/* ---- spectral field: cft_Nd::ft(isign) (cft_3d.cpp:58-74) with the
+ * scale() normalisation, in place on one component; isign not +-1 is a
+ * no-op as in the reference */
+int atus_ft(atus_ctx *ctx, int comp, int isign);
+
+/* ---- step functions of the registry m_map_stepfcts (CRT_Base.h:152-155,
+ * CRT_Base_IF.h:124-126, bragg.cpp:82-83) */
+/* Do_FT_Step_half (frac 0.5) / Do_FT_Step_full (frac 1) (CRT_Base.h:337-392) */
+int atus_kinetic(atus_ctx *ctx, double frac, double dt);
+/* CRT_Base::Do_NL_Step (CRT_Base.h:397-441), "freeprop" of rt_solver */
+int atus_nl_step(atus_ctx *ctx, double dt);
+/* CRT_Base_IF::Do_NL_Step (CRT_Base_IF.h:381-419), "freeprop" of bragg:
+ * deltaL has ncomp entries */
+int atus_nl_if_step(atus_ctx *ctx, double dt, double beta0, const double *deltaL);
+
+/* laser parameters of CRT_Base_IF (CRT_Base_IF.h:63-72, UpdateParams :145-168) */
+typedef struct {
+  double amp0;        /* Amp[0] */
+  double laser_k;     /* laser_k[0] */
+  double laser_domh;  /* laser_domh[0] */
+  double laser_dk;    /* laser_dk[0] */
+  double chirp;       /* chirp */
+  double chirp_rate0; /* chirp_rate[0] */
+  double phase0;      /* phase[0] */
+  double beta0;       /* beta[0] */
+  double deltaL[4];   /* DeltaL */
+} atus_laser;
+
+/* Bragg_single::Do_Bragg_ad (bragg.cpp:186-304), "bragg_ad": t = Get_t()
+ * before the step, F = Amplitude_at_time() (CRT_Base_IF.h:171-198) */
+int atus_bragg_ad(atus_ctx *ctx, double dt, double t, double F, const atus_laser *laser);
+/* CRT_Base::Setup_Momentum (CRT_Base.h:449-473): psi_comp *= exp(i p.x) */
+int atus_set_momentum(atus_ctx *ctx, int comp, const double p[3]);
+
+/* ---- one outer iteration of the sequence hot loop (CRT_Base.h:776-786,
+ * CRT_Base_IF.h:779-788): half kinetic, (V, full kinetic) x (Nk-1), V, half
+ * kinetic, with the reference's time bookkeeping on *t.  The kinetic steps
+ * and the V steps are fused across FFT passes on the device. */
+#define ATUS_STEP_FREEPROP 0    /* CRT_Base::Do_NL_Step */
+#define ATUS_STEP_FREEPROP_IF 1 /* CRT_Base_IF::Do_NL_Step */
+#define ATUS_STEP_BRAGG_AD 2    /* Bragg_single::Do_Bragg_ad */
+
+/* AMP_T as a callback: returns Amplitude_at_time() for the time t1 that
+ * CRT_Base_IF::Amplitude_at_time would evaluate (Get_t()+0.5*Get_dt()) */
+typedef double (*atus_amp_fn)(double t1, void *user);
+
+typedef struct {
+  int kind;            /* ATUS_STEP_* */
+  double dt;
+  atus_laser laser;    /* FREEPROP_IF uses beta0 and deltaL; BRAGG_AD all */
+  atus_amp_fn amp;     /* NULL: Amplitude_at_time() == 1 */
+  void *amp_user;
+} atus_step_params;
+
+int atus_run_block(atus_ctx *ctx, const atus_step_params *sp, int Nk, double *t);
+
+/* ---- reductions (CRT_Base.h:481-599, CRT_Base_IF.h:208-281) */
+/* Get_Particle_Number: ar * sum |psi_comp|^2 */
+int atus_particle_number(atus_ctx *ctx, int comp, double *out);
+/* Expval_Position: ar * sum x |psi|^2 (dim entries) */
+int atus_expval_x(atus_ctx *ctx, int comp, double out[3]);
+/* Expval_Momentum: ar_k * sum k |psi^|^2 (psi is left untouched) */
+int atus_expval_k(atus_ctx *ctx, int comp, double out[3]);
+/* compute_rabi_integrals: k0 holds n points of 3 doubles, n <= 8;
+ * out[q] = ar_k * sum of |psi^|^2 in slot q with the reference's slot rule */
+int atus_rabi_integrals(atus_ctx *ctx, int comp, const double *k0, int n, double threshold, double *out);
+/* energy functional (no reference counterpart):
+ * ar_k sum_i sum_k alpha.k^2 |psi_i^|^2 + ar sum_i sum_x (V_i + 1/2 sum_j g_ij |psi_j|^2) |psi_i|^2 */
+int atus_energy(atus_ctx *ctx, double *out);
+
+/* ---- Sobolev gradient minimiser (CSOB_Base.h) on the context's
+ * components (no_wf = ncomp <= 2).  gs must already carry the
+ * sqrt(N_i N_j) factor (CSOB_Base.h:177); alpha_iso[i] = Alpha_i[0]. */
+/* CSOB_Base ctor + Init (CSOB_Base.h:97-193, 229-248) */
+int atus_sob_setup(atus_ctx *ctx, const double *alpha_iso, double stepsize);
+/* Renormalize_All_Psi (CSOB_Base.h:523-543) */
+int atus_sob_renormalize(atus_ctx *ctx);
+/* one body of the do-while loop of CSOB_Base::run (CSOB_Base.h:431-503);
+ * res_tot and renormalised (0/1) are optional outputs */
+int atus_sob_iter(atus_ctx *ctx, double *res_tot, int *renormalised);
+/* Compute_mu (CSOB_Base.h:400-424, with Get_Ar() = m_ar): ncomp values */
+int atus_sob_mu(atus_ctx *ctx, double *mu);
+
+/* ---- timing helper: average device time (ms) of the last atus_run_block
+ * call's kernels, measured with HIP events on the context's stream; 0 if
+ * timing is off.  enable: 1 turns per-block event timing on. */
+int atus_set_timing(atus_ctx *ctx, int enable);
+int atus_last_block_ms(atus_ctx *ctx, double *ms);
+
+/* ---- per-kernel device times: runs one block exactly as atus_run_block,
+ * with a HIP event pair on the context's stream around every launch.
+ * ms[f] is the summed duration and count[f] the number of launches of
+ * kernel family f. */
+#define ATUS_KFAM_PASS_CONTIG 0  /* DFT pass along the contiguous axis */
+#define ATUS_KFAM_PASS_STRIDED 1 /* DFT pass along a strided axis */
+#define ATUS_KFAM_KIN 2          /* fused kinetic step along axis 0 */
+#define ATUS_KFAM_VFUSE 3        /* fused inverse / V step / forward along the last axis */
+#define ATUS_KFAM_POINT 4        /* standalone pointwise V step */
+#define ATUS_NKFAM 5
+int atus_run_block_profiled(atus_ctx *ctx, const atus_step_params *sp, int Nk, double *t, double *ms, int *count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

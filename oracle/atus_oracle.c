@@ -30,6 +30,14 @@
 
 long long orc_npts(const orc_grid *g) { return g->n[0] * g->n[1] * g->n[2]; }
 
+int orc_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
 /* CRT_shared::Read_header m_ar / m_ar_k (CRT_shared.h:142-169) */
 double orc_ar(const orc_grid *g) {
   double a = g->dx[0];

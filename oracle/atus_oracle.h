@@ -32,6 +32,7 @@ typedef struct {
 
 /* ---- spectral field (cft_base / cft_Nd) ---- */
 long long orc_npts(const orc_grid *g);
+int orc_threads(void);  /* OpenMP threads the oracle runs with */
 double orc_ar(const orc_grid *g);
 double orc_ar_k(const orc_grid *g);
 void orc_get_x(const orc_grid *g, long long l, double x[3]);

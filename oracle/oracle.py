@@ -79,7 +79,7 @@ def lib():
         P, D, I, LL = C.c_void_p, C.c_double, C.c_int, C.c_longlong
         G = C.POINTER(Grid)
         sig = {
-            "orc_npts": (LL, [G]), "orc_ar": (D, [G]), "orc_ar_k": (D, [G]),
+            "orc_npts": (LL, [G]), "orc_threads": (I, []), "orc_ar": (D, [G]), "orc_ar_k": (D, [G]),
             "orc_get_x": (None, [G, LL, P]), "orc_get_k": (None, [G, LL, P]),
             "orc_fft": (None, [G, P, I]), "orc_ft": (None, [G, P, I]),
             "orc_kinetic_table": (None, [G, P, D, D, P]),
@@ -139,6 +139,10 @@ def make_grid(shape, L=None, dx=None, dk=None) -> Grid:
 
 def grid_shape(g: Grid):
     return tuple(int(g.n[a]) for a in range(g.dim))
+
+
+def threads():
+    return lib().orc_threads()
 
 
 def ar(g):
