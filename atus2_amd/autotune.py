@@ -1,0 +1,41 @@
+"""Autotuner for the FFT kernels: picks, per kernel family, the number of
+values each thread holds (8 or 16) by timing profiled blocks of the actual
+workload on the actual context (HIP events on the context's stream).
+
+The blocks it runs advance the context's state like any other block, so it
+belongs in the warm-up of a run.  Choices are cached per (grid shape,
+components, step kind) for the life of the process.
+"""
+from __future__ import annotations
+
+from . import hip
+
+FAMILIES = ["pass_contig", "pass_strided", "kin", "vfuse"]
+_cache: dict = {}
+
+
+def autotune(ctx: "hip.Context", kind: int, dt: float, t: float, las=None, amp=None, Nk: int = 10,
+             reps: int = 2):
+    """Returns (t after the tuning blocks, {family: values_per_thread})."""
+    key = (ctx.shape, ctx.ncomp, kind)
+    if key in _cache:
+        for f, rm in _cache[key].items():
+            ctx.tune(f, rm)
+        return t, dict(_cache[key])
+    best = {f: (float("inf"), 16) for f in FAMILIES}
+    for rm in (16, 8):
+        for f in FAMILIES:
+            ctx.tune(f, rm)
+        for _ in range(reps):
+            t, prof = ctx.run_block_profiled(kind, dt, Nk, t, las, amp)
+            for f in FAMILIES:
+                ms, n = prof[f]
+                if n:
+                    avg = ms / n
+                    if avg < best[f][0]:
+                        best[f] = (avg, rm)
+    choice = {f: best[f][1] for f in FAMILIES}
+    for f, rm in choice.items():
+        ctx.tune(f, rm)
+    _cache[key] = dict(choice)
+    return t, choice

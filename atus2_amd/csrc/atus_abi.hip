@@ -97,6 +97,9 @@ struct atus_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms = 0;
 
+  // values per thread of the FFT kernels, per kernel family (ATUS_KFAM_*)
+  int rm[ATUS_NKFAM] = {16, 16, 16, 16, 16};
+
   // per-launch event pairs while profiling
   bool prof = false;
   std::vector<hipEvent_t> prof_ev;
@@ -241,8 +244,9 @@ int pass(atus_ctx* c, int a, int dir, const double2* src, double2* dst, int ncom
   const int N = c->n[a];
   ATUS_TRY(ensure_twiddles(c, N));
   const long long inner = c->inner_of(a), outer = c->outer_of(a);
-  ProfScope ps(c, inner == 1 ? ATUS_KFAM_PASS_CONTIG : ATUS_KFAM_PASS_STRIDED);
-  HIP_TRY(launch_pass(N, inner == 1, dir, src, dst, c->npts, ncomp, inner, outer, c->tw[N], scale, c->max_lds,
+  const int fam = inner == 1 ? ATUS_KFAM_PASS_CONTIG : ATUS_KFAM_PASS_STRIDED;
+  ProfScope ps(c, fam);
+  HIP_TRY(launch_pass(N, c->rm[fam], inner == 1, dir, src, dst, c->npts, ncomp, inner, outer, c->tw[N], scale, c->max_lds,
                       c->stream));
   return ATUS_OK;
 }
@@ -268,7 +272,7 @@ int kinetic_unfused(atus_ctx* c, int f) {
   VArgs va{};
   {
     ProfScope ps(c, ATUS_KFAM_KIN);
-    HIP_TRY(launch_kin(N0, c->dim == 1, V_NONE, 1, c->psi, c->npts, c->ncomp, c->inner_of(0), c->tw[N0],
+    HIP_TRY(launch_kin(N0, c->rm[ATUS_KFAM_KIN], c->dim == 1, V_NONE, 1, c->psi, c->npts, c->ncomp, c->inner_of(0), c->tw[N0],
                        kin_tabs(c, f), va, c->dx[0], c->max_lds, c->stream));
   }
   for (int a = 1; a < c->dim; ++a) ATUS_TRY(pass(c, a, +1, c->psi, c->psi, c->ncomp, 1.0));
@@ -562,7 +566,7 @@ int atus_run_block(atus_ctx* c, const atus_step_params* sp, int Nk, double* t) {
     if (c->dim == 3) ATUS_TRY(pass(c, 1, -1, c->psi, c->psi, c->ncomp, 1.0));
     {
       ProfScope ps(c, ATUS_KFAM_KIN);
-      HIP_TRY(launch_kin(N0, false, V_NONE, 1, c->psi, c->npts, c->ncomp, inner0, c->tw[N0], kin_tabs(c, 0), base,
+      HIP_TRY(launch_kin(N0, c->rm[ATUS_KFAM_KIN], false, V_NONE, 1, c->psi, c->npts, c->ncomp, inner0, c->tw[N0], kin_tabs(c, 0), base,
                          c->dx[0], c->max_lds, c->stream));
     }
     if (c->dim == 3) ATUS_TRY(pass(c, 1, +1, c->psi, c->psi, c->ncomp, 1.0));
@@ -584,20 +588,20 @@ int atus_run_block(atus_ctx* c, const atus_step_params* sp, int Nk, double* t) {
       ATUS_TRY(kinetic_unfused(c, f));
     } else if (c->dim == 1) {
       ProfScope ps(c, ATUS_KFAM_KIN);
-      HIP_TRY(launch_kin(N0, true, vk, c->ncomp, c->psi, c->npts, c->ncomp, 1, c->tw[N0], kin_tabs(c, f), va,
+      HIP_TRY(launch_kin(N0, c->rm[ATUS_KFAM_KIN], true, vk, c->ncomp, c->psi, c->npts, c->ncomp, 1, c->tw[N0], kin_tabs(c, f), va,
                          c->dx[0], c->max_lds, c->stream));
     } else {
       const long long outer = c->npts / NL;
       const long long ppx = c->dim == 3 ? c->n[1] : 1;
       {
         ProfScope ps(c, ATUS_KFAM_VFUSE);
-        HIP_TRY(launch_vfuse(NL, vk, c->ncomp, c->psi, c->npts, outer, c->tw[NL], va, ppx, c->n[0], c->dx[0],
+        HIP_TRY(launch_vfuse(NL, c->rm[ATUS_KFAM_VFUSE], vk, c->ncomp, c->psi, c->npts, outer, c->tw[NL], va, ppx, c->n[0], c->dx[0],
                              c->max_lds, c->stream));
       }
       if (c->dim == 3) ATUS_TRY(pass(c, 1, -1, c->psi, c->psi, c->ncomp, 1.0));
       {
         ProfScope ps(c, ATUS_KFAM_KIN);
-        HIP_TRY(launch_kin(N0, false, V_NONE, 1, c->psi, c->npts, c->ncomp, inner0, c->tw[N0], kin_tabs(c, f), va,
+        HIP_TRY(launch_kin(N0, c->rm[ATUS_KFAM_KIN], false, V_NONE, 1, c->psi, c->npts, c->ncomp, inner0, c->tw[N0], kin_tabs(c, f), va,
                            c->dx[0], c->max_lds, c->stream));
       }
       if (c->dim == 3) ATUS_TRY(pass(c, 1, +1, c->psi, c->psi, c->ncomp, 1.0));
@@ -721,7 +725,7 @@ int sob_filter(atus_ctx* c, int comp, const double2* src, double2* out_sob, doub
   }
   const int N0 = c->n[0];
   ATUS_TRY(ensure_twiddles(c, N0));
-  HIP_TRY(launch_sobf(N0, c->dim == 1, s, w, out_lap ? out_lap + off : nullptr, c->npts, 1, c->inner_of(0),
+  HIP_TRY(launch_sobf(N0, c->rm[ATUS_KFAM_KIN], c->dim == 1, s, w, out_lap ? out_lap + off : nullptr, c->npts, 1, c->inner_of(0),
                       c->tw[N0], sa, c->max_lds, c->stream));
   for (int a = 1; a < c->dim; ++a) {
     ATUS_TRY(pass(c, a, +1, w, w, 1, 1.0));
@@ -779,7 +783,7 @@ int atus_sob_renormalize(atus_ctx* c) {
   return ATUS_OK;
 }
 
-int atus_sob_iter(atus_ctx* c, double* res_tot, int* renormalised) {
+int atus_sob_iter(atus_ctx* c, double* res_tot, int* renormalised, double* N_out) {
   ATUS_TRY(check_ctx(c));
   if (!c->sob_ready) return fail(ATUS_ERR_STATE, "call atus_sob_setup first");
   const int nw = c->ncomp;
@@ -826,6 +830,8 @@ int atus_sob_iter(atus_ctx* c, double* res_tot, int* renormalised) {
   const double rt = sqrt(res_sq);
   if (res_tot) *res_tot = rt;
   if (renormalised) *renormalised = brenorm ? 1 : 0;
+  if (N_out)
+    for (int i = 0; i < nw; ++i) N_out[i] = N[i];
   if (!std::isfinite(rt)) return fail(ATUS_ERR_NUMERIC, "Sobolev residual is not finite");
   return ATUS_OK;
 }
@@ -874,6 +880,38 @@ int atus_run_block_profiled(atus_ctx* c, const atus_step_params* sp, int Nk, dou
   c->prof_ev.clear();
   c->prof_fam.clear();
   return rc;
+}
+
+int atus_copy_bandwidth(atus_ctx* c, int reps, double* gbps) {
+  ATUS_TRY(check_ctx(c));
+  if (reps < 1 || !gbps) return fail(ATUS_ERR_ARG, "need reps >= 1 and an output");
+  const long long n = c->npts * c->ncomp;
+  double2* dst = nullptr;
+  if (hipMalloc(&dst, sizeof(double2) * n) != hipSuccess) return fail(ATUS_ERR_NOMEM, "copy probe buffer");
+  long long nb = (n + 255) / 256;
+  if (nb > 65536) nb = 65536;
+  hipLaunchKernelGGL(k_copy, dim3((unsigned)nb), dim3(256), 0, c->stream, c->psi, dst, n);  // warm up
+  int rc = ATUS_OK;
+  if (hipEventRecord(c->ev0, c->stream) != hipSuccess) rc = fail(ATUS_ERR_HIP, "event record");
+  for (int r = 0; rc == ATUS_OK && r < reps; ++r)
+    hipLaunchKernelGGL(k_copy, dim3((unsigned)nb), dim3(256), 0, c->stream, c->psi, dst, n);
+  float ms = 0;
+  if (rc == ATUS_OK && (hipEventRecord(c->ev1, c->stream) != hipSuccess || hipEventSynchronize(c->ev1) != hipSuccess ||
+                        hipEventElapsedTime(&ms, c->ev0, c->ev1) != hipSuccess))
+    rc = fail(ATUS_ERR_HIP, "copy probe timing");
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipFree(dst);
+  if (rc != ATUS_OK) return rc;
+  *gbps = 2.0 * sizeof(double2) * (double)n * reps / (ms * 1e-3) / 1e9;
+  return ATUS_OK;
+}
+
+int atus_tune(atus_ctx* c, int family, int values_per_thread) {
+  ATUS_TRY(check_ctx(c));
+  if (family < 0 || family >= ATUS_NKFAM) return fail(ATUS_ERR_ARG, "unknown kernel family");
+  if (values_per_thread != 8 && values_per_thread != 16) return fail(ATUS_ERR_ARG, "values per thread must be 8 or 16");
+  c->rm[family] = values_per_thread;
+  return ATUS_OK;
 }
 
 int atus_set_timing(atus_ctx* c, int enable) {

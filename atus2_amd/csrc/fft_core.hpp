@@ -1,7 +1,8 @@
 // fft_core.hpp -- register/LDS Stockham FFT building blocks for gfx950.
 //
 // One pencil of N complex128 points is owned by P = N/R threads; thread t
-// holds R values in registers.  Between stages the values of a pencil are
+// holds R values in registers (R = min(RM, N), RM = 8 or 16 chosen per
+// kernel: 16 halves the LDS exchanges, 8 halves the registers).  Between stages the values of a pencil are
 // exchanged through LDS.  The layout invariant at the start of every stage is
 //     register i of thread t  <->  element t + P*i
 // which is also the layout the last stage leaves behind (natural order), so
@@ -10,7 +11,7 @@
 //   * the last stage stores straight to HBM in the same pattern, and
 //   * a forward transform can be followed by a pointwise operation and an
 //     inverse transform without any extra exchange.
-// Stage radices are R (= min(16, N)) repeated, then one smaller power of two.
+// Stage radices are R repeated, then one smaller power of two.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -96,9 +97,9 @@ __device__ __forceinline__ void dft(double2* v) {
 __host__ __device__ constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n / 2); }
 __host__ __device__ constexpr int ipow(int b, int e) { return e == 0 ? 1 : b * ipow(b, e - 1); }
 
-template <int N>
+template <int N, int RM = 16>
 struct Plan {
-  static constexpr int R = N < 16 ? N : 16;           // values per thread
+  static constexpr int R = N < RM ? N : RM;           // values per thread
   static constexpr int P = N / R;                      // threads per pencil
   static constexpr int A = ilog2c(N) / ilog2c(R);      // stages of radix R
   static constexpr int Q = N / ipow(R, A);             // radix of the tail stage (1: none)
@@ -123,9 +124,9 @@ __device__ __forceinline__ void lds_sync() {
 }
 
 // compute of stage s (twiddle + r-point DFTs); twiddle table tw[j] = e^{-2 pi i j / N}
-template <int N, int s, int DIR>
-__device__ __forceinline__ void stage_compute(double2 (&v)[Plan<N>::R], int t, const double2* __restrict__ tw) {
-  using PL = Plan<N>;
+template <int N, int RM, int s, int DIR>
+__device__ __forceinline__ void stage_compute(double2 (&v)[(Plan<N, RM>::R)], int t, const double2* __restrict__ tw) {
+  using PL = Plan<N, RM>;
   constexpr int R = PL::R, P = PL::P;
   constexpr int r = PL::template radix<s>();
   constexpr int Ls = PL::template span<s>();
@@ -153,9 +154,9 @@ __device__ __forceinline__ void stage_compute(double2 (&v)[Plan<N>::R], int t, c
 
 // scatter the outputs of stage s to their Stockham positions and gather the
 // next stage's inputs (element t + P*i into register i)
-template <int N, int s, bool WAVE_ONLY>
-__device__ __forceinline__ void stage_exchange(double2 (&v)[Plan<N>::R], int t, double2* lds) {
-  using PL = Plan<N>;
+template <int N, int RM, int s, bool WAVE_ONLY>
+__device__ __forceinline__ void stage_exchange(double2 (&v)[(Plan<N, RM>::R)], int t, double2* lds) {
+  using PL = Plan<N, RM>;
   constexpr int R = PL::R, P = PL::P;
   constexpr int r = PL::template radix<s>();
   constexpr int Ls = PL::template span<s>();
@@ -174,20 +175,22 @@ __device__ __forceinline__ void stage_exchange(double2 (&v)[Plan<N>::R], int t, 
   for (int i = 0; i < R; ++i) v[i] = lds[lds_pos(t + P * i)];
 }
 
-template <int N, int s, int DIR, bool WAVE_ONLY>
-__device__ __forceinline__ void run_stages(double2 (&v)[Plan<N>::R], int t, double2* lds, const double2* __restrict__ tw) {
-  stage_compute<N, s, DIR>(v, t, tw);
-  if constexpr (s + 1 < Plan<N>::S) {
-    stage_exchange<N, s, WAVE_ONLY>(v, t, lds);
-    run_stages<N, s + 1, DIR, WAVE_ONLY>(v, t, lds, tw);
+template <int N, int RM, int s, int DIR, bool WAVE_ONLY>
+__device__ __forceinline__ void run_stages(double2 (&v)[(Plan<N, RM>::R)], int t, double2* lds,
+                                           const double2* __restrict__ tw) {
+  stage_compute<N, RM, s, DIR>(v, t, tw);
+  if constexpr (s + 1 < Plan<N, RM>::S) {
+    stage_exchange<N, RM, s, WAVE_ONLY>(v, t, lds);
+    run_stages<N, RM, s + 1, DIR, WAVE_ONLY>(v, t, lds, tw);
   }
 }
 
 // full unnormalised N-point DFT of the pencil held in registers (layout
 // t + P*i in and out).  lds: this pencil's N-element region.
-template <int N, int DIR, bool WAVE_ONLY>
-__device__ __forceinline__ void fft_pencil(double2 (&v)[Plan<N>::R], int t, double2* lds, const double2* __restrict__ tw) {
-  if constexpr (N > 1) run_stages<N, 0, DIR, WAVE_ONLY>(v, t, lds, tw);
+template <int N, int RM, int DIR, bool WAVE_ONLY>
+__device__ __forceinline__ void fft_pencil(double2 (&v)[(Plan<N, RM>::R)], int t, double2* lds,
+                                           const double2* __restrict__ tw) {
+  if constexpr (N > 1) run_stages<N, RM, 0, DIR, WAVE_ONLY>(v, t, lds, tw);
 }
 
 }  // namespace atus

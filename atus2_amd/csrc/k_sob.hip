@@ -3,23 +3,31 @@
 
 namespace atus {
 
-template <int N>
-static hipError_t sob_n(bool contig, const double2* src, double2* o1, double2* o2, long long cs, int ncomp,
-                        long long inner, const double2* tw, const SobArgs& sa, size_t max_lds, hipStream_t s) {
-  PassCfg c = make_pass_cfg(N, contig ? 1 : inner, 1, ncomp, max_lds);
+template <int N, int RM>
+static hipError_t sob_nr(bool contig, const double2* src, double2* o1, double2* o2, long long cs, int ncomp,
+                         long long inner, const double2* tw, const SobArgs& sa, size_t max_lds, hipStream_t s) {
+  PassCfg c = make_pass_cfg(N, RM, contig ? 1 : inner, 1, ncomp, max_lds);
+  if (c.block.x == 0 || c.block.x > 256 || c.grid.x == 0) return hipErrorInvalidConfiguration;
   if (contig)
-    hipLaunchKernelGGL((k_sobf<N, true>), c.grid, c.block, c.lds, s, src, o1, o2, cs, c.g, tw, sa);
+    hipLaunchKernelGGL((k_sobf<N, RM, true>), c.grid, c.block, c.lds, s, src, o1, o2, cs, c.g, tw, sa);
   else
-    hipLaunchKernelGGL((k_sobf<N, false>), c.grid, c.block, c.lds, s, src, o1, o2, cs, c.g, tw, sa);
+    hipLaunchKernelGGL((k_sobf<N, RM, false>), c.grid, c.block, c.lds, s, src, o1, o2, cs, c.g, tw, sa);
   return hipGetLastError();
 }
 
-hipError_t launch_sobf(int N, bool contig, const double2* src, double2* out_sob, double2* out_lap,
+template <int N>
+static hipError_t sob_n(int rm, bool contig, const double2* src, double2* o1, double2* o2, long long cs, int ncomp,
+                        long long inner, const double2* tw, const SobArgs& sa, size_t max_lds, hipStream_t s) {
+  if (rm == 8 && N > 8 && N / 8 <= 256) return sob_nr<N, 8>(contig, src, o1, o2, cs, ncomp, inner, tw, sa, max_lds, s);
+  return sob_nr<N, 16>(contig, src, o1, o2, cs, ncomp, inner, tw, sa, max_lds, s);
+}
+
+hipError_t launch_sobf(int N, int rm, bool contig, const double2* src, double2* out_sob, double2* out_lap,
                        long long comp_stride, int ncomp, long long inner, const double2* tw, const SobArgs& sa,
                        size_t max_lds, hipStream_t s) {
   switch (N) {
 #define ATUS_CASE(n) \
-  case n: return sob_n<n>(contig, src, out_sob, out_lap, comp_stride, ncomp, inner, tw, sa, max_lds, s);
+  case n: return sob_n<n>(rm, contig, src, out_sob, out_lap, comp_stride, ncomp, inner, tw, sa, max_lds, s);
     ATUS_CASE(2) ATUS_CASE(4) ATUS_CASE(8) ATUS_CASE(16) ATUS_CASE(32) ATUS_CASE(64) ATUS_CASE(128)
     ATUS_CASE(256) ATUS_CASE(512) ATUS_CASE(1024) ATUS_CASE(2048) ATUS_CASE(4096)
 #undef ATUS_CASE

@@ -4,7 +4,8 @@
 // components are npts apart in one allocation.  A transform along axis a
 // sees the array as [outer][N][inner] (inner = product of the faster axes).
 //
-// Kernels (all launched with blockDim.x = T * P, P = N / R threads per pencil):
+// Kernels (all launched with blockDim.x = T * P, P = N / R threads per pencil,
+// R = min(RM, N) values per thread):
 //   k_pass   : one unnormalised DFT direction along one axis, times a scale
 //   k_kin    : forward DFT along axis 0, multiply by the kinetic phase, inverse
 //              DFT along axis 0 (the whole k-space step of a 1D grid; the
@@ -64,7 +65,7 @@ __device__ __forceinline__ long long xcd_tile(long long b, long long nb) {
   return xcd * per + (xcd < rem ? xcd : rem) + idx;
 }
 
-template <int N, bool CONTIG>
+template <int N, int RM, bool CONTIG>
 struct PencilMap {
   int t;
   int p;
@@ -73,7 +74,7 @@ struct PencilMap {
   long long stride;  // element stride along the pencil
   long long pid;     // pencil id: CONTIG -> pencil index; strided -> o*inner + col
   __device__ __forceinline__ PencilMap(const AxisGeo& g) {
-    constexpr int P = Plan<N>::P;
+    constexpr int P = Plan<N, RM>::P;
     const long long tile = xcd_tile(blockIdx.x, gridDim.x);
     const int tid = threadIdx.x;
     if constexpr (CONTIG) {
@@ -97,19 +98,19 @@ struct PencilMap {
   __device__ __forceinline__ long long elem(int j) const { return base + (long long)j * stride; }
 };
 
-template <int N>
-__device__ __forceinline__ void load_pencil(double2 (&v)[Plan<N>::R], const double2* src,
+template <int N, int RM>
+__device__ __forceinline__ void load_pencil(double2 (&v)[(Plan<N, RM>::R)], const double2* src,
                                             long long base, long long stride, int t, bool valid) {
-  constexpr int R = Plan<N>::R, P = Plan<N>::P;
+  constexpr int R = Plan<N, RM>::R, P = Plan<N, RM>::P;
 #pragma unroll
   for (int i = 0; i < R; ++i)
     v[i] = valid ? src[base + (long long)(t + P * i) * stride] : make_double2(0.0, 0.0);
 }
 
-template <int N>
-__device__ __forceinline__ void store_pencil(const double2 (&v)[Plan<N>::R], double2* dst,
+template <int N, int RM>
+__device__ __forceinline__ void store_pencil(const double2 (&v)[(Plan<N, RM>::R)], double2* dst,
                                              long long base, long long stride, int t, bool valid) {
-  constexpr int R = Plan<N>::R, P = Plan<N>::P;
+  constexpr int R = Plan<N, RM>::R, P = Plan<N, RM>::P;
   if (!valid) return;
 #pragma unroll
   for (int i = 0; i < R; ++i) dst[base + (long long)(t + P * i) * stride] = v[i];
@@ -224,23 +225,23 @@ __device__ __forceinline__ void v_point(double2 (&a)[NC], double x0, long long l
 
 // ------------------------------------------------------------ k_pass
 
-template <int N, bool CONTIG, int DIR>
+template <int N, int RM, bool CONTIG, int DIR>
 __global__ __launch_bounds__(256) void k_pass(const double2* src, double2* dst,
                                               long long comp_stride, AxisGeo g, const double2* __restrict__ tw,
                                               double scale) {
   extern __shared__ double2 lds_all[];
-  constexpr int R = Plan<N>::R;
-  PencilMap<N, CONTIG> m(g);
+  constexpr int R = Plan<N, RM>::R;
+  PencilMap<N, RM, CONTIG> m(g);
   double2* lds = lds_all + m.p * N;
   const long long coff = (long long)blockIdx.y * comp_stride;
   double2 v[R];
-  load_pencil<N>(v, src + coff, m.base, m.stride, m.t, m.valid);
-  fft_pencil<N, DIR, CONTIG && (Plan<N>::P <= 64)>(v, m.t, lds, tw);
+  load_pencil<N, RM>(v, src + coff, m.base, m.stride, m.t, m.valid);
+  fft_pencil<N, RM, DIR, CONTIG && (Plan<N, RM>::P <= 64)>(v, m.t, lds, tw);
   if (scale != 1.0) {
 #pragma unroll
     for (int i = 0; i < R; ++i) v[i] = cscale(v[i], scale);
   }
-  store_pencil<N>(v, dst + coff, m.base, m.stride, m.t, m.valid);
+  store_pencil<N, RM>(v, dst + coff, m.base, m.stride, m.t, m.valid);
 }
 
 // ------------------------------------------------------------ k_kin
@@ -256,13 +257,13 @@ struct KinTabs {
   int n2;
 };
 
-template <int N, bool CONTIG, int VK, int NC>
+template <int N, int RM, bool CONTIG, int VK, int NC>
 __global__ __launch_bounds__(256) void k_kin(double2* __restrict__ psi, long long comp_stride, AxisGeo g,
                                              const double2* __restrict__ tw, KinTabs kt, VArgs va, double x0_dx) {
   extern __shared__ double2 lds_all[];
-  constexpr int R = Plan<N>::R, P = Plan<N>::P;
+  constexpr int R = Plan<N, RM>::R, P = Plan<N, RM>::P;
   constexpr bool WAVE = CONTIG && (P <= 64);
-  PencilMap<N, CONTIG> m(g);
+  PencilMap<N, RM, CONTIG> m(g);
   double2* lds = lds_all + m.p * N;
   // pencil constant of the phase (axes 1, 2)
   double2 cp = make_double2(1.0, 0.0);
@@ -277,7 +278,7 @@ __global__ __launch_bounds__(256) void k_kin(double2* __restrict__ psi, long lon
   double2 v[NC][R];
   const long long c0 = (NC == 1) ? (long long)blockIdx.y * comp_stride : 0;
 #pragma unroll
-  for (int c = 0; c < NC; ++c) load_pencil<N>(v[c], psi + c0 + c * comp_stride, m.base, m.stride, m.t, m.valid);
+  for (int c = 0; c < NC; ++c) load_pencil<N, RM>(v[c], psi + c0 + c * comp_stride, m.base, m.stride, m.t, m.valid);
   if constexpr (VK != V_NONE) {
     // 1D: the pencil is axis 0 itself, x0 = (j - n0/2) dx
 #pragma unroll
@@ -297,15 +298,15 @@ __global__ __launch_bounds__(256) void k_kin(double2* __restrict__ psi, long lon
   }
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
-    fft_pencil<N, -1, WAVE>(v[c], m.t, lds, tw);
+    fft_pencil<N, RM, -1, WAVE>(v[c], m.t, lds, tw);
 #pragma unroll
     for (int i = 0; i < R; ++i) {
       double2 e = kt.e0[m.t + P * i];
       if constexpr (!CONTIG) e = cmul(e, cp);
       v[c][i] = cmul(v[c][i], e);
     }
-    fft_pencil<N, +1, WAVE>(v[c], m.t, lds, tw);
-    store_pencil<N>(v[c], psi + c0 + c * comp_stride, m.base, m.stride, m.t, m.valid);
+    fft_pencil<N, RM, +1, WAVE>(v[c], m.t, lds, tw);
+    store_pencil<N, RM>(v[c], psi + c0 + c * comp_stride, m.base, m.stride, m.t, m.valid);
   }
 }
 
@@ -313,14 +314,14 @@ __global__ __launch_bounds__(256) void k_kin(double2* __restrict__ psi, long lon
 // inverse DFT along the last axis, V step, forward DFT along the last axis.
 // The pencil (contiguous) has fixed coordinates on the other axes, so
 // x0 and the Bragg field are per-pencil constants.
-template <int N, int VK, int NC>
+template <int N, int RM, int VK, int NC>
 __global__ __launch_bounds__(256) void k_vfuse(double2* __restrict__ psi, long long comp_stride, AxisGeo g,
                                                const double2* __restrict__ tw, VArgs va, long long pencils_per_x0,
                                                int n0, double dx0) {
   extern __shared__ double2 lds_all[];
-  constexpr int R = Plan<N>::R, P = Plan<N>::P;
+  constexpr int R = Plan<N, RM>::R, P = Plan<N, RM>::P;
   constexpr bool WAVE = P <= 64;
-  PencilMap<N, true> m(g);
+  PencilMap<N, RM, true> m(g);
   double2* lds = lds_all + m.p * N;
   const int i0 = (int)(m.pid / pencils_per_x0);
   const double x0 = (double)(i0 - n0 / 2) * dx0;
@@ -330,8 +331,8 @@ __global__ __launch_bounds__(256) void k_vfuse(double2* __restrict__ psi, long l
   double2 v[NC][R];
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
-    load_pencil<N>(v[c], psi + c * comp_stride, m.base, 1, m.t, m.valid);
-    fft_pencil<N, +1, WAVE>(v[c], m.t, lds, tw);
+    load_pencil<N, RM>(v[c], psi + c * comp_stride, m.base, 1, m.t, m.valid);
+    fft_pencil<N, RM, +1, WAVE>(v[c], m.t, lds, tw);
   }
 #pragma unroll
   for (int i = 0; i < R; ++i) {
@@ -344,8 +345,8 @@ __global__ __launch_bounds__(256) void k_vfuse(double2* __restrict__ psi, long l
   }
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
-    fft_pencil<N, -1, WAVE>(v[c], m.t, lds, tw);
-    store_pencil<N>(v[c], psi + c * comp_stride, m.base, 1, m.t, m.valid);
+    fft_pencil<N, RM, -1, WAVE>(v[c], m.t, lds, tw);
+    store_pencil<N, RM>(v[c], psi + c * comp_stride, m.base, 1, m.t, m.valid);
   }
 }
 
@@ -368,13 +369,13 @@ __device__ __forceinline__ double kval(int i, int n, double dk) {
   return dk * (double)(((i + s) % n) - s);
 }
 
-template <int N, bool CONTIG>
+template <int N, int RM, bool CONTIG>
 __global__ __launch_bounds__(256) void k_sobf(const double2* src, double2* out_sob, double2* out_lap, long long comp_stride, AxisGeo g,
                                               const double2* __restrict__ tw, SobArgs sa) {
   extern __shared__ double2 lds_all[];
-  constexpr int R = Plan<N>::R, P = Plan<N>::P;
+  constexpr int R = Plan<N, RM>::R, P = Plan<N, RM>::P;
   constexpr bool WAVE = CONTIG && (P <= 64);
-  PencilMap<N, CONTIG> m(g);
+  PencilMap<N, RM, CONTIG> m(g);
   double2* lds = lds_all + m.p * N;
   const long long coff = (long long)blockIdx.y * comp_stride;
   double k12 = 0.0;  // sum over axes 1, 2 of k^2 (k.scale(alpha).k with isotropic alpha)
@@ -392,8 +393,8 @@ __global__ __launch_bounds__(256) void k_sobf(const double2* src, double2* out_s
     }
   }
   double2 v[R];
-  load_pencil<N>(v, src + coff, m.base, m.stride, m.t, m.valid);
-  fft_pencil<N, -1, WAVE>(v, m.t, lds, tw);
+  load_pencil<N, RM>(v, src + coff, m.base, m.stride, m.t, m.valid);
+  fft_pencil<N, RM, -1, WAVE>(v, m.t, lds, tw);
   double2 w[R];
 #pragma unroll
   for (int i = 0; i < R; ++i) {
@@ -402,11 +403,11 @@ __global__ __launch_bounds__(256) void k_sobf(const double2* src, double2* out_s
     w[i] = cscale(v[i], -phi * sa.scale);
     v[i] = cscale(v[i], sa.scale / (1.0 + phi));
   }
-  fft_pencil<N, +1, WAVE>(v, m.t, lds, tw);
-  store_pencil<N>(v, out_sob + coff, m.base, m.stride, m.t, m.valid);
+  fft_pencil<N, RM, +1, WAVE>(v, m.t, lds, tw);
+  store_pencil<N, RM>(v, out_sob + coff, m.base, m.stride, m.t, m.valid);
   if (out_lap) {
-    fft_pencil<N, +1, WAVE>(w, m.t, lds, tw);
-    store_pencil<N>(w, out_lap + coff, m.base, m.stride, m.t, m.valid);
+    fft_pencil<N, RM, +1, WAVE>(w, m.t, lds, tw);
+    store_pencil<N, RM>(w, out_lap + coff, m.base, m.stride, m.t, m.valid);
   }
 }
 

@@ -41,6 +41,12 @@ __global__ __launch_bounds__(256) void k_mul_phase(double2* __restrict__ psi, lo
   }
 }
 
+// streaming copy (bandwidth probe)
+__global__ __launch_bounds__(256) void k_copy(const double2* __restrict__ src, double2* __restrict__ dst, long long n) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long l = (long long)blockIdx.x * blockDim.x + threadIdx.x; l < n; l += stride) dst[l] = src[l];
+}
+
 // ------------------------------------------------------------ Sobolev pointwise
 
 // L2 gradient -Lap psi + (V + sum_j g_ij |psi_j|^2) psi  (CSOB_Base.h:322-327)

@@ -15,30 +15,33 @@ struct PassCfg {
   AxisGeo g;
 };
 
-inline int threads_per_pencil(int N) { return N / (N < 16 ? N : 16); }
+// values per thread R = min(rm, N); threads per pencil P = N / R
+inline int threads_per_pencil(int N, int rm) { return N / (N < rm ? N : rm); }
 
 // contiguous pencils if inner == 1; the pass covers `outer` slabs of
 // [N][inner]; ycount = gridDim.y (components)
-PassCfg make_pass_cfg(int N, long long inner, long long outer, int ycount, size_t max_lds, int max_threads = 256);
+PassCfg make_pass_cfg(int N, int rm, long long inner, long long outer, int ycount, size_t max_lds,
+                      int max_threads = 256);
 
 bool supported_len(int N);  // power of two in [2, 4096]
 
-hipError_t launch_pass(int N, bool contig, int dir, const double2* src, double2* dst, long long comp_stride,
+// rm: values per thread (8 or 16) of the FFT kernels
+hipError_t launch_pass(int N, int rm, bool contig, int dir, const double2* src, double2* dst, long long comp_stride,
                        int ncomp, long long inner, long long outer, const double2* tw, double scale,
                        size_t max_lds, hipStream_t s);
 
 // kinetic step along axis 0; vk/nc select the fused V prologue (1D only)
-hipError_t launch_kin(int N, bool contig, int vk, int nc, double2* psi, long long comp_stride, int ncomp,
+hipError_t launch_kin(int N, int rm, bool contig, int vk, int nc, double2* psi, long long comp_stride, int ncomp,
                       long long inner, const double2* tw, const KinTabs& kt, const VArgs& va, double dx0,
                       size_t max_lds, hipStream_t s);
 
 // inverse / V / forward along the last (contiguous) axis of a 2D/3D grid
-hipError_t launch_vfuse(int N, int vk, int nc, double2* psi, long long comp_stride, long long outer,
+hipError_t launch_vfuse(int N, int rm, int vk, int nc, double2* psi, long long comp_stride, long long outer,
                         const double2* tw, const VArgs& va, long long pencils_per_x0, int n0, double dx0,
                         size_t max_lds, hipStream_t s);
 
 // Sobolev filters along axis 0
-hipError_t launch_sobf(int N, bool contig, const double2* src, double2* out_sob, double2* out_lap,
+hipError_t launch_sobf(int N, int rm, bool contig, const double2* src, double2* out_sob, double2* out_lap,
                        long long comp_stride, int ncomp, long long inner, const double2* tw, const SobArgs& sa,
                        size_t max_lds, hipStream_t s);
 

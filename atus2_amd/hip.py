@@ -22,7 +22,7 @@ EXPORTED = [
     "atus_set_alpha", "atus_ft", "atus_kinetic", "atus_nl_step", "atus_nl_if_step", "atus_bragg_ad",
     "atus_set_momentum", "atus_run_block", "atus_particle_number", "atus_expval_x", "atus_expval_k",
     "atus_rabi_integrals", "atus_energy", "atus_sob_setup", "atus_sob_renormalize", "atus_sob_iter",
-    "atus_sob_mu", "atus_set_timing", "atus_last_block_ms", "atus_run_block_profiled",
+    "atus_sob_mu", "atus_set_timing", "atus_last_block_ms", "atus_run_block_profiled", "atus_copy_bandwidth", "atus_tune",
 ]
 
 KFAM = ["pass_contig", "pass_strided", "kin", "vfuse", "point"]
@@ -86,12 +86,14 @@ def load():
         "atus_energy": (I, [P, C.POINTER(D)]),
         "atus_sob_setup": (I, [P, P, D]),
         "atus_sob_renormalize": (I, [P]),
-        "atus_sob_iter": (I, [P, C.POINTER(D), C.POINTER(I)]),
+        "atus_sob_iter": (I, [P, C.POINTER(D), C.POINTER(I), C.POINTER(D)]),
         "atus_sob_mu": (I, [P, P]),
         "atus_set_timing": (I, [P, I]),
         "atus_last_block_ms": (I, [P, C.POINTER(D)]),
         "atus_run_block_profiled": (I, [P, C.POINTER(StepParams), I, C.POINTER(D), C.POINTER(D),
                                         C.POINTER(I)]),
+        "atus_copy_bandwidth": (I, [P, I, C.POINTER(D)]),
+        "atus_tune": (I, [P, I, I]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -282,10 +284,12 @@ class Context:
         _check(_lib.atus_sob_renormalize(self._h))
 
     def sob_iter(self):
+        """One Sobolev iteration: (res_tot, renormalised, particle numbers)."""
         r = C.c_double()
         rn = C.c_int()
-        _check(_lib.atus_sob_iter(self._h, C.byref(r), C.byref(rn)))
-        return r.value, bool(rn.value)
+        N = (C.c_double * 4)()
+        _check(_lib.atus_sob_iter(self._h, C.byref(r), C.byref(rn), N))
+        return r.value, bool(rn.value), list(N[:self.ncomp])
 
     def sob_mu(self):
         out = np.zeros(self.ncomp)
@@ -303,6 +307,17 @@ class Context:
 
     def synchronize(self):
         _check(_lib.atus_synchronize(self._h))
+
+    def tune(self, family, values_per_thread):
+        """family: index or name in KFAM; values_per_thread: 8 or 16."""
+        f = KFAM.index(family) if isinstance(family, str) else int(family)
+        _check(_lib.atus_tune(self._h, f, int(values_per_thread)))
+
+    def copy_bandwidth(self, reps=20):
+        """GB/s of a plain streaming copy of all components (HIP events)."""
+        out = C.c_double()
+        _check(_lib.atus_copy_bandwidth(self._h, int(reps), C.byref(out)))
+        return out.value
 
 
 def device_info():

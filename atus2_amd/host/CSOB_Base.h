@@ -165,10 +165,12 @@ void CSOB_Base<T, dim, no_wf>::run() {
   int counter = 0;
   do {
     int renorm = 0;
-    atus_host::check(atus_sob_iter(m_set->device(), &m_res_tot, &renorm), "atus_sob_iter");
+    double N[2] = {0, 0};
+    atus_host::check(atus_sob_iter(m_set->device(), &m_res_tot, &renorm, N), "atus_sob_iter");
     for (int i = 0; i < no_wf; i++) {
       // the particle numbers the iteration checked, before its update
-      std::cout << "N[" << i << "] = " << (renorm ? m_N[i] : 1.0) << std::endl;
+      m_N[i] = renorm ? 1.0 : N[i];
+      std::cout << "N[" << i << "] = " << N[i] << std::endl;
     }
     printf("brenorm = %s\n", renorm ? "true" : "false");
     std::cout << "--- " << counter << std::endl;

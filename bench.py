@@ -40,6 +40,11 @@ def gpu_run(args, wl):
     npts = float(math.prod(shape))
     N0 = ctx.particle_number(0) + ctx.particle_number(1)
     t = 0.0
+    choice = {}
+    if not args.no_autotune:
+        from atus2_amd.autotune import autotune
+        t, choice = autotune(ctx, hip.STEP_BRAGG_AD, wl.dt, t, las, wl.amp, Nk=10, reps=2)
+        log(f"[bench] autotuned values per thread: {choice}")
     log(f"[bench] warmup {args.warmup} steps")
     left = args.warmup
     while left > 0:
@@ -73,6 +78,8 @@ def gpu_run(args, wl):
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     kernels = {k: {"avg_us": 1e3 * v[0] / v[1], "launches": v[1],
                    "GBps": bytes_per_launch / (v[0] / v[1] * 1e-3) / 1e9} for k, v in fams.items()}
+    copy_gbps = ctx.copy_bandwidth(20)
+    log(f"[bench] copy probe {copy_gbps:.0f} GB/s; dominant kernel {dom} {achieved:.0f} GB/s")
     ctx.close()
     return {
         "elapsed": elapsed,
@@ -83,6 +90,8 @@ def gpu_run(args, wl):
         "achieved": achieved,
         "bytes_per_launch": bytes_per_launch,
         "kernels": kernels,
+        "copy_gbps": copy_gbps,
+        "tuning": choice,
     }
 
 
@@ -112,6 +121,7 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -146,9 +156,11 @@ def main():
         "roofline": {"bound": "hbm", "achieved": r["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": r["achieved"] / HBM_PEAK_GBS, "traffic": None,
                      "kernel": r["dominant"], "avg_us": 1e3 * r["dom_avg_ms"],
-                     "algorithmic_bytes_per_launch": r["bytes_per_launch"]},
+                     "algorithmic_bytes_per_launch": r["bytes_per_launch"],
+                     "measured_copy_GBps": r["copy_gbps"]},
         "cpu_baseline": cpu,
         "kernels": r["kernels"],
+        "tuning": r["tuning"],
         "checks": {"norm_drift_per_step": r["norm_drift_per_step"], "rabi_last_block": r["rabi"]},
     }
     print(json.dumps(out))

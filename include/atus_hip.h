@@ -140,8 +140,9 @@ int atus_sob_setup(atus_ctx *ctx, const double *alpha_iso, double stepsize);
 /* Renormalize_All_Psi (CSOB_Base.h:523-543) */
 int atus_sob_renormalize(atus_ctx *ctx);
 /* one body of the do-while loop of CSOB_Base::run (CSOB_Base.h:431-503);
- * res_tot and renormalised (0/1) are optional outputs */
-int atus_sob_iter(atus_ctx *ctx, double *res_tot, int *renormalised);
+ * optional outputs: res_tot, renormalised (0/1) and N (ncomp particle
+ * numbers checked before the update, CSOB_Base.h:470-474) */
+int atus_sob_iter(atus_ctx *ctx, double *res_tot, int *renormalised, double *N);
 /* Compute_mu (CSOB_Base.h:400-424, with Get_Ar() = m_ar): ncomp values */
 int atus_sob_mu(atus_ctx *ctx, double *mu);
 
@@ -162,6 +163,17 @@ int atus_last_block_ms(atus_ctx *ctx, double *ms);
 #define ATUS_KFAM_POINT 4        /* standalone pointwise V step */
 #define ATUS_NKFAM 5
 int atus_run_block_profiled(atus_ctx *ctx, const atus_step_params *sp, int Nk, double *t, double *ms, int *count);
+
+/* tuning: values per thread (8 or 16) of the FFT kernels of one kernel
+ * family (ATUS_KFAM_*; the Sobolev filter follows ATUS_KFAM_KIN).  16 halves
+ * the LDS exchanges, 8 halves the registers per thread.  Default 16. */
+int atus_tune(atus_ctx *ctx, int family, int values_per_thread);
+
+/* HBM copy probe: copies all components (ncomp * npts complex128) into a
+ * scratch buffer `reps` times with a plain streaming kernel and returns the
+ * achieved GB/s (bytes read + written per second, HIP events).  Measures
+ * the copy peak the kernels are compared with; no reference counterpart. */
+int atus_copy_bandwidth(atus_ctx *ctx, int reps, double *gbps);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,132 @@
+"""The rebuilt drivers end to end on the GPU: an XML file and .bin inputs in,
+.bin / Rabi_*.txt out, compared with the oracle run from the same files."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from test_host import read_bin, tool
+
+pytestmark = pytest.mark.gpu
+
+
+def write_bin(path, psi, L, dt=0.001, t=0.0):
+    """A .bin file as gen_psi_0 writes it (my_structs.h:40-72)."""
+    import struct
+    shape = psi.shape
+    dim = len(shape)
+    n = list(shape) + [1] * (3 - dim)
+    dx = [L[a] / shape[a] for a in range(dim)] + [0.0] * (3 - dim)
+    dk = [2 * np.pi / L[a] for a in range(dim)] + [0.0] * (3 - dim)
+    mins = [-L[a] / 2 for a in range(dim)] + [-10.0] * (3 - dim)
+    maxs = [L[a] / 2 for a in range(dim)] + [10.0] * (3 - dim)
+    head = struct.pack("<7q2i14d2i", 1380, 16, 0, dim, *n, 1, 1, t, mins[0], maxs[0], mins[1], maxs[1],
+                       mins[2], maxs[2], *dx, *dk, dt, 0, 0)
+    head += b"\0" * (1380 - len(head))
+    with open(path, "wb") as f:
+        f.write(head)
+        f.write(np.ascontiguousarray(psi, dtype=np.complex128).tobytes())
+
+
+def rel_l2(a, b):
+    return np.linalg.norm(np.ravel(a) - np.ravel(b)) / np.linalg.norm(np.ravel(b))
+
+
+def test_rt_solver_1d_full_run(orc, tmp_path):
+    # the 1D rt_solver workload (N=4096 on [-40,40], g=10, V=x^2/2, 1000 steps)
+    n, L = 4096, 80.0
+    g = orc.make_grid((n,), L=L)
+    (x,) = orc.coords_x(g)
+    rng = np.random.default_rng(1)
+    psi = np.pi ** -0.25 * np.exp(-x ** 2 / 2) + 1e-3 * (rng.standard_normal(n) + 1j * rng.standard_normal(n))
+    psi /= np.sqrt(orc.particle_number(g, psi))
+    write_bin(tmp_path / "psi0.bin", psi, [L])
+    (tmp_path / "rt.xml").write_text("""<SIMULATION><DIM>1</DIM><FILENAME>psi0.bin</FILENAME>
+      <POTENTIAL_1D>0.5*x^2</POTENTIAL_1D>
+      <VCONSTANTS><GS_1>10</GS_1><Alpha_1>0.5</Alpha_1></VCONSTANTS>
+      <ALGORITHM><DT>0.001</DT></ALGORITHM>
+      <SEQUENCE><freeprop dt="0.001" Nk="100" pn_freq="each" output_freq="last">1.0</freeprop></SEQUENCE>
+      </SIMULATION>""")
+    r = subprocess.run([tool("rt_solver"), "rt.xml"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    # pn_freq="each" prints N after every block and nothing more at the end
+    # (CRT_Base.h:807-832)
+    assert "FYI: Na          : 10" in r.stdout and r.stdout.count("N[0] = ") == 10
+    h, out = read_bin(tmp_path / "1.000_1.bin")
+    assert abs(h["t"] - 1.0) < 1e-12 and h["dt"] == 0.001
+    S = orc.RTSolver(g, [psi], [10.0], [0.5], dt=1e-3, pots=[0.5 * x ** 2])
+    for _ in range(10):
+        S.block(orc.FREEPROP, 100)
+    assert rel_l2(out, S.psis[0]) <= 1e-10
+
+
+def test_bragg_3d_pulse_with_rabi_output(orc, tmp_path):
+    # the 3D two-component Bragg workload on a 64^3 proxy of the 256^3 grid
+    n, L = 64, 64.0
+    g = orc.make_grid((n, n, n), L=L)
+    x, y, z = orc.coords_x(g)
+    psi1 = (np.exp(-x ** 2 / 72 - y ** 2 / 32 - z ** 2 / 32) * np.ones((n, n, n))).astype(np.complex128)
+    psi1 /= np.sqrt(orc.particle_number(g, psi1))
+    psi2 = np.zeros_like(psi1)
+    write_bin(tmp_path / "a.bin", psi1, [L] * 3)
+    write_bin(tmp_path / "b.bin", psi2, [L] * 3)
+    (tmp_path / "br.xml").write_text("""<SIMULATION><DIM>3</DIM>
+      <FILENAME>a.bin</FILENAME><FILENAME_2>b.bin</FILENAME_2>
+      <AMP_T>exp(-0.5*((t-0.5)/0.5)^2)</AMP_T>
+      <CONSTANTS><laser_k>4</laser_k><laser_domh>8</laser_domh><laser_dk>0</laser_dk>
+        <rabi_threshold>2</rabi_threshold><chirp>0</chirp></CONSTANTS>
+      <VCONSTANTS><GS_1>100,95</GS_1><GS_2>95,100</GS_2><Alpha_1>0.5,0.5,0.5</Alpha_1>
+        <Beta>0,0,0</Beta><Delta_L>0,0</Delta_L><Amp_1>5,5</Amp_1></VCONSTANTS>
+      <ALGORITHM><DT>2e-4</DT></ALGORITHM>
+      <SEQUENCE><bragg_ad dt="2e-4" Nk="100" rabi_output_freq="each" output_freq="last">0.04</bragg_ad></SEQUENCE>
+      </SIMULATION>""")
+    r = subprocess.run([tool("bragg"), "br.xml"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    amp = lambda t: float(np.exp(-0.5 * ((t - 0.5) / 0.5) ** 2))
+    S = orc.RTSolver(g, [psi1, psi2], [100., 95., 95., 100.], [0.5] * 3, dt=2e-4, amp=amp, deltaL=[0., 0.],
+                     amp0=5.0, laser_k=4.0, laser_domh=8.0)
+    rabi_ref = []
+    for _ in range(2):
+        S.block(orc.BRAGG_AD, 100)
+        rb, _ = orc.rabi_integrals(g, S.psis[0], [(0, 0, 0), (8, 0, 0), (-8, 0, 0)], 2.0)
+        rabi_ref.append(rb)
+    outs = [read_bin(tmp_path / f"0.040_{c}.bin")[1].reshape(n, n, n) for c in (1, 2)]
+    err = rel_l2(np.concatenate([o.ravel() for o in outs]), np.concatenate([s_.ravel() for s_ in S.psis]))
+    assert err < 1e-11
+    lines = (tmp_path / "Rabi_1_0.txt").read_text().splitlines()
+    assert lines[0].startswith("# time \t0,0,0\t8,0,0\t-8,0,0")
+    rows = [[float(v) for v in ln.split("\t") if v] for ln in lines[1:]]
+    assert [r_[0] for r_ in rows] == pytest.approx([0.0, 0.02])
+    for r_, ref in zip(rows, rabi_ref):
+        assert np.allclose(r_[1:], ref, rtol=1e-5, atol=1e-9)   # 6 significant digits in the file
+
+
+def test_sobmin_3d_ground_state(orc, tmp_path):
+    n = 32
+    (tmp_path / "gs.xml").write_text(f"""<SIMULATION><DIM>3</DIM><FILENAME>gs.bin</FILENAME>
+      <GUESS_3D>exp(-0.25*(x^2+y^2+z^2))</GUESS_3D>
+      <POTENTIAL_3D>0.5*(x^2+1.5^2*y^2+0.75^2*z^2)</POTENTIAL_3D>
+      <CONSTANTS><epsilon>1e-18</epsilon><stepsize>0.02</stepsize></CONSTANTS>
+      <VCONSTANTS><Alpha_1>0.5</Alpha_1><GS_1>5</GS_1><N>2</N></VCONSTANTS>
+      <ALGORITHM><NX>{n}</NX><NY>{n}</NY><NZ>{n}</NZ><XMIN>-8</XMIN><XMAX>8</XMAX>
+        <YMIN>-8</YMIN><YMAX>8</YMAX><ZMIN>-8</ZMIN><ZMAX>8</ZMAX></ALGORITHM></SIMULATION>""")
+    env = dict(os.environ, ATUS_SOB_MAXITER="3000")
+    r = subprocess.run([tool("sobmin"), "gs.xml"], cwd=tmp_path, capture_output=True, text=True, env=env,
+                       timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr
+    iters = r.stdout.count("--- ")
+    h, out = read_bin(tmp_path / "gs.bin")
+    assert h["nDimX"] == n and (tmp_path / "zero.bin").exists()
+    # the oracle from the same guess, same number of iterations
+    g = orc.make_grid((n, n, n), L=16.0)
+    x, y, z = orc.coords_x(g)
+    V = 0.5 * (x ** 2 + 2.25 * y ** 2 + 0.5625 * z ** 2) * np.ones((n, n, n))
+    guess = (np.exp(-0.25 * (x ** 2 + y ** 2 + z ** 2)) * np.ones((n, n, n))).astype(np.complex128)
+    S = orc.SobSolver(g, [guess], [V], [5.0 * 2.0], [0.5], stepsize=0.02)
+    S.renormalize()
+    for _ in range(iters):
+        res = S.iterate()
+    ref = S.psi[0] * np.sqrt(2.0)             # Save(..., rescale = true) multiplies by sqrt(N)
+    assert rel_l2(out.reshape(n, n, n), ref) < 1e-9
+    assert res < 1e-18 or iters == 3000

@@ -32,11 +32,14 @@ SHAPES = [(2,), (64,), (4096,), (16, 32), (128, 8), (2048, 4), (8, 16, 32), (32,
           (64, 4, 512)]
 
 
+@pytest.mark.parametrize("rm", [16, 8])
 @pytest.mark.parametrize("shape", SHAPES)
-def test_ft_matches_oracle(hip, orc, shape):
+def test_ft_matches_oracle(hip, orc, shape, rm):
     g = orc.make_grid(shape, L=[11.0, 13.0, 17.0][:len(shape)])
     psi = rand_field(shape, 10)
     with hip.Context.from_grid(g, 1) as ctx:
+        for f in hip.KFAM:
+            ctx.tune(f, rm)
         ctx.upload(0, psi)
         ctx.ft(0, -1)
         fwd = ctx.download(0)
@@ -50,7 +53,8 @@ def test_ft_matches_oracle(hip, orc, shape):
 
 @pytest.mark.parametrize("shape", [(64,), (4096,), (32, 64), (8, 16, 32), (64, 64, 64)])
 @pytest.mark.parametrize("frac", [0.5, 1.0])
-def test_kinetic_matches_oracle(hip, orc, shape, frac):
+@pytest.mark.parametrize("rm", [16, 8])
+def test_kinetic_matches_oracle(hip, orc, shape, frac, rm):
     dim = len(shape)
     g = orc.make_grid(shape, L=[20.0, 24.0, 28.0][:dim])
     alpha = [0.5, 0.3, 0.7][:dim]
@@ -59,6 +63,8 @@ def test_kinetic_matches_oracle(hip, orc, shape, frac):
     tab = orc.kinetic_table(g, alpha, dt, frac)
     ref = orc.ft_step(g, psis, tab)
     with hip.Context.from_grid(g, 2) as ctx:
+        for f in hip.KFAM:
+            ctx.tune(f, rm)
         for c in range(2):
             ctx.upload(c, psis[c])
         ctx.set_alpha(alpha)
@@ -112,9 +118,10 @@ def _state(orc, g, seed, ncomp):
     return out
 
 
-@pytest.mark.parametrize("shape", [(4096,), (256,), (64, 128), (32, 16, 64)])
+@pytest.mark.parametrize("shape,rm", [((4096,), 16), ((256,), 8), ((64, 128), 16), ((32, 16, 64), 16),
+                                      ((32, 16, 64), 8)])
 @pytest.mark.parametrize("kind", ["freeprop", "freeprop_if", "bragg_ad"])
-def test_run_block_matches_oracle(hip, orc, shape, kind):
+def test_run_block_matches_oracle(hip, orc, shape, rm, kind):
     dim = len(shape)
     g = orc.make_grid(shape, L=[40.0, 30.0, 36.0][:dim])
     ncomp = 1 if kind == "freeprop" else 2
@@ -156,6 +163,8 @@ def test_run_block_matches_oracle(hip, orc, shape, kind):
         pert = [p * (1 + 2e-16 * rng.standard_normal(p.shape)) for p in psis]
         Sp = orc.RTSolver(g, pert, gs, alpha, dt=dt, t=t0, pots=pots, amp=amp, **dict(ifp))
     with hip.Context.from_grid(g, ncomp) as ctx:
+        for f in hip.KFAM:
+            ctx.tune(f, rm)
         for c in range(ncomp):
             ctx.upload(c, psis[c])
         ctx.set_g(gs)
@@ -263,9 +272,62 @@ def test_sobolev_iterations_match_oracle(hip, orc, shape):
         ctx.sob_renormalize()
         for it in range(30):
             r_ref = S.iterate()
-            r, rn = ctx.sob_iter()
+            r, rn, Ns = ctx.sob_iter()
+            assert abs(Ns[0] - S.s.N[0]) < 1e-13 or rn
             assert abs(r / r_ref - 1) < 1e-10, (it, r, r_ref)
             assert rn == bool(S.s.renormalised)
         assert rel_l2(ctx.download(0), S.psi[0]) < 1e-12
         mu = ctx.sob_mu()[0]
         assert abs(mu / S.mu()[0] - 1) < 1e-12
+
+
+@pytest.mark.parametrize("name", ["rt1d", "if2d", "bragg3d"])
+def test_hip_matches_golden_fixtures(hip, name):
+    from golden_cases import load, rel_l2 as rl2
+    c = load(name)
+    ncomp = len(c["psi0"])
+    dx = [c["L"][a] / c["shape"][a] for a in range(len(c["shape"]))]
+    dk = [2 * np.pi / c["L"][a] for a in range(len(c["shape"]))]
+    kind = {"freeprop": hip.STEP_FREEPROP, "freeprop_if": hip.STEP_FREEPROP_IF,
+            "bragg_ad": hip.STEP_BRAGG_AD}[c["kind"]]
+    ifp = dict(c["ifp"])
+    las = hip.laser(**{k: v for k, v in ifp.items() if k != "deltaL"},
+                    deltaL=tuple(ifp.get("deltaL", (0.0, 0.0))))
+    with hip.Context(c["shape"], dx, dk, ncomp) as ctx:
+        for i, p in enumerate(c["psi0"]):
+            ctx.upload(i, p)
+        ctx.set_g(c["gs"])
+        ctx.set_alpha(c["alpha"])
+        if c["pots"] is not None:
+            ctx.set_potential(0, c["pots"][0])
+        t = c["t0"]
+        for b in range(c["nblocks"]):
+            t = ctx.run_block(kind, c["dt"], c["Nk"], t, las, c["amp"])
+            norms = [ctx.particle_number(i) for i in range(ncomp)]
+            assert np.allclose(norms, c["norms"][b], rtol=0, atol=1e-13)
+        out = [ctx.download(i) for i in range(ncomp)]
+    assert t == pytest.approx(c["t"], abs=1e-15)
+    assert rl2(out, c["out"]) < 1e-12
+
+
+def test_autotune_choices_keep_parity(hip, orc):
+    from atus2_amd.autotune import autotune
+    shape = (32, 32, 32)
+    g = orc.make_grid(shape, L=[24.0] * 3)
+    psis = _state(orc, g, 41, 2)
+    gs = [10.0, 9.5, 9.5, 10.0]
+    las = hip.laser(amp0=2.0, laser_k=2 * np.pi / 24.0 * 4, laser_domh=1.0)
+    amp = lambda t: 1.0
+    with hip.Context.from_grid(g, 2) as ctx:
+        for c in range(2):
+            ctx.upload(c, psis[c])
+        ctx.set_g(gs)
+        ctx.set_alpha([0.5] * 3)
+        t, choice = autotune(ctx, hip.STEP_BRAGG_AD, 1e-3, 0.0, las, amp, Nk=5, reps=1)
+        assert set(choice.values()) <= {8, 16}
+        start = [ctx.download(c) for c in range(2)]
+        S = orc.RTSolver(g, start, gs, [0.5] * 3, dt=1e-3, t=t, amp=amp, deltaL=[0.0, 0.0], amp0=2.0,
+                         laser_k=2 * np.pi / 24.0 * 4, laser_domh=1.0)
+        S.block(orc.BRAGG_AD, 20)
+        t = ctx.run_block(hip.STEP_BRAGG_AD, 1e-3, 20, t, las, amp)
+        assert rel_l2([ctx.download(c) for c in range(2)], S.psis) < 1e-12
