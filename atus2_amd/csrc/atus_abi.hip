@@ -377,7 +377,7 @@ int atus_ctx_create(atus_ctx** out, int dim, const long long n[3], const double 
   if (hipGetDevice(&c->device) != hipSuccess) return bail(ATUS_ERR_HIP, "no HIP device");
   int lds = 0;
   if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) == hipSuccess && lds > 0)
-    c->max_lds = std::min<size_t>((size_t)lds, 65536);
+    c->max_lds = std::min<size_t>((size_t)lds, 163840);
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
     return bail(ATUS_ERR_HIP, "stream creation");
   if (hipMalloc(&c->psi, sizeof(double2) * c->npts * ncomp) != hipSuccess)

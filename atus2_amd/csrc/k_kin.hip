@@ -7,8 +7,11 @@ template <int N, int RM, bool CONTIG, int VK, int NC>
 static hipError_t kin_go(double2* psi, long long cs, int ycount, long long inner, long long outer,
                          const double2* tw, const KinTabs& kt, const VArgs& va, double dx0, size_t max_lds,
                          hipStream_t s) {
-  PassCfg c = make_pass_cfg(N, RM, inner, outer, ycount, max_lds);
+  PassCfg c = make_pass_cfg(N, RM, inner, outer, ycount, max_lds, 2);
   if (c.block.x == 0 || c.block.x > 256 || c.grid.x == 0) return hipErrorInvalidConfiguration;
+  static bool ok = false;
+  hipError_t e = allow_lds(k_kin<N, RM, CONTIG, VK, NC>, c.lds, ok);
+  if (e != hipSuccess) return e;
   hipLaunchKernelGGL((k_kin<N, RM, CONTIG, VK, NC>), c.grid, c.block, c.lds, s, psi, cs, c.g, tw, kt, va, dx0);
   return hipGetLastError();
 }

@@ -5,13 +5,15 @@ namespace atus {
 
 bool supported_len(int N) { return N >= 2 && N <= 4096 && (N & (N - 1)) == 0; }
 
-PassCfg make_pass_cfg(int N, int rm, long long inner, long long outer, int ycount, size_t max_lds, int max_threads) {
+PassCfg make_pass_cfg(int N, int rm, long long inner, long long outer, int ycount, size_t max_lds, int ntables,
+                      int max_threads) {
   PassCfg c{};
   const int P = threads_per_pencil(N, rm);
-  const size_t pencil_bytes = (size_t)N * sizeof(double2);
+  const size_t pencil_bytes = (size_t)(N + 1) * sizeof(double2);  // lds_stride<N>()
+  const size_t table_bytes = N <= 1024 ? (size_t)ntables * N * sizeof(double2) : 0;
   long long T = max_threads / P;
   if (T < 1) T = 1;
-  while (T > 1 && (size_t)T * pencil_bytes > max_lds) T >>= 1;
+  while (T > 1 && (size_t)T * pencil_bytes + table_bytes > max_lds) T >>= 1;
   if (inner == 1) {
     long long tiles = (outer + T - 1) / T;
     c.g.inner = 1;
@@ -32,7 +34,8 @@ PassCfg make_pass_cfg(int N, int rm, long long inner, long long outer, int ycoun
   if (T * P > 256) T = 0;
   c.block = dim3((unsigned)(T * P));
   c.grid = dim3((unsigned)c.g.ntiles, (unsigned)ycount);
-  c.lds = (size_t)T * pencil_bytes;
+  c.lds = (size_t)T * pencil_bytes + table_bytes;
+  if (c.lds > max_lds) c.block = dim3(0);  // rejected by the launchers
   return c;
 }
 
@@ -42,17 +45,30 @@ static hipError_t pass_nr(bool contig, int dir, const double2* src, double2* dst
                           hipStream_t s) {
   PassCfg c = make_pass_cfg(N, RM, inner, outer, ncomp, max_lds);
   if (c.block.x == 0 || c.block.x > 256 || c.grid.x == 0) return hipErrorInvalidConfiguration;
+  static bool ok[4] = {false, false, false, false};
+  hipError_t e = hipSuccess;
   if (contig) {
-    if (dir < 0)
-      hipLaunchKernelGGL((k_pass<N, RM, true, -1>), c.grid, c.block, c.lds, s, src, dst, cs, c.g, tw, scale);
-    else
-      hipLaunchKernelGGL((k_pass<N, RM, true, 1>), c.grid, c.block, c.lds, s, src, dst, cs, c.g, tw, scale);
+    if (dir < 0) {
+      e = allow_lds(k_pass<N, RM, true, -1>, c.lds, ok[0]);
+      if (e == hipSuccess)
+        hipLaunchKernelGGL((k_pass<N, RM, true, -1>), c.grid, c.block, c.lds, s, src, dst, cs, c.g, tw, scale);
+    } else {
+      e = allow_lds(k_pass<N, RM, true, 1>, c.lds, ok[1]);
+      if (e == hipSuccess)
+        hipLaunchKernelGGL((k_pass<N, RM, true, 1>), c.grid, c.block, c.lds, s, src, dst, cs, c.g, tw, scale);
+    }
   } else {
-    if (dir < 0)
-      hipLaunchKernelGGL((k_pass<N, RM, false, -1>), c.grid, c.block, c.lds, s, src, dst, cs, c.g, tw, scale);
-    else
-      hipLaunchKernelGGL((k_pass<N, RM, false, 1>), c.grid, c.block, c.lds, s, src, dst, cs, c.g, tw, scale);
+    if (dir < 0) {
+      e = allow_lds(k_pass<N, RM, false, -1>, c.lds, ok[2]);
+      if (e == hipSuccess)
+        hipLaunchKernelGGL((k_pass<N, RM, false, -1>), c.grid, c.block, c.lds, s, src, dst, cs, c.g, tw, scale);
+    } else {
+      e = allow_lds(k_pass<N, RM, false, 1>, c.lds, ok[3]);
+      if (e == hipSuccess)
+        hipLaunchKernelGGL((k_pass<N, RM, false, 1>), c.grid, c.block, c.lds, s, src, dst, cs, c.g, tw, scale);
+    }
   }
+  if (e != hipSuccess) return e;
   return hipGetLastError();
 }
 

@@ -8,10 +8,18 @@ static hipError_t sob_nr(bool contig, const double2* src, double2* o1, double2* 
                          long long inner, const double2* tw, const SobArgs& sa, size_t max_lds, hipStream_t s) {
   PassCfg c = make_pass_cfg(N, RM, contig ? 1 : inner, 1, ncomp, max_lds);
   if (c.block.x == 0 || c.block.x > 256 || c.grid.x == 0) return hipErrorInvalidConfiguration;
-  if (contig)
-    hipLaunchKernelGGL((k_sobf<N, RM, true>), c.grid, c.block, c.lds, s, src, o1, o2, cs, c.g, tw, sa);
-  else
-    hipLaunchKernelGGL((k_sobf<N, RM, false>), c.grid, c.block, c.lds, s, src, o1, o2, cs, c.g, tw, sa);
+  static bool ok[2] = {false, false};
+  hipError_t e;
+  if (contig) {
+    e = allow_lds(k_sobf<N, RM, true>, c.lds, ok[0]);
+    if (e == hipSuccess)
+      hipLaunchKernelGGL((k_sobf<N, RM, true>), c.grid, c.block, c.lds, s, src, o1, o2, cs, c.g, tw, sa);
+  } else {
+    e = allow_lds(k_sobf<N, RM, false>, c.lds, ok[1]);
+    if (e == hipSuccess)
+      hipLaunchKernelGGL((k_sobf<N, RM, false>), c.grid, c.block, c.lds, s, src, o1, o2, cs, c.g, tw, sa);
+  }
+  if (e != hipSuccess) return e;
   return hipGetLastError();
 }
 

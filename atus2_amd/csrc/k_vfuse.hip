@@ -8,6 +8,9 @@ static hipError_t vf_go(double2* psi, long long cs, long long outer, const doubl
                         long long ppx, int n0, double dx0, size_t max_lds, hipStream_t s) {
   PassCfg c = make_pass_cfg(N, RM, 1, outer, 1, max_lds);
   if (c.block.x == 0 || c.block.x > 256 || c.grid.x == 0) return hipErrorInvalidConfiguration;
+  static bool ok = false;
+  hipError_t e = allow_lds(k_vfuse<N, RM, VK, NC>, c.lds, ok);
+  if (e != hipSuccess) return e;
   hipLaunchKernelGGL((k_vfuse<N, RM, VK, NC>), c.grid, c.block, c.lds, s, psi, cs, c.g, tw, va, ppx, n0, dx0);
   return hipGetLastError();
 }

@@ -57,6 +57,46 @@ struct VArgs {
   double amp0, laser_k, laser_domh, laser_dk, chirp, chirp_rate0, phase0;
 };
 
+// sin and cos: Taylor series (to x^11 and x^10) for |x| < 2^-5, where the
+// truncation error is below 1e-27 relative; the library routine otherwise.
+// The phases of the potential and Bragg steps are dt*E, usually small.
+__device__ __forceinline__ void sincos_fast(double x, double* s, double* c) {
+  if (fabs(x) < 0.03125) {
+    const double x2 = x * x;
+    *s = x * (1.0 + x2 * (-1.0 / 6.0 + x2 * (1.0 / 120.0 + x2 * (-1.0 / 5040.0 + x2 * (1.0 / 362880.0 +
+                                                                                  x2 * (-1.0 / 39916800.0))))));
+    *c = 1.0 + x2 * (-0.5 + x2 * (1.0 / 24.0 + x2 * (-1.0 / 720.0 + x2 * (1.0 / 40320.0 + x2 * (-1.0 / 3628800.0)))));
+  } else {
+    sincos(x, s, c);
+  }
+}
+
+// LDS elements per pencil region: one element of padding shifts the
+// regions of neighbouring pencils by 16 B, so the pencils a wavefront
+// touches together do not map to the same banks (regions 4 KiB apart would)
+template <int N>
+__host__ __device__ constexpr int lds_stride() {
+  return N + 1;
+}
+
+// lookup tables (twiddles, kinetic phases) of length N are staged in LDS
+// behind the pencil regions when they are small enough
+template <int N>
+constexpr bool tables_in_lds() {
+  return N <= 1024;
+}
+// copies src[0..N) to lds_tab (then a workgroup barrier) and returns the
+// pointer the kernel should read; the global table is returned otherwise
+template <int N>
+__device__ __forceinline__ const double2* stage_table(double2* lds_tab, const double2* src) {
+  if constexpr (tables_in_lds<N>()) {
+    for (int j = threadIdx.x; j < N; j += blockDim.x) lds_tab[j] = src[j];
+    return lds_tab;
+  } else {
+    return src;
+  }
+}
+
 // blockIdx.x -> tile so that the workgroups one XCD runs (round-robin
 // dispatch: XCD = blockIdx.x % 8) cover one contiguous range of tiles
 __device__ __forceinline__ long long xcd_tile(long long b, long long nb) {
@@ -139,7 +179,7 @@ __device__ __forceinline__ void nl_point(double2 (&a)[NC], const double (&extra)
 #pragma unroll
   for (int i = 0; i < NC; ++i) {
     double s, c;
-    sincos(phi[i], &s, &c);
+    sincos_fast(phi[i], &s, &c);
     const double re = a[i].x;
     a[i].x = re * c - a[i].y * s;
     a[i].y = a[i].y * c + re * s;
@@ -167,11 +207,11 @@ __device__ __forceinline__ void bragg_point(double2& a, double2& b, double bx, d
   const double V22 = p.gs[2] * n1 + p.gs[3] * n2 - p.deltaL[1] + bx;
   double s, c;
   if (Om == 0.0) {
-    sincos(-dt * V11, &s, &c);
+    sincos_fast(-dt * V11, &s, &c);
     double re = a.x;
     a.x = re * c - a.y * s;
     a.y = re * s + a.y * c;
-    sincos(-dt * V22, &s, &c);
+    sincos_fast(-dt * V22, &s, &c);
     re = b.x;
     b.x = re * c - b.y * s;
     b.y = re * s + b.y * c;
@@ -180,10 +220,10 @@ __device__ __forceinline__ void bragg_point(double2& a, double2& b, double bx, d
   const double r = sqrt((V11 - V22) * (V11 - V22) + 4.0 * Om * Om);
   const double Ep = 0.5 * (V11 + V22 + r);
   const double Em = 0.5 * (V11 + V22 - r);
-  sincos(-dt * Ep, &s, &c);
+  sincos_fast(-dt * Ep, &s, &c);
   double nrm = 1.0 / fabs((V11 - Ep) * (V11 - Ep) + Om * Om);
   const double gpr = nrm * c, gpi = nrm * s;
-  sincos(-dt * Em, &s, &c);
+  sincos_fast(-dt * Em, &s, &c);
   nrm = 1.0 / fabs((V11 - Em) * (V11 - Em) + Om * Om);
   const double gmr = nrm * c, gmi = nrm * s;
   const double o2 = Om * Om;
@@ -232,11 +272,13 @@ __global__ __launch_bounds__(256) void k_pass(const double2* src, double2* dst,
   extern __shared__ double2 lds_all[];
   constexpr int R = Plan<N, RM>::R;
   PencilMap<N, RM, CONTIG> m(g);
-  double2* lds = lds_all + m.p * N;
+  double2* lds = lds_all + m.p * lds_stride<N>();
+  const double2* twl = stage_table<N>(lds_all + (size_t)g.T * lds_stride<N>(), tw);
   const long long coff = (long long)blockIdx.y * comp_stride;
   double2 v[R];
   load_pencil<N, RM>(v, src + coff, m.base, m.stride, m.t, m.valid);
-  fft_pencil<N, RM, DIR, CONTIG && (Plan<N, RM>::P <= 64)>(v, m.t, lds, tw);
+  if constexpr (tables_in_lds<N>()) __syncthreads();
+  fft_pencil<N, RM, DIR, CONTIG && (Plan<N, RM>::P <= 64)>(v, m.t, lds, twl);
   if (scale != 1.0) {
 #pragma unroll
     for (int i = 0; i < R; ++i) v[i] = cscale(v[i], scale);
@@ -264,7 +306,9 @@ __global__ __launch_bounds__(256) void k_kin(double2* __restrict__ psi, long lon
   constexpr int R = Plan<N, RM>::R, P = Plan<N, RM>::P;
   constexpr bool WAVE = CONTIG && (P <= 64);
   PencilMap<N, RM, CONTIG> m(g);
-  double2* lds = lds_all + m.p * N;
+  double2* lds = lds_all + m.p * lds_stride<N>();
+  const double2* twl = stage_table<N>(lds_all + (size_t)g.T * lds_stride<N>(), tw);
+  const double2* e0l = stage_table<N>(lds_all + (size_t)g.T * lds_stride<N>() + N, kt.e0);
   // pencil constant of the phase (axes 1, 2)
   double2 cp = make_double2(1.0, 0.0);
   if constexpr (!CONTIG) {
@@ -296,16 +340,17 @@ __global__ __launch_bounds__(256) void k_kin(double2* __restrict__ psi, long lon
       for (int c = 0; c < NC; ++c) v[c][i] = a[c];
     }
   }
+  if constexpr (tables_in_lds<N>()) __syncthreads();
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
-    fft_pencil<N, RM, -1, WAVE>(v[c], m.t, lds, tw);
+    fft_pencil<N, RM, -1, WAVE>(v[c], m.t, lds, twl);
 #pragma unroll
     for (int i = 0; i < R; ++i) {
-      double2 e = kt.e0[m.t + P * i];
+      double2 e = e0l[m.t + P * i];
       if constexpr (!CONTIG) e = cmul(e, cp);
       v[c][i] = cmul(v[c][i], e);
     }
-    fft_pencil<N, RM, +1, WAVE>(v[c], m.t, lds, tw);
+    fft_pencil<N, RM, +1, WAVE>(v[c], m.t, lds, twl);
     store_pencil<N, RM>(v[c], psi + c0 + c * comp_stride, m.base, m.stride, m.t, m.valid);
   }
 }
@@ -322,7 +367,8 @@ __global__ __launch_bounds__(256) void k_vfuse(double2* __restrict__ psi, long l
   constexpr int R = Plan<N, RM>::R, P = Plan<N, RM>::P;
   constexpr bool WAVE = P <= 64;
   PencilMap<N, RM, true> m(g);
-  double2* lds = lds_all + m.p * N;
+  double2* lds = lds_all + m.p * lds_stride<N>();
+  const double2* twl = stage_table<N>(lds_all + (size_t)g.T * lds_stride<N>(), tw);
   const int i0 = (int)(m.pid / pencils_per_x0);
   const double x0 = (double)(i0 - n0 / 2) * dx0;
   double Om = 0.0;
@@ -330,10 +376,10 @@ __global__ __launch_bounds__(256) void k_vfuse(double2* __restrict__ psi, long l
   if constexpr (VK == V_BRAGG) bragg_field(x0, va, Om, eta);
   double2 v[NC][R];
 #pragma unroll
-  for (int c = 0; c < NC; ++c) {
-    load_pencil<N, RM>(v[c], psi + c * comp_stride, m.base, 1, m.t, m.valid);
-    fft_pencil<N, RM, +1, WAVE>(v[c], m.t, lds, tw);
-  }
+  for (int c = 0; c < NC; ++c) load_pencil<N, RM>(v[c], psi + c * comp_stride, m.base, 1, m.t, m.valid);
+  if constexpr (tables_in_lds<N>()) __syncthreads();
+#pragma unroll
+  for (int c = 0; c < NC; ++c) fft_pencil<N, RM, +1, WAVE>(v[c], m.t, lds, twl);
 #pragma unroll
   for (int i = 0; i < R; ++i) {
     double2 a[NC];
@@ -345,7 +391,7 @@ __global__ __launch_bounds__(256) void k_vfuse(double2* __restrict__ psi, long l
   }
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
-    fft_pencil<N, RM, -1, WAVE>(v[c], m.t, lds, tw);
+    fft_pencil<N, RM, -1, WAVE>(v[c], m.t, lds, twl);
     store_pencil<N, RM>(v[c], psi + c * comp_stride, m.base, 1, m.t, m.valid);
   }
 }
@@ -376,7 +422,8 @@ __global__ __launch_bounds__(256) void k_sobf(const double2* src, double2* out_s
   constexpr int R = Plan<N, RM>::R, P = Plan<N, RM>::P;
   constexpr bool WAVE = CONTIG && (P <= 64);
   PencilMap<N, RM, CONTIG> m(g);
-  double2* lds = lds_all + m.p * N;
+  double2* lds = lds_all + m.p * lds_stride<N>();
+  const double2* twl = stage_table<N>(lds_all + (size_t)g.T * lds_stride<N>(), tw);
   const long long coff = (long long)blockIdx.y * comp_stride;
   double k12 = 0.0;  // sum over axes 1, 2 of k^2 (k.scale(alpha).k with isotropic alpha)
   if constexpr (!CONTIG) {
@@ -394,7 +441,8 @@ __global__ __launch_bounds__(256) void k_sobf(const double2* src, double2* out_s
   }
   double2 v[R];
   load_pencil<N, RM>(v, src + coff, m.base, m.stride, m.t, m.valid);
-  fft_pencil<N, RM, -1, WAVE>(v, m.t, lds, tw);
+  if constexpr (tables_in_lds<N>()) __syncthreads();
+  fft_pencil<N, RM, -1, WAVE>(v, m.t, lds, twl);
   double2 w[R];
 #pragma unroll
   for (int i = 0; i < R; ++i) {
@@ -403,10 +451,10 @@ __global__ __launch_bounds__(256) void k_sobf(const double2* src, double2* out_s
     w[i] = cscale(v[i], -phi * sa.scale);
     v[i] = cscale(v[i], sa.scale / (1.0 + phi));
   }
-  fft_pencil<N, RM, +1, WAVE>(v, m.t, lds, tw);
+  fft_pencil<N, RM, +1, WAVE>(v, m.t, lds, twl);
   store_pencil<N, RM>(v, out_sob + coff, m.base, m.stride, m.t, m.valid);
   if (out_lap) {
-    fft_pencil<N, RM, +1, WAVE>(w, m.t, lds, tw);
+    fft_pencil<N, RM, +1, WAVE>(w, m.t, lds, twl);
     store_pencil<N, RM>(w, out_lap + coff, m.base, m.stride, m.t, m.valid);
   }
 }

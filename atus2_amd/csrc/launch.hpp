@@ -20,10 +20,22 @@ inline int threads_per_pencil(int N, int rm) { return N / (N < rm ? N : rm); }
 
 // contiguous pencils if inner == 1; the pass covers `outer` slabs of
 // [N][inner]; ycount = gridDim.y (components)
+// ntables: lookup tables of length N the kernel stages in LDS (N <= 1024)
 PassCfg make_pass_cfg(int N, int rm, long long inner, long long outer, int ycount, size_t max_lds,
-                      int max_threads = 256);
+                      int ntables = 1, int max_threads = 256);
 
 bool supported_len(int N);  // power of two in [2, 4096]
+
+// raises the dynamic-LDS limit of `kernel` once when a launch needs more
+// than the 64 KiB default (gfx950 allows 160 KiB per workgroup)
+template <class K>
+inline hipError_t allow_lds(K kernel, size_t bytes, bool& done) {
+  if (bytes <= 65536 || done) return hipSuccess;
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)bytes);
+  done = e == hipSuccess;
+  return e;
+}
 
 // rm: values per thread (8 or 16) of the FFT kernels
 hipError_t launch_pass(int N, int rm, bool contig, int dir, const double2* src, double2* dst, long long comp_stride,

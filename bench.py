@@ -41,7 +41,14 @@ def gpu_run(args, wl):
     N0 = ctx.particle_number(0) + ctx.particle_number(1)
     t = 0.0
     choice = {}
-    if not args.no_autotune:
+    if args.tuning:
+        # fixed choices, e.g. "pass_strided=8,vfuse=8"
+        for item in args.tuning.split(","):
+            fam, rm = item.split("=")
+            ctx.tune(fam.strip(), int(rm))
+            choice[fam.strip()] = int(rm)
+        log(f"[bench] fixed values per thread: {choice}")
+    elif not args.no_autotune:
         from atus2_amd.autotune import autotune
         t, choice = autotune(ctx, hip.STEP_BRAGG_AD, wl.dt, t, las, wl.amp, Nk=10, reps=2)
         log(f"[bench] autotuned values per thread: {choice}")
@@ -122,6 +129,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--tuning", default="", help="fixed values per thread, e.g. pass_strided=8,vfuse=8")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
