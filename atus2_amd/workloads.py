@@ -75,3 +75,22 @@ class BraggWorkload:
     def rabi_k0(self):
         k = 2 * self.laser_k
         return [(0.0, 0.0, 0.0), (k, 0.0, 0.0), (-k, 0.0, 0.0)]
+
+    # the reference's bragg sequence for this workload: bragg_ad for 1.0
+    # (Rabi populations after every block), then freeprop for 0.4, both at
+    # dt = 2e-4 and Nk = 100 (int(1.0/2e-4) = 5000, int(0.4/2e-4) = 2000)
+    SEQUENCE = (("bragg_ad", 1.0), ("freeprop", 0.4))
+
+    def blocks(self, duration):
+        """Number of Nk-step blocks of a sequence item (CRT_Base.h:742-755)."""
+        return int(duration / self.dt) // self.Nk
+
+    @classmethod
+    def proxy(cls, n):
+        """The same physics on an n^3 grid of the same spacing (dx = 0.25):
+        box and widths scaled by s = n/256, couplings by s^3, so that the
+        peak g*|psi|^2 (the nonlinear phase rate; the density scales as
+        1/s^3) is that of the 256^3 case."""
+        s = n / 256.0
+        return cls(n=n, L=64.0 * s, widths=tuple(w * s for w in cls.widths),
+                   gs=tuple(g * s ** 3 for g in cls.gs))
