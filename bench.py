@@ -20,6 +20,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+# HBM bytes per launch of each kernel family at 256^3, from rocprofv3
+# FETCH_SIZE / WRITE_SIZE passes scaled by a copy kernel of known size
+TRAFFIC_TABLE = os.path.join(ROOT, "profiles", "hbm_traffic_bragg256.json")
 
 
 def log(msg):
@@ -102,6 +105,16 @@ def gpu_run(args, wl):
     }
 
 
+def measured_traffic(n, family, values_per_thread):
+    """HBM bytes per launch measured by counters for this family, or None
+    when the table does not cover the configuration."""
+    if n != 256 or not os.path.exists(TRAFFIC_TABLE):
+        return None
+    with open(TRAFFIC_TABLE) as f:
+        tab = json.load(f)
+    return tab["families"].get(family, {}).get(str(values_per_thread))
+
+
 def cpu_baseline(wl, steps):
     """The oracle on the same 256^3 Bragg workload, `steps` steps."""
     from oracle import oracle as O
@@ -140,6 +153,7 @@ def main():
     r = gpu_run(args, wl)
     npts = float(math.prod(wl.shape))
     value = npts * args.steps / r["elapsed"] / 1e9
+    traffic = measured_traffic(wl.n, r["dominant"], r["tuning"].get(r["dominant"], 16))
     cpu = None
     if not args.no_cpu_baseline:
         log(f"[bench] CPU baseline: oracle, {args.cpu_steps} steps")
@@ -162,7 +176,7 @@ def main():
                                f"dt=2e-4, Nk=100, Rabi populations per block",
                    "grid": list(wl.shape), "components": 2, "Nk": wl.Nk},
         "roofline": {"bound": "hbm", "achieved": r["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": r["achieved"] / HBM_PEAK_GBS, "traffic": None,
+                     "frac": r["achieved"] / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": r["dominant"], "avg_us": 1e3 * r["dom_avg_ms"],
                      "algorithmic_bytes_per_launch": r["bytes_per_launch"],
                      "measured_copy_GBps": r["copy_gbps"]},
