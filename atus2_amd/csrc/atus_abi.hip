@@ -66,8 +66,10 @@ struct atus_ctx {
   int n[3] = {1, 1, 1};
   double dx[3] = {0, 0, 0};
   double dk[3] = {0, 0, 0};
-  int ncomp = 0;
+  int ncomp = 0;            // all components held (nbatch * nc_run)
   long long npts = 0;
+  int nbatch = 1;           // independent runs (atus_batch_create)
+  int nc_run = 0;           // components per run
   int device = 0;
   hipStream_t stream = nullptr;
   size_t max_lds = 65536;
@@ -87,6 +89,16 @@ struct atus_ctx {
   double* red_part = nullptr;
   double* red_out = nullptr;
   double* red_host = nullptr;
+
+  // batches: per-run parameters, the amplitude table of one block and
+  // the segmented-reduction buffers
+  RunParams* runs = nullptr;
+  bool runs_set = false;
+  double* ftab = nullptr;
+  size_t ftab_cap = 0;
+  double* sred_part = nullptr;
+  double* sred_out = nullptr;
+  size_t sred_cap = 0;
 
   bool sob_ready = false;
   double sob_alpha[2] = {0, 0};
@@ -145,6 +157,14 @@ namespace {
 int check_ctx(atus_ctx* c) {
   if (!c) return fail(ATUS_ERR_ARG, "null context");
   return hipSetDevice(c->device) == hipSuccess ? ATUS_OK : fail(ATUS_ERR_HIP, "hipSetDevice failed");
+}
+
+// entry points written for one run of up to four coupled components
+int check_single(atus_ctx* c, const char* what) {
+  if (c->nbatch > 1)
+    return fail(ATUS_ERR_STATE, std::string(what) + " acts on a single run; this context holds a batch of " +
+                                    std::to_string(c->nbatch) + " runs (use the atus_batch_* entry points)");
+  return ATUS_OK;
 }
 
 int check_comp(atus_ctx* c, int comp) {
@@ -239,25 +259,26 @@ int ensure_scratch(atus_ctx* c) {
   return ATUS_OK;
 }
 
-// one pass along axis a of `ncomp` components starting at src/dst
-int pass(atus_ctx* c, int a, int dir, const double2* src, double2* dst, int ncomp, double scale) {
+// one pass along axis a of `ncomp` components starting at src/dst, `cs`
+// elements apart (0: npts, i.e. consecutive components)
+int pass(atus_ctx* c, int a, int dir, const double2* src, double2* dst, int ncomp, double scale, long long cs = 0) {
   const int N = c->n[a];
   ATUS_TRY(ensure_twiddles(c, N));
   const long long inner = c->inner_of(a), outer = c->outer_of(a);
   const int fam = inner == 1 ? ATUS_KFAM_PASS_CONTIG : ATUS_KFAM_PASS_STRIDED;
   ProfScope ps(c, fam);
-  HIP_TRY(launch_pass(N, c->rm[fam], inner == 1, dir, src, dst, c->npts, ncomp, inner, outer, c->tw[N], scale, c->max_lds,
-                      c->stream));
+  HIP_TRY(launch_pass(N, c->rm[fam], inner == 1, dir, src, dst, cs ? cs : c->npts, ncomp, inner, outer, c->tw[N], scale,
+                      c->max_lds, c->stream));
   return ATUS_OK;
 }
 
 // full forward (dir -1) or backward (+1) transform with the cft scale,
 // src -> dst (dst may equal src); axes from fastest to slowest
-int full_ft(atus_ctx* c, int dir, const double2* src, double2* dst, int ncomp) {
+int full_ft(atus_ctx* c, int dir, const double2* src, double2* dst, int ncomp, long long cs = 0) {
   const double scale = ft_factor(c->dim, dir < 0 ? c->dx : c->dk);
   const double2* s = src;
   for (int a = c->dim - 1; a >= 0; --a) {
-    ATUS_TRY(pass(c, a, dir, s, dst, ncomp, a == 0 ? scale : 1.0));
+    ATUS_TRY(pass(c, a, dir, s, dst, ncomp, a == 0 ? scale : 1.0, cs));
     s = dst;
   }
   return ATUS_OK;
@@ -291,6 +312,37 @@ int reduce(atus_ctx* c, const Fn& fn, long long n, double* out) {
   HIP_TRY(hipMemcpyAsync(c->red_host, c->red_out, sizeof(double) * NV, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   for (int v = 0; v < NV; ++v) out[v] = c->red_host[v];
+  return ATUS_OK;
+}
+
+// nseg reductions of n elements each, segment y starting y * seg_stride
+// elements after the functor's data: out[y * NV + v]
+template <class Fn>
+int reduce_seg(atus_ctx* c, const Fn& fn, long long n, long long seg_stride, int nseg, double* out) {
+  constexpr int NV = Fn::NV;
+  // about 2048 workgroups over all segments, at least one per segment
+  unsigned nb = red_blocks(n);
+  const unsigned want = (unsigned)std::max(1, 2048 / nseg);
+  if (nb > want) nb = want;
+  const size_t need = (size_t)nseg * nb * NV;
+  if (c->sred_cap < need) {
+    (void)hipFree(c->sred_part);
+    (void)hipFree(c->sred_out);
+    c->sred_part = c->sred_out = nullptr;
+    c->sred_cap = 0;
+    if (hipMalloc(&c->sred_part, sizeof(double) * need) != hipSuccess ||
+        hipMalloc(&c->sred_out, sizeof(double) * (size_t)nseg * RED_MAX_NV) != hipSuccess)
+      return fail(ATUS_ERR_NOMEM, "segmented reduction buffers");
+    c->sred_cap = need;
+  }
+  hipLaunchKernelGGL((k_reduce_seg<Fn>), dim3(nb, (unsigned)nseg), dim3(RED_THREADS), 0, c->stream, fn, n, seg_stride,
+                     c->sred_part);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL((k_reduce_seg_final<NV>), dim3((unsigned)nseg), dim3(RED_THREADS), 0, c->stream, c->sred_part,
+                     (int)nb, c->sred_out);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(out, c->sred_out, sizeof(double) * (size_t)nseg * NV, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
   return ATUS_OK;
 }
 
@@ -344,12 +396,19 @@ int atus_device_info(char* name, int len, int* cu_count, double* hbm_gbps_peak) 
   return ATUS_OK;
 }
 
-int atus_ctx_create(atus_ctx** out, int dim, const long long n[3], const double dx[3], const double dk[3],
-                    int ncomp) {
+}  // extern "C"
+
+namespace {
+
+int ctx_create(atus_ctx** out, int dim, const long long n[3], const double dx[3], const double dk[3], int nc_run,
+               int nbatch, bool batch) {
   if (!out) return fail(ATUS_ERR_ARG, "null output pointer");
   *out = nullptr;
   if (dim < 1 || dim > 3) return fail(ATUS_ERR_ARG, "dim must be 1, 2 or 3");
-  if (ncomp < 1 || ncomp > 4) return fail(ATUS_ERR_ARG, "ncomp must be 1..4");
+  if (nc_run < 1 || nc_run > 4) return fail(ATUS_ERR_ARG, "ncomp must be 1..4");
+  // every component is one blockIdx.y of the per-component passes (<= 65535)
+  if (nbatch < 1 || nbatch > 16384) return fail(ATUS_ERR_ARG, "nbatch must be 1..16384");
+  const int ncomp = nc_run * nbatch;
   for (int a = 0; a < 3; ++a) {
     if (a < dim) {
       if (!supported_len((int)n[a]) || n[a] > 4096)
@@ -362,6 +421,8 @@ int atus_ctx_create(atus_ctx** out, int dim, const long long n[3], const double 
   atus_ctx* c = new atus_ctx();
   c->dim = dim;
   c->ncomp = ncomp;
+  c->nc_run = nc_run;
+  c->nbatch = nbatch;
   c->npts = 1;
   for (int a = 0; a < 3; ++a) {
     c->n[a] = (int)n[a];
@@ -380,6 +441,8 @@ int atus_ctx_create(atus_ctx** out, int dim, const long long n[3], const double 
     c->max_lds = std::min<size_t>((size_t)lds, 163840);
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
     return bail(ATUS_ERR_HIP, "stream creation");
+  if (batch && hipMalloc(&c->runs, sizeof(RunParams) * nbatch) != hipSuccess)
+    return bail(ATUS_ERR_NOMEM, "per-run parameters");
   if (hipMalloc(&c->psi, sizeof(double2) * c->npts * ncomp) != hipSuccess)
     return bail(ATUS_ERR_NOMEM, "wavefunction allocation (" + std::to_string(sizeof(double2) * c->npts * ncomp) +
                                     " bytes)");
@@ -400,6 +463,15 @@ int atus_ctx_create(atus_ctx** out, int dim, const long long n[3], const double 
   return ATUS_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
+int atus_ctx_create(atus_ctx** out, int dim, const long long n[3], const double dx[3], const double dk[3],
+                    int ncomp) {
+  return ctx_create(out, dim, n, dx, dk, ncomp, 1, false);
+}
+
 int atus_ctx_destroy(atus_ctx* c) {
   if (!c) return ATUS_OK;
   (void)hipSetDevice(c->device);
@@ -415,6 +487,10 @@ int atus_ctx_destroy(atus_ctx* c) {
   (void)hipFree(c->sob);
   (void)hipFree(c->lap);
   (void)hipFree(c->grad);
+  (void)hipFree(c->runs);
+  (void)hipFree(c->ftab);
+  (void)hipFree(c->sred_part);
+  (void)hipFree(c->sred_out);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -448,6 +524,7 @@ int atus_download(atus_ctx* c, int comp, double* host) {
 
 int atus_set_potential(atus_ctx* c, int comp, const double* host) {
   ATUS_TRY(check_ctx(c));
+  ATUS_TRY(check_single(c, "atus_set_potential"));
   if (!host) {
     HIP_TRY(hipStreamSynchronize(c->stream));
     (void)hipFree(c->pot);
@@ -467,6 +544,7 @@ int atus_set_potential(atus_ctx* c, int comp, const double* host) {
 
 int atus_set_g(atus_ctx* c, const double* gs) {
   ATUS_TRY(check_ctx(c));
+  ATUS_TRY(check_single(c, "atus_set_g"));
   if (!gs) return fail(ATUS_ERR_ARG, "null g");
   for (int i = 0; i < 16; ++i) c->gs[i] = 0;
   for (int i = 0; i < c->ncomp * c->ncomp; ++i) c->gs[i] = gs[i];
@@ -501,6 +579,7 @@ int atus_kinetic(atus_ctx* c, double frac, double dt) {
 
 int atus_nl_step(atus_ctx* c, double dt) {
   ATUS_TRY(check_ctx(c));
+  ATUS_TRY(check_single(c, "atus_nl_step"));
   VArgs va;
   fill_vargs(c, va, dt, nullptr);
   HIP_TRY(launch_point(V_NL, c->ncomp, c->psi, c->grid(), va, c->stream));
@@ -510,6 +589,7 @@ int atus_nl_step(atus_ctx* c, double dt) {
 
 int atus_nl_if_step(atus_ctx* c, double dt, double beta0, const double* deltaL) {
   ATUS_TRY(check_ctx(c));
+  ATUS_TRY(check_single(c, "atus_nl_if_step"));
   atus_laser L{};
   L.beta0 = beta0;
   for (int i = 0; i < c->ncomp; ++i) L.deltaL[i] = deltaL ? deltaL[i] : 0.0;
@@ -522,6 +602,7 @@ int atus_nl_if_step(atus_ctx* c, double dt, double beta0, const double* deltaL) 
 
 int atus_bragg_ad(atus_ctx* c, double dt, double t, double F, const atus_laser* laser) {
   ATUS_TRY(check_ctx(c));
+  ATUS_TRY(check_single(c, "atus_bragg_ad"));
   if (c->ncomp != 2) return fail(ATUS_ERR_STATE, "bragg_ad needs exactly two components");
   if (!laser) return fail(ATUS_ERR_ARG, "null laser parameters");
   VArgs va;
@@ -545,6 +626,7 @@ int atus_set_momentum(atus_ctx* c, int comp, const double p[3]) {
 
 int atus_run_block(atus_ctx* c, const atus_step_params* sp, int Nk, double* t) {
   ATUS_TRY(check_ctx(c));
+  ATUS_TRY(check_single(c, "atus_run_block"));
   if (!sp || !t) return fail(ATUS_ERR_ARG, "null argument");
   if (Nk < 1) return fail(ATUS_ERR_ARG, "Nk must be >= 1");
   const int vk = vkind_of(sp->kind);
@@ -677,6 +759,7 @@ int atus_rabi_integrals(atus_ctx* c, int comp, const double* k0, int n, double t
 
 int atus_energy(atus_ctx* c, double* out) {
   ATUS_TRY(check_ctx(c));
+  ATUS_TRY(check_single(c, "atus_energy"));
   if (!out) return fail(ATUS_ERR_ARG, "null output");
   ATUS_TRY(ensure_scratch(c));
   double ekin = 0;
@@ -757,6 +840,7 @@ struct RedMu {
 
 int atus_sob_setup(atus_ctx* c, const double* alpha_iso, double stepsize) {
   ATUS_TRY(check_ctx(c));
+  ATUS_TRY(check_single(c, "atus_sob_setup"));
   if (c->ncomp > 2) return fail(ATUS_ERR_UNSUPPORTED, "Sobolev minimiser supports 1 or 2 wavefunctions");
   if (!alpha_iso) return fail(ATUS_ERR_ARG, "null alpha");
   if (!c->pot) return fail(ATUS_ERR_STATE, "set the potential before atus_sob_setup");
@@ -772,6 +856,7 @@ int atus_sob_setup(atus_ctx* c, const double* alpha_iso, double stepsize) {
 
 int atus_sob_renormalize(atus_ctx* c) {
   ATUS_TRY(check_ctx(c));
+  ATUS_TRY(check_single(c, "atus_sob_renormalize"));
   std::vector<double> N(c->ncomp);
   for (int i = 0; i < c->ncomp; ++i) ATUS_TRY(atus_particle_number(c, i, &N[i]));
   for (int i = 0; i < c->ncomp; ++i) {
@@ -785,6 +870,7 @@ int atus_sob_renormalize(atus_ctx* c) {
 
 int atus_sob_iter(atus_ctx* c, double* res_tot, int* renormalised, double* N_out) {
   ATUS_TRY(check_ctx(c));
+  ATUS_TRY(check_single(c, "atus_sob_iter"));
   if (!c->sob_ready) return fail(ATUS_ERR_STATE, "call atus_sob_setup first");
   const int nw = c->ncomp;
   const long long np = c->npts;
@@ -838,6 +924,7 @@ int atus_sob_iter(atus_ctx* c, double* res_tot, int* renormalised, double* N_out
 
 int atus_sob_mu(atus_ctx* c, double* mu) {
   ATUS_TRY(check_ctx(c));
+  ATUS_TRY(check_single(c, "atus_sob_mu"));
   if (!c->sob_ready) return fail(ATUS_ERR_STATE, "call atus_sob_setup first");
   if (!mu) return fail(ATUS_ERR_ARG, "null output");
   const int nw = c->ncomp;
@@ -924,6 +1011,170 @@ int atus_last_block_ms(atus_ctx* c, double* ms) {
   ATUS_TRY(check_ctx(c));
   if (!ms) return fail(ATUS_ERR_ARG, "null output");
   *ms = c->timing ? c->last_ms : 0.0;
+  return ATUS_OK;
+}
+
+
+// ---------------------------------------------------------------- batches
+
+int atus_batch_create(atus_ctx** out, int dim, const long long n[3], const double dx[3], const double dk[3],
+                      int ncomp_per_run, int nbatch) {
+  if (ncomp_per_run != 2) return fail(ATUS_ERR_UNSUPPORTED, "batched runs have two components");
+  return ctx_create(out, dim, n, dx, dk, ncomp_per_run, nbatch, true);
+}
+
+int atus_batch_set_runs(atus_ctx* c, const atus_batch_run* runs, int nbatch) {
+  ATUS_TRY(check_ctx(c));
+  if (!runs) return fail(ATUS_ERR_ARG, "null run parameters");
+  if (!c->runs) return fail(ATUS_ERR_STATE, "not a batch context (atus_batch_create)");
+  if (nbatch != c->nbatch)
+    return fail(ATUS_ERR_ARG, "expected " + std::to_string(c->nbatch) + " runs, got " + std::to_string(nbatch));
+  std::vector<RunParams> h(nbatch);
+  for (int r = 0; r < nbatch; ++r) {
+    const atus_batch_run& a = runs[r];
+    RunParams& p = h[r];
+    for (int i = 0; i < 4; ++i) p.gs[i] = a.gs[i];
+    p.amp0 = a.laser.amp0;
+    p.laser_k = a.laser.laser_k;
+    p.laser_domh = a.laser.laser_domh;
+    p.laser_dk = a.laser.laser_dk;
+    p.chirp = a.laser.chirp;
+    p.chirp_rate0 = a.laser.chirp_rate0;
+    p.phase0 = a.laser.phase0;
+    p.beta0 = a.laser.beta0;
+    for (int i = 0; i < 4; ++i) p.deltaL[i] = a.laser.deltaL[i];
+  }
+  HIP_TRY(hipMemcpyAsync(c->runs, h.data(), sizeof(RunParams) * nbatch, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->runs_set = true;
+  return ATUS_OK;
+}
+
+int atus_batch_run_block(atus_ctx* c, int kind, double dt, int Nk, double* t, const double* F) {
+  ATUS_TRY(check_ctx(c));
+  if (!t) return fail(ATUS_ERR_ARG, "null time");
+  if (Nk < 1) return fail(ATUS_ERR_ARG, "Nk must be >= 1");
+  if (!c->runs || c->nc_run != 2) return fail(ATUS_ERR_STATE, "not a batch context (atus_batch_create)");
+  if (!c->runs_set) return fail(ATUS_ERR_STATE, "run parameters not set (atus_batch_set_runs)");
+  const int vk = vkind_of(kind);
+  if (vk != V_BRAGG && vk != V_NL_IF) return fail(ATUS_ERR_ARG, "batches run bragg_ad or freeprop (IF) steps");
+  const int B = c->nbatch;
+  const size_t nf = (size_t)Nk * B;
+  std::vector<double> ones;
+  if (!F) {
+    ones.assign(nf, 1.0);
+    F = ones.data();
+  }
+  for (size_t i = 0; i < nf; ++i)
+    if (!std::isfinite(F[i])) return fail(ATUS_ERR_NUMERIC, "non-finite amplitude in the table");
+  if (c->ftab_cap < nf) {
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    (void)hipFree(c->ftab);
+    c->ftab = nullptr;
+    c->ftab_cap = 0;
+    if (hipMalloc(&c->ftab, sizeof(double) * nf) != hipSuccess) return fail(ATUS_ERR_NOMEM, "amplitude table");
+    c->ftab_cap = nf;
+  }
+  HIP_TRY(hipMemcpyAsync(c->ftab, F, sizeof(double) * nf, hipMemcpyHostToDevice, c->stream));
+  ATUS_TRY(ensure_kin(c, dt));
+  VArgs base;
+  fill_vargs(c, base, dt, nullptr);
+  base.runs = c->runs;
+  base.run_shift = (long long)(c->nc_run - 1) * c->npts;
+  const int N0 = c->n[0], NL = c->n[c->dim - 1];
+  const long long inner0 = c->inner_of(0);
+  base.pencils_per_run = c->dim == 1 ? 1 : c->npts / NL;
+  if (c->timing) HIP_TRY(hipEventRecord(c->ev0, c->stream));
+
+  double tt = *t;
+  if (c->dim >= 2) {
+    ATUS_TRY(pass(c, c->dim - 1, -1, c->psi, c->psi, c->ncomp, 1.0));
+    if (c->dim == 3) ATUS_TRY(pass(c, 1, -1, c->psi, c->psi, c->ncomp, 1.0));
+    {
+      ProfScope ps(c, ATUS_KFAM_KIN);
+      HIP_TRY(launch_kin(N0, c->rm[ATUS_KFAM_KIN], false, V_NONE, 1, c->psi, c->npts, c->ncomp, inner0, c->tw[N0],
+                         kin_tabs(c, 0), base, c->dx[0], c->max_lds, c->stream));
+    }
+    if (c->dim == 3) ATUS_TRY(pass(c, 1, +1, c->psi, c->psi, c->ncomp, 1.0));
+  } else {
+    ATUS_TRY(kinetic_unfused(c, 0));
+  }
+  tt += 0.5 * dt;
+  for (int m = 1; m <= Nk; ++m) {
+    VArgs va = base;
+    va.t1 = tt + 0.5 * dt;
+    va.Fm = c->ftab + (size_t)(m - 1) * B;
+    const int f = (m < Nk) ? 1 : 0;
+    if (c->dim == 1) {
+      ProfScope ps(c, ATUS_KFAM_KIN);
+      HIP_TRY(launch_vstep_batch(N0, c->rm[ATUS_KFAM_KIN], true, vk, c->psi, c->npts, B, c->tw[N0], kin_tabs(c, f), va,
+                                 1, N0, c->dx[0], c->max_lds, c->stream));
+    } else {
+      const long long ppx = c->dim == 3 ? c->n[1] : 1;
+      {
+        ProfScope ps(c, ATUS_KFAM_VFUSE);
+        HIP_TRY(launch_vstep_batch(NL, c->rm[ATUS_KFAM_VFUSE], false, vk, c->psi, c->npts,
+                                   (long long)B * base.pencils_per_run, c->tw[NL], kin_tabs(c, f), va, ppx, c->n[0],
+                                   c->dx[0], c->max_lds, c->stream));
+      }
+      if (c->dim == 3) ATUS_TRY(pass(c, 1, -1, c->psi, c->psi, c->ncomp, 1.0));
+      {
+        ProfScope ps(c, ATUS_KFAM_KIN);
+        HIP_TRY(launch_kin(N0, c->rm[ATUS_KFAM_KIN], false, V_NONE, 1, c->psi, c->npts, c->ncomp, inner0, c->tw[N0],
+                           kin_tabs(c, f), va, c->dx[0], c->max_lds, c->stream));
+      }
+      if (c->dim == 3) ATUS_TRY(pass(c, 1, +1, c->psi, c->psi, c->ncomp, 1.0));
+    }
+    tt += (m < Nk) ? dt : 0.5 * dt;
+  }
+  if (c->dim >= 2) ATUS_TRY(pass(c, c->dim - 1, +1, c->psi, c->psi, c->ncomp, 1.0));
+  if (c->timing) {
+    HIP_TRY(hipEventRecord(c->ev1, c->stream));
+    HIP_TRY(hipEventSynchronize(c->ev1));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+  } else {
+    HIP_TRY(hipStreamSynchronize(c->stream));
+  }
+  *t = tt;
+  return ATUS_OK;
+}
+
+int atus_batch_particle_numbers(atus_ctx* c, double* out) {
+  ATUS_TRY(check_ctx(c));
+  if (!c->runs) return fail(ATUS_ERR_STATE, "not a batch context (atus_batch_create)");
+  if (!out) return fail(ATUS_ERR_ARG, "null output");
+  RedNorm fn{c->psi};
+  ATUS_TRY(reduce_seg(c, fn, c->npts, c->npts, c->ncomp, out));
+  for (int i = 0; i < c->ncomp; ++i) {
+    out[i] *= c->ar();
+    if (!std::isfinite(out[i])) return fail(ATUS_ERR_NUMERIC, "particle number is not finite");
+  }
+  return ATUS_OK;
+}
+
+int atus_batch_rabi_integrals(atus_ctx* c, int comp, const double* k0, int n, double threshold, double* out) {
+  ATUS_TRY(check_ctx(c));
+  if (!c->runs) return fail(ATUS_ERR_STATE, "not a batch context (atus_batch_create)");
+  if (comp < 0 || comp >= c->nc_run) return fail(ATUS_ERR_ARG, "component index within a run out of range");
+  if (n < 1 || n > RED_MAX_NV || !k0 || !out) return fail(ATUS_ERR_ARG, "need 1..8 momentum points");
+  const long long cs = (long long)c->nc_run * c->npts;
+  double2* p = c->psi + (long long)comp * c->npts;
+  ATUS_TRY(full_ft(c, -1, p, p, c->nbatch, cs));
+  RedRabi fn{};
+  fn.psik = p;
+  fn.gr = c->grid();
+  fn.nk0 = n;
+  for (int q = 0; q < n; ++q)
+    for (int a = 0; a < 3; ++a) fn.k0[q][a] = k0[3 * q + a];
+  fn.thr = threshold;
+  std::vector<double> s((size_t)c->nbatch * RED_MAX_NV);
+  ATUS_TRY(reduce_seg(c, fn, c->npts, cs, c->nbatch, s.data()));
+  ATUS_TRY(full_ft(c, +1, p, p, c->nbatch, cs));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  for (int r = 0; r < c->nbatch; ++r)
+    for (int q = 0; q < n; ++q) out[(size_t)r * n + q] = c->ar_k() * s[(size_t)r * RED_MAX_NV + q];
   return ATUS_OK;
 }
 

@@ -1,0 +1,68 @@
+// k_batch.hip -- V-step kernels of batches of independent two-component
+// runs: the 1D fused step (whole pencil = one run) and the contiguous-axis
+// pass of 2D/3D runs, with the parameters of each run read per pencil.
+#include "launch.hpp"
+
+namespace atus {
+
+template <int N, int RM, int VK>
+static hipError_t kinb_go(double2* psi, long long cs, long long nbatch, const double2* tw, const KinTabs& kt,
+                          const VArgs& va, double dx0, size_t max_lds, hipStream_t s) {
+  PassCfg c = make_pass_cfg(N, RM, 1, nbatch, 1, max_lds, 2);
+  if (c.block.x == 0 || c.block.x > 256 || c.grid.x == 0) return hipErrorInvalidConfiguration;
+  static bool ok = false;
+  hipError_t e = allow_lds(k_kin<N, RM, true, VK, 2, true>, c.lds, ok);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_kin<N, RM, true, VK, 2, true>), c.grid, c.block, c.lds, s, psi, cs, c.g, tw, kt, va, dx0);
+  return hipGetLastError();
+}
+
+template <int N, int RM, int VK>
+static hipError_t vfb_go(double2* psi, long long cs, long long outer, const double2* tw, const VArgs& va,
+                         long long ppx, int n0, double dx0, size_t max_lds, hipStream_t s) {
+  PassCfg c = make_pass_cfg(N, RM, 1, outer, 1, max_lds);
+  if (c.block.x == 0 || c.block.x > 256 || c.grid.x == 0) return hipErrorInvalidConfiguration;
+  static bool ok = false;
+  hipError_t e = allow_lds(k_vfuse<N, RM, VK, 2, true>, c.lds, ok);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_vfuse<N, RM, VK, 2, true>), c.grid, c.block, c.lds, s, psi, cs, c.g, tw, va, ppx, n0, dx0);
+  return hipGetLastError();
+}
+
+template <int N, int RM>
+static hipError_t batch_nr(bool one_d, int vk, double2* psi, long long cs, long long count, const double2* tw,
+                           const KinTabs& kt, const VArgs& va, long long ppx, int n0, double dx0, size_t max_lds,
+                           hipStream_t s) {
+  if (one_d) {
+    if (vk == V_BRAGG) return kinb_go<N, RM, V_BRAGG>(psi, cs, count, tw, kt, va, dx0, max_lds, s);
+    if (vk == V_NL_IF) return kinb_go<N, RM, V_NL_IF>(psi, cs, count, tw, kt, va, dx0, max_lds, s);
+  } else {
+    if (vk == V_BRAGG) return vfb_go<N, RM, V_BRAGG>(psi, cs, count, tw, va, ppx, n0, dx0, max_lds, s);
+    if (vk == V_NL_IF) return vfb_go<N, RM, V_NL_IF>(psi, cs, count, tw, va, ppx, n0, dx0, max_lds, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+template <int N>
+static hipError_t batch_n(int rm, bool one_d, int vk, double2* psi, long long cs, long long count, const double2* tw,
+                          const KinTabs& kt, const VArgs& va, long long ppx, int n0, double dx0, size_t max_lds,
+                          hipStream_t s) {
+  if (rm == 8 && N > 8 && N / 8 <= 256)
+    return batch_nr<N, 8>(one_d, vk, psi, cs, count, tw, kt, va, ppx, n0, dx0, max_lds, s);
+  return batch_nr<N, 16>(one_d, vk, psi, cs, count, tw, kt, va, ppx, n0, dx0, max_lds, s);
+}
+
+hipError_t launch_vstep_batch(int N, int rm, bool one_d, int vk, double2* psi, long long comp_stride, long long count,
+                              const double2* tw, const KinTabs& kt, const VArgs& va, long long pencils_per_x0, int n0,
+                              double dx0, size_t max_lds, hipStream_t s) {
+  switch (N) {
+#define ATUS_CASE(n) \
+  case n: return batch_n<n>(rm, one_d, vk, psi, comp_stride, count, tw, kt, va, pencils_per_x0, n0, dx0, max_lds, s);
+    ATUS_CASE(2) ATUS_CASE(4) ATUS_CASE(8) ATUS_CASE(16) ATUS_CASE(32) ATUS_CASE(64) ATUS_CASE(128)
+    ATUS_CASE(256) ATUS_CASE(512) ATUS_CASE(1024) ATUS_CASE(2048) ATUS_CASE(4096)
+#undef ATUS_CASE
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace atus

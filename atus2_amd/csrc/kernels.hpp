@@ -42,6 +42,14 @@ struct AxisGeo {
   int T;                // pencils per workgroup
 };
 
+// per-run parameters of a batch of independent two-component runs: the
+// members of one Bragg_single / CRT_Base_IF instance that differ per run
+struct RunParams {
+  double gs[4];
+  double amp0, laser_k, laser_domh, laser_dk, chirp, chirp_rate0, phase0, beta0;
+  double deltaL[4];
+};
+
 // parameters of the pointwise step V (CRT_Base::Do_NL_Step,
 // CRT_Base_IF::Do_NL_Step, Bragg_single::Do_Bragg_ad)
 struct VArgs {
@@ -55,7 +63,31 @@ struct VArgs {
   double t1;                // Get_t() + 0.5*dt
   double F;                 // Amplitude_at_time()
   double amp0, laser_k, laser_domh, laser_dk, chirp, chirp_rate0, phase0;
+  // batches (kernels instantiated with BATCH): run r owns the elements
+  // [r * run_elems, (r + 1) * run_elems), components npts apart
+  const RunParams* runs;    // per run
+  const double* Fm;         // Amplitude_at_time() of each run at this step
+  long long pencils_per_run;
+  long long run_shift;      // run_elems - npts
 };
+
+// the parameters of run r replace the shared ones
+__device__ __forceinline__ void bind_run(VArgs& v, long long r) {
+  const RunParams& p = v.runs[r];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) v.gs[i] = p.gs[i];
+  v.amp0 = p.amp0;
+  v.laser_k = p.laser_k;
+  v.laser_domh = p.laser_domh;
+  v.laser_dk = p.laser_dk;
+  v.chirp = p.chirp;
+  v.chirp_rate0 = p.chirp_rate0;
+  v.phase0 = p.phase0;
+  v.beta0 = p.beta0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) v.deltaL[i] = p.deltaL[i];
+  v.F = v.Fm[r];
+}
 
 // sin and cos: Taylor series (to x^11 and x^10) for |x| < 2^-5, where the
 // truncation error is below 1e-27 relative; the library routine otherwise.
@@ -299,7 +331,7 @@ struct KinTabs {
   int n2;
 };
 
-template <int N, int RM, bool CONTIG, int VK, int NC>
+template <int N, int RM, bool CONTIG, int VK, int NC, bool BATCH = false>
 __global__ __launch_bounds__(256) void k_kin(double2* __restrict__ psi, long long comp_stride, AxisGeo g,
                                              const double2* __restrict__ tw, KinTabs kt, VArgs va, double x0_dx) {
   extern __shared__ double2 lds_all[];
@@ -320,7 +352,13 @@ __global__ __launch_bounds__(256) void k_kin(double2* __restrict__ psi, long lon
     }
   }
   double2 v[NC][R];
-  const long long c0 = (NC == 1) ? (long long)blockIdx.y * comp_stride : 0;
+  long long c0 = (NC == 1) ? (long long)blockIdx.y * comp_stride : 0;
+  if constexpr (BATCH) {
+    // 1D batch: pencil m.pid is run m.pid
+    static_assert(CONTIG && VK != V_NONE, "batched kinetic steps carry the V step of their run");
+    if (m.valid) bind_run(va, m.pid);
+    c0 = m.pid * va.run_shift;
+  }
 #pragma unroll
   for (int c = 0; c < NC; ++c) load_pencil<N, RM>(v[c], psi + c0 + c * comp_stride, m.base, m.stride, m.t, m.valid);
   if constexpr (VK != V_NONE) {
@@ -359,7 +397,7 @@ __global__ __launch_bounds__(256) void k_kin(double2* __restrict__ psi, long lon
 // inverse DFT along the last axis, V step, forward DFT along the last axis.
 // The pencil (contiguous) has fixed coordinates on the other axes, so
 // x0 and the Bragg field are per-pencil constants.
-template <int N, int RM, int VK, int NC>
+template <int N, int RM, int VK, int NC, bool BATCH = false>
 __global__ __launch_bounds__(256) void k_vfuse(double2* __restrict__ psi, long long comp_stride, AxisGeo g,
                                                const double2* __restrict__ tw, VArgs va, long long pencils_per_x0,
                                                int n0, double dx0) {
@@ -369,7 +407,14 @@ __global__ __launch_bounds__(256) void k_vfuse(double2* __restrict__ psi, long l
   PencilMap<N, RM, true> m(g);
   double2* lds = lds_all + m.p * lds_stride<N>();
   const double2* twl = stage_table<N>(lds_all + (size_t)g.T * lds_stride<N>(), tw);
-  const int i0 = (int)(m.pid / pencils_per_x0);
+  long long pid = m.pid;
+  if constexpr (BATCH) {
+    const long long r = m.pid / va.pencils_per_run;
+    pid = m.pid - r * va.pencils_per_run;
+    if (m.valid) bind_run(va, r);
+    psi += r * va.run_shift;
+  }
+  const int i0 = (int)(pid / pencils_per_x0);
   const double x0 = (double)(i0 - n0 / 2) * dx0;
   double Om = 0.0;
   double2 eta = make_double2(1.0, 0.0);
@@ -524,6 +569,36 @@ __global__ __launch_bounds__(RED_THREADS) void k_reduce(Fn fn, long long n, doub
   block_reduce_store<NV>(acc, partials + (long long)blockIdx.x * NV);
 }
 
+// segmented form: blockIdx.y selects segment y, whose data start
+// y * seg_stride elements after the functor's; partials[y][blockIdx.x]
+template <class Fn>
+__global__ __launch_bounds__(RED_THREADS) void k_reduce_seg(Fn fn, long long n, long long seg_stride,
+                                                            double* __restrict__ partials) {
+  constexpr int NV = Fn::NV;
+  fn.shift((long long)blockIdx.y * seg_stride);
+  double acc[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long l = (long long)blockIdx.x * blockDim.x + threadIdx.x; l < n; l += stride) fn(l, acc);
+  block_reduce_store<NV>(acc, partials + ((long long)blockIdx.y * gridDim.x + blockIdx.x) * NV);
+}
+
+// one workgroup per segment: out[blockIdx.x][v]
+template <int NV>
+__global__ __launch_bounds__(RED_THREADS) void k_reduce_seg_final(const double* __restrict__ partials, int nblocks,
+                                                                  double* __restrict__ out) {
+  partials += (long long)blockIdx.x * nblocks * NV;
+  double acc[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+  for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc[v] += partials[(long long)b * NV + v];
+  }
+  block_reduce_store<NV>(acc, out + (long long)blockIdx.x * NV);
+}
+
 template <int NV>
 __global__ __launch_bounds__(RED_THREADS) void k_reduce_final(const double* __restrict__ partials, int nblocks,
                                                               double* __restrict__ out) {
@@ -541,6 +616,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_reduce_final(const double* __re
 struct RedNorm {
   static constexpr int NV = 1;
   const double2* psi;
+  __device__ void shift(long long off) { psi += off; }
   __device__ void operator()(long long l, double (&acc)[1]) const {
     const double2 a = psi[l];
     acc[0] += a.x * a.x + a.y * a.y;
@@ -600,6 +676,7 @@ struct RedRabi {
   int nk0;
   double k0[RED_MAX_NV][3];
   double thr;
+  __device__ void shift(long long off) { psik += off; }
   __device__ void operator()(long long l, double (&acc)[RED_MAX_NV]) const {
     const double2 a = psik[l];
     const double d = a.x * a.x + a.y * a.y;

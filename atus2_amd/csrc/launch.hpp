@@ -52,6 +52,15 @@ hipError_t launch_vfuse(int N, int rm, int vk, int nc, double2* psi, long long c
                         const double2* tw, const VArgs& va, long long pencils_per_x0, int n0, double dx0,
                         size_t max_lds, hipStream_t s);
 
+// V step of a batch of independent two-component runs (VArgs::runs, ::Fm,
+// ::pencils_per_run, ::run_shift set).  one_d: the fused 1D step over
+// `count` runs (the kinetic phase of slot kt follows the V step); otherwise
+// the contiguous-axis pass (inverse DFT, V, forward DFT) over `count`
+// pencils of all runs.
+hipError_t launch_vstep_batch(int N, int rm, bool one_d, int vk, double2* psi, long long comp_stride, long long count,
+                              const double2* tw, const KinTabs& kt, const VArgs& va, long long pencils_per_x0, int n0,
+                              double dx0, size_t max_lds, hipStream_t s);
+
 // Sobolev filters along axis 0
 hipError_t launch_sobf(int N, int rm, bool contig, const double2* src, double2* out_sob, double2* out_lap,
                        long long comp_stride, int ncomp, long long inner, const double2* tw, const SobArgs& sa,

@@ -23,6 +23,8 @@ EXPORTED = [
     "atus_set_momentum", "atus_run_block", "atus_particle_number", "atus_expval_x", "atus_expval_k",
     "atus_rabi_integrals", "atus_energy", "atus_sob_setup", "atus_sob_renormalize", "atus_sob_iter",
     "atus_sob_mu", "atus_set_timing", "atus_last_block_ms", "atus_run_block_profiled", "atus_copy_bandwidth", "atus_tune",
+    "atus_batch_create", "atus_batch_set_runs", "atus_batch_run_block", "atus_batch_particle_numbers",
+    "atus_batch_rabi_integrals",
 ]
 
 KFAM = ["pass_contig", "pass_strided", "kin", "vfuse", "point"]
@@ -39,6 +41,10 @@ class Laser(C.Structure):
 
 
 AMP_FN = C.CFUNCTYPE(C.c_double, C.c_double, C.c_void_p)
+
+
+class BatchRun(C.Structure):
+    _fields_ = [("gs", C.c_double * 4), ("laser", Laser)]
 
 
 class StepParams(C.Structure):
@@ -94,6 +100,11 @@ def load():
                                         C.POINTER(I)]),
         "atus_copy_bandwidth": (I, [P, I, C.POINTER(D)]),
         "atus_tune": (I, [P, I, I]),
+        "atus_batch_create": (I, [pctx, I, C.POINTER(C.c_longlong), C.POINTER(D), C.POINTER(D), I, I]),
+        "atus_batch_set_runs": (I, [P, C.POINTER(BatchRun), I]),
+        "atus_batch_run_block": (I, [P, I, D, I, C.POINTER(D), P]),
+        "atus_batch_particle_numbers": (I, [P, P]),
+        "atus_batch_rabi_integrals": (I, [P, I, P, I, D, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -327,3 +338,73 @@ def device_info():
     bw = C.c_double()
     _check(L.atus_device_info(name, 256, C.byref(cu), C.byref(bw)))
     return name.value.decode(), cu.value, bw.value
+
+
+class BatchContext(Context):
+    """`nbatch` independent two-component runs of one grid (atus_batch_*).
+    Component c of run r is component r*2 + c of the underlying context."""
+
+    def __init__(self, shape, dx, dk, nbatch):
+        L = load()
+        shape = tuple(int(v) for v in shape)
+        self.shape = shape
+        self.dim = len(shape)
+        self.ncomp = 2 * int(nbatch)
+        self.nbatch = int(nbatch)
+        n = (C.c_longlong * 3)(*(list(shape) + [1] * (3 - self.dim)))
+        dxa = (C.c_double * 3)(*(list(dx)[:self.dim] + [0.0] * (3 - self.dim)))
+        dka = (C.c_double * 3)(*(list(dk)[:self.dim] + [0.0] * (3 - self.dim)))
+        h = C.c_void_p()
+        _check(L.atus_batch_create(C.byref(h), self.dim, n, dxa, dka, 2, self.nbatch))
+        self._h = h
+        self._amp_keep = None
+
+    def set_runs(self, gs, lasers):
+        """gs: nbatch x 4 couplings; lasers: nbatch Laser structures."""
+        arr = (BatchRun * self.nbatch)()
+        g = np.asarray(gs, dtype=np.float64).reshape(self.nbatch, 4)
+        for r in range(self.nbatch):
+            for i in range(4):
+                arr[r].gs[i] = float(g[r, i])
+            arr[r].laser = lasers[r]
+        _check(_lib.atus_batch_set_runs(self._h, arr, self.nbatch))
+
+    @staticmethod
+    def step_times(t, dt, Nk):
+        """The t1 of V steps 1..Nk of a block starting at t, accumulated as
+        the block runner does (t += dt/2, then t1 = t + dt/2, t += dt), so
+        that amplitudes evaluated here equal the single-run path's."""
+        out = []
+        tt = t + 0.5 * dt
+        for m in range(1, Nk + 1):
+            out.append(tt + 0.5 * dt)
+            if m < Nk:
+                tt += dt
+        return out
+
+    def run_batch_block(self, kind, dt, Nk, t, F=None):
+        """One block of every run; F: (Nk, nbatch) amplitudes of V steps
+        m = 1..Nk (t1 = t + m dt) or None (all 1).  Returns the new t."""
+        tt = C.c_double(t)
+        if F is None:
+            _check(_lib.atus_batch_run_block(self._h, kind, dt, Nk, C.byref(tt), None))
+        else:
+            a = np.ascontiguousarray(F, dtype=np.float64)
+            if a.shape != (Nk, self.nbatch):
+                raise ValueError(f"amplitude table shape {a.shape} != {(Nk, self.nbatch)}")
+            _check(_lib.atus_batch_run_block(self._h, kind, dt, Nk, C.byref(tt), _dp(a)))
+        return tt.value
+
+    def particle_numbers(self):
+        out = np.empty((self.nbatch, 2))
+        _check(_lib.atus_batch_particle_numbers(self._h, _dp(out)))
+        return out
+
+    def batch_rabi_integrals(self, comp, k0list, threshold):
+        k0 = np.zeros((len(k0list), 3))
+        for i, k in enumerate(k0list):
+            k0[i, :len(k)] = k
+        out = np.empty((self.nbatch, len(k0list)))
+        _check(_lib.atus_batch_rabi_integrals(self._h, comp, _dp(np.ascontiguousarray(k0)), len(k0list),
+                                              float(threshold), _dp(out)))
+        return out

@@ -175,6 +175,33 @@ int atus_tune(atus_ctx *ctx, int family, int values_per_thread);
  * the copy peak the kernels are compared with; no reference counterpart. */
 int atus_copy_bandwidth(atus_ctx *ctx, int reps, double *gbps);
 
+/* ---- batches of independent runs of one grid.  The reference scans
+ * parameters with one process per case (tammy/src/
+ * batch_case_generation_4_tammy.cpp:45-291, each case a bragg run); here B
+ * runs share one context and every launch advances all of them.  Run r owns
+ * components r*ncomp_per_run .. (r+1)*ncomp_per_run - 1, which atus_upload,
+ * atus_download and the single-component reductions address by that index.
+ * Runs have two components (the Bragg pair of Bragg_single). */
+typedef struct {
+  double gs[4];     /* m_gs of the run, row-major 2x2 */
+  atus_laser laser; /* the run's CRT_Base_IF laser parameters */
+} atus_batch_run;
+
+int atus_batch_create(atus_ctx **out, int dim, const long long n[3], const double dx[3], const double dk[3],
+                      int ncomp_per_run, int nbatch);
+int atus_batch_set_runs(atus_ctx *ctx, const atus_batch_run *runs, int nbatch);
+/* one block of every run, as atus_run_block with kind ATUS_STEP_BRAGG_AD or
+ * ATUS_STEP_FREEPROP_IF; F[(m-1)*nbatch + r] is Amplitude_at_time() of run
+ * r for V step m = 1..Nk (NULL: all 1).  The t1 of step m is accumulated as
+ * in atus_run_block: tt = *t + dt/2, then per step t1 = tt + dt/2, tt += dt
+ * (mathematically *t + m*dt). */
+int atus_batch_run_block(atus_ctx *ctx, int kind, double dt, int Nk, double *t, const double *F);
+/* Get_Particle_Number of every component: out[r*ncomp_per_run + c] */
+int atus_batch_particle_numbers(atus_ctx *ctx, double *out);
+/* compute_rabi_integrals of component `comp` of every run, transformed in
+ * place and back as in CRT_Base_IF.h:208-281: out[r*n + q] */
+int atus_batch_rabi_integrals(atus_ctx *ctx, int comp, const double *k0, int n, double threshold, double *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -44,3 +44,18 @@ def test_error_path_without_device_is_an_error_not_a_crash():
     assert rc == -3 and b"power of two" in lib.atus_last_error()
     rc = lib.atus_ctx_create(ctypes.byref(h), 4, n, d, d, 1)
     assert rc == -1
+
+
+def test_batch_create_rejects_invalid_arguments_before_any_device_call():
+    from atus2_amd import hip
+    lib = hip.load()
+    h = ctypes.c_void_p()
+    n = (ctypes.c_longlong * 3)(64, 1, 1)
+    d = (ctypes.c_double * 3)(1.0, 0.0, 0.0)
+    rc = lib.atus_batch_create(ctypes.byref(h), 1, n, d, d, 3, 4)
+    assert rc == -3 and b"two components" in lib.atus_last_error()
+    rc = lib.atus_batch_create(ctypes.byref(h), 1, n, d, d, 2, 0)
+    assert rc == -1 and b"nbatch" in lib.atus_last_error()
+    rc = lib.atus_batch_create(ctypes.byref(h), 1, n, d, d, 2, 20000)
+    assert rc == -1
+    assert not h.value

@@ -1,0 +1,131 @@
+"""Batches of independent Bragg runs (atus_batch_*): every run of a batch
+must evolve exactly as the same run through the single-run path, and as
+the oracle.  Runs differ in couplings, Rabi amplitude, detuning, pulse
+width and the detuning of the second level, as in a parameter scan."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    a = np.concatenate([np.ravel(x) for x in a])
+    b = np.concatenate([np.ravel(x) for x in b])
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.fixture(scope="module")
+def hip():
+    from atus2_amd import hip as H
+    H.load()
+    return H
+
+
+def _runs(nb, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for r in range(nb):
+        g = rng.uniform(1.0, 200.0)
+        out.append(dict(gs=[g, 0.95 * g, 0.95 * g, g], amp0=rng.uniform(1.0, 10.0),
+                        laser_domh=8.0 + rng.uniform(-2.0, 2.0), sigma=rng.uniform(0.02, 0.1),
+                        deltaL=[0.0, rng.uniform(-0.5, 0.5)], beta0=rng.uniform(-0.02, 0.02)))
+    return out
+
+
+def _amp(run, t0):
+    s = run["sigma"]
+    return lambda t: float(np.exp(-0.5 * ((t - t0) / s) ** 2))
+
+
+def _initial(orc, g, r):
+    xs = orc.coords_x(g)
+    base = np.exp(-0.5 * sum((x / (2.0 + 0.1 * r + a)) ** 2 for a, x in enumerate(xs))).astype(np.complex128)
+    base /= np.sqrt(orc.particle_number(g, base))
+    return base, np.zeros_like(base)
+
+
+@pytest.mark.parametrize("shape,rm", [((1024,), 16), ((256,), 8), ((64, 128), 16), ((32, 64), 8),
+                                      ((16, 8, 32), 16)])
+@pytest.mark.parametrize("kind", ["bragg_ad", "freeprop_if"])
+def test_batch_matches_single_runs_and_oracle(hip, orc, shape, rm, kind):
+    dim = len(shape)
+    L = [40.0, 30.0, 36.0][:dim]
+    g = orc.make_grid(shape, L=L)
+    nb, dt, Nk, t0 = 5, 1e-3, 20, 0.05
+    runs = _runs(nb, 11 + dim)
+    laser_k = 3 * (2 * np.pi / L[0])
+    hkind = hip.STEP_BRAGG_AD if kind == "bragg_ad" else hip.STEP_FREEPROP_IF
+    okind = orc.BRAGG_AD if kind == "bragg_ad" else orc.FREEPROP_IF
+    lasers = [hip.laser(amp0=r["amp0"], laser_k=laser_k, laser_domh=r["laser_domh"], beta0=r["beta0"],
+                        deltaL=tuple(r["deltaL"])) for r in runs]
+    alpha = [0.5] * dim
+    dx = [g.dx[a] for a in range(dim)]
+    dk = [g.dk[a] for a in range(dim)]
+    with hip.BatchContext(shape, dx, dk, nb) as B:
+        for f in hip.KFAM:
+            B.tune(f, rm)
+        states = [_initial(orc, g, r) for r in range(nb)]
+        for r in range(nb):
+            B.upload(2 * r, states[r][0])
+            B.upload(2 * r + 1, states[r][1])
+        B.set_alpha(alpha)
+        B.set_runs([r["gs"] for r in runs], lasers)
+        t = t0
+        for blk in range(2):
+            F = np.array([[_amp(runs[r], t0 + 0.02)(t1) for r in range(nb)] for t1 in B.step_times(t, dt, Nk)])
+            if kind != "bragg_ad":
+                F = None
+            t = B.run_batch_block(hkind, dt, Nk, t, F)
+        out = [(B.download(2 * r), B.download(2 * r + 1)) for r in range(nb)]
+        Ns = B.particle_numbers()
+        k0 = [(0.0,) * dim, (2 * laser_k,) + (0.0,) * (dim - 1)]
+        rabi = B.batch_rabi_integrals(0, k0, 0.5 * laser_k)
+        after_rabi = [(B.download(2 * r), B.download(2 * r + 1)) for r in range(nb)]
+    assert abs(t - (t0 + 2 * Nk * dt)) < 1e-12
+    for r in range(nb):
+        # the in-place transform and back of the Rabi integrals leaves psi
+        # unchanged to rounding
+        assert rel_l2(after_rabi[r], out[r]) < 1e-14
+        # the single-run path with the same parameters
+        with hip.Context(shape, dx, dk, 2) as S:
+            S.upload(0, states[r][0])
+            S.upload(1, states[r][1])
+            S.set_alpha(alpha)
+            S.set_g(runs[r]["gs"])
+            for f in hip.KFAM:
+                S.tune(f, rm)
+            ts = t0
+            for blk in range(2):
+                ts = S.run_block(hkind, dt, Nk, ts, lasers[r], _amp(runs[r], t0 + 0.02) if kind == "bragg_ad" else None)
+            single = (S.download(0), S.download(1))
+            assert rel_l2(out[r], single) < 1e-13, r
+            for c in range(2):
+                assert abs(Ns[r, c] - S.particle_number(c)) < 1e-14
+            rs = S.rabi_integrals(0, k0, 0.5 * laser_k)
+            assert np.allclose(rabi[r], rs, rtol=0, atol=1e-13), (r, rabi[r], rs)
+    # the oracle, for two of the runs
+    for r in (0, nb - 1):
+        run = runs[r]
+        ifp = dict(beta0=run["beta0"], deltaL=list(run["deltaL"]), amp0=run["amp0"], laser_k=laser_k,
+                   laser_domh=run["laser_domh"])
+        S = orc.RTSolver(g, list(states[r]), run["gs"], alpha, dt=dt, t=t0,
+                         amp=_amp(run, t0 + 0.02) if kind == "bragg_ad" else None, **ifp)
+        S.block(okind, Nk)
+        S.block(okind, Nk)
+        err = rel_l2(out[r], S.psis)
+        assert err < 1e-11, (r, err)
+
+
+def test_batch_rejects_single_run_calls(hip):
+    with hip.BatchContext((64,), [0.5], [2 * np.pi / 32], 3) as B:
+        with pytest.raises(hip.AtusError, match="batch"):
+            B.set_g([1.0, 0.0, 0.0, 1.0])
+        with pytest.raises(hip.AtusError, match="batch"):
+            B.run_block(hip.STEP_BRAGG_AD, 1e-3, 2, 0.0, hip.laser())
+        with pytest.raises(hip.AtusError, match="run parameters not set"):
+            B.run_batch_block(hip.STEP_BRAGG_AD, 1e-3, 2, 0.0)
+        B.set_runs([[1.0, 0.0, 0.0, 1.0]] * 3, [hip.laser()] * 3)
+        with pytest.raises(hip.AtusError, match="bragg_ad or freeprop"):
+            B.run_batch_block(hip.STEP_FREEPROP, 1e-3, 2, 0.0)
+        with pytest.raises(ValueError):
+            B.run_batch_block(hip.STEP_BRAGG_AD, 1e-3, 2, 0.0, np.ones((3, 3)))
