@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -70,6 +71,11 @@ struct atus_ctx {
   long long npts = 0;
   int nbatch = 1;           // independent runs (atus_batch_create)
   int nc_run = 0;           // components per run
+  // 1D grids longer than 4096: four-step transform on [fs_n1][fs_n2]
+  bool fs = false;
+  int fs_n1 = 0, fs_n2 = 0;
+  double2* kinz = nullptr;  // [2 fracs][N] kinetic phases in [k1][k2] order
+  double2* tbuf = nullptr;  // ncomp * npts, the transposes of full transforms
   int device = 0;
   hipStream_t stream = nullptr;
   size_t max_lds = 65536;
@@ -236,6 +242,16 @@ int ensure_kin(atus_ctx* c, double dt) {
     }
   }
   HIP_TRY(hipMemcpyAsync(c->kin, h.data(), sizeof(double2) * h.size(), hipMemcpyHostToDevice, c->stream));
+  std::vector<double2> hz;
+  if (c->fs) {
+    // four-step spectrum: row k1, column k2 holds frequency k1 + N1 k2
+    const long long N = c->npts, n1 = c->fs_n1, n2 = c->fs_n2;
+    hz.resize((size_t)2 * N);
+    for (int f = 0; f < 2; ++f)
+      for (long long k1 = 0; k1 < n1; ++k1)
+        for (long long k2 = 0; k2 < n2; ++k2) hz[(size_t)f * N + k1 * n2 + k2] = h[((size_t)f * 3) * nm + k1 + n1 * k2];
+    HIP_TRY(hipMemcpyAsync(c->kinz, hz.data(), sizeof(double2) * hz.size(), hipMemcpyHostToDevice, c->stream));
+  }
   HIP_TRY(hipStreamSynchronize(c->stream));
   c->kin_dt = dt;
   c->kin_valid = true;
@@ -249,6 +265,10 @@ KinTabs kin_tabs(atus_ctx* c, int f) {
   kt.e1 = c->dim > 1 ? c->kin + ((size_t)f * 3 + 1) * nm : nullptr;
   kt.e2 = c->dim > 2 ? c->kin + ((size_t)f * 3 + 2) * nm : nullptr;
   kt.n2 = c->dim > 2 ? c->n[2] : 1;
+  if (c->fs) {
+    kt.ez = c->kinz + (size_t)f * c->npts;
+    kt.ez_rows = c->fs_n1;
+  }
   return kt;
 }
 
@@ -272,10 +292,78 @@ int pass(atus_ctx* c, int a, int dir, const double2* src, double2* dst, int ncom
   return ATUS_OK;
 }
 
+// four-step column pass (FS_ENTER / FS_EXIT: ycount components cs apart;
+// FS_STEP: V step of nc components, or of every run of a batch)
+int fs_col(atus_ctx* c, int mode, double2* p, int ycount, long long cs, int vk, int nc, bool batch, const VArgs& va,
+           double scale) {
+  ProfScope ps(c, mode == FS_STEP ? ATUS_KFAM_VFUSE : ATUS_KFAM_PASS_STRIDED);
+  HIP_TRY(launch_fs_col(c->fs_n1, c->rm[mode == FS_STEP ? ATUS_KFAM_VFUSE : ATUS_KFAM_PASS_STRIDED], mode, vk, nc, batch,
+                        p, cs, ycount, c->fs_n2, c->tw[c->fs_n1], c->tw[(int)c->npts], va, c->dx[0], scale,
+                        c->max_lds, c->stream));
+  return ATUS_OK;
+}
+
+// four-step row pass of ncomp components: kinetic phase of slot f
+int fs_row_kin(atus_ctx* c, int f, const VArgs& va) {
+  ProfScope ps(c, ATUS_KFAM_KIN);
+  HIP_TRY(launch_kin(c->fs_n2, c->rm[ATUS_KFAM_KIN], true, V_NONE, 1, c->psi, c->npts, c->ncomp, 1, c->tw[c->fs_n2],
+                     kin_tabs(c, f), va, c->dx[0], c->max_lds, c->stream, c->fs_n1));
+  return ATUS_OK;
+}
+
+int transpose(atus_ctx* c, const double2* src, double2* dst, long long rows, long long cols, int ncomp, long long cs,
+              double scale) {
+  const long long tiles = ((rows + 31) / 32) * ((cols + 31) / 32) * ncomp;
+  hipLaunchKernelGGL(k_transpose, dim3((unsigned)tiles), dim3(256), 0, c->stream, src, dst, rows, cols, cs, scale);
+  HIP_TRY(hipGetLastError());
+  return ATUS_OK;
+}
+
+int ensure_tbuf(atus_ctx* c) {
+  if (c->tbuf) return ATUS_OK;
+  if (hipMalloc(&c->tbuf, sizeof(double2) * c->npts * c->ncomp) != hipSuccess)
+    return fail(ATUS_ERR_NOMEM, "transpose buffer");
+  return ATUS_OK;
+}
+
 // full forward (dir -1) or backward (+1) transform with the cft scale,
 // src -> dst (dst may equal src); axes from fastest to slowest
 int full_ft(atus_ctx* c, int dir, const double2* src, double2* dst, int ncomp, long long cs = 0) {
   const double scale = ft_factor(c->dim, dir < 0 ? c->dx : c->dk);
+  if (c->fs) {
+    // natural order in and out: [N1][N2] spectrum <-> frequency k1 + N1 k2
+    ATUS_TRY(ensure_tbuf(c));
+    const long long s_ = cs ? cs : c->npts;
+    // tbuf mirrors psi: dst's offset when dst lies in psi, else its start
+    const uintptr_t d = (uintptr_t)dst, p0 = (uintptr_t)c->psi;
+    const uintptr_t p1 = p0 + sizeof(double2) * (uintptr_t)(c->npts * c->ncomp);
+    double2* t = c->tbuf + ((d >= p0 && d < p1) ? (long long)((d - p0) / sizeof(double2)) : 0);
+    VArgs none{};
+    if (dir < 0) {
+      if (dst != src) HIP_TRY(hipMemcpy2DAsync(dst, s_ * sizeof(double2), src, s_ * sizeof(double2), c->npts * sizeof(double2),
+                                               ncomp, hipMemcpyDeviceToDevice, c->stream));
+      ATUS_TRY(fs_col(c, FS_ENTER, dst, ncomp, s_, V_NONE, 1, false, none, 1.0));
+      {
+        ProfScope ps(c, ATUS_KFAM_PASS_CONTIG);
+        HIP_TRY(launch_pass(c->fs_n2, c->rm[ATUS_KFAM_PASS_CONTIG], true, -1, dst, dst, s_, ncomp, 1, c->fs_n1,
+                            c->tw[c->fs_n2], 1.0, c->max_lds, c->stream));
+      }
+      ATUS_TRY(transpose(c, dst, t, c->fs_n1, c->fs_n2, ncomp, s_, scale));
+      HIP_TRY(hipMemcpy2DAsync(dst, s_ * sizeof(double2), t, s_ * sizeof(double2), c->npts * sizeof(double2), ncomp,
+                               hipMemcpyDeviceToDevice, c->stream));
+    } else {
+      ATUS_TRY(transpose(c, src, t, c->fs_n2, c->fs_n1, ncomp, s_, scale));
+      {
+        ProfScope ps(c, ATUS_KFAM_PASS_CONTIG);
+        HIP_TRY(launch_pass(c->fs_n2, c->rm[ATUS_KFAM_PASS_CONTIG], true, +1, t, t, s_, ncomp, 1, c->fs_n1,
+                            c->tw[c->fs_n2], 1.0, c->max_lds, c->stream));
+      }
+      ATUS_TRY(fs_col(c, FS_EXIT, t, ncomp, s_, V_NONE, 1, false, none, 1.0));
+      HIP_TRY(hipMemcpy2DAsync(dst, s_ * sizeof(double2), t, s_ * sizeof(double2), c->npts * sizeof(double2), ncomp,
+                               hipMemcpyDeviceToDevice, c->stream));
+    }
+    return ATUS_OK;
+  }
   const double2* s = src;
   for (int a = c->dim - 1; a >= 0; --a) {
     ATUS_TRY(pass(c, a, dir, s, dst, ncomp, a == 0 ? scale : 1.0, cs));
@@ -286,6 +374,13 @@ int full_ft(atus_ctx* c, int dir, const double2* src, double2* dst, int ncomp, l
 
 // kinetic step on all components with table slot f (0 half, 1 full)
 int kinetic_unfused(atus_ctx* c, int f) {
+  if (c->fs) {
+    VArgs none{};
+    ATUS_TRY(fs_col(c, FS_ENTER, c->psi, c->ncomp, c->npts, V_NONE, 1, false, none, 1.0));
+    ATUS_TRY(fs_row_kin(c, f, none));
+    ATUS_TRY(fs_col(c, FS_EXIT, c->psi, c->ncomp, c->npts, V_NONE, 1, false, none, 1.0));
+    return ATUS_OK;
+  }
   // forward along the fast axes, fused phase along axis 0, inverse back
   for (int a = c->dim - 1; a >= 1; --a) ATUS_TRY(pass(c, a, -1, c->psi, c->psi, c->ncomp, 1.0));
   const int N0 = c->n[0];
@@ -411,9 +506,10 @@ int ctx_create(atus_ctx** out, int dim, const long long n[3], const double dx[3]
   const int ncomp = nc_run * nbatch;
   for (int a = 0; a < 3; ++a) {
     if (a < dim) {
-      if (!supported_len((int)n[a]) || n[a] > 4096)
+      const bool four_step = dim == 1 && n[a] > 4096 && n[a] <= (1 << 20) && (n[a] & (n[a] - 1)) == 0;
+      if (!four_step && (n[a] > 4096 || !supported_len((int)n[a])))
         return fail(ATUS_ERR_UNSUPPORTED, "axis " + std::to_string(a) + " length " + std::to_string(n[a]) +
-                                              " is not a power of two in [2, 4096]");
+                                              " is not a power of two in [2, 4096] (1D: up to 2^20)");
     } else if (n[a] != 1) {
       return fail(ATUS_ERR_ARG, "unused axes must have length 1");
     }
@@ -435,6 +531,14 @@ int ctx_create(atus_ctx** out, int dim, const long long n[3], const double dx[3]
     atus_ctx_destroy(c);
     return fail(code, m);
   };
+  if (dim == 1 && c->n[0] > 4096) {
+    // N1 = 2^floor(log2(N)/2) <= N2 = N / N1, both in 64..1024
+    int l = 0;
+    while ((1 << (l + 1)) <= c->n[0]) ++l;
+    c->fs = true;
+    c->fs_n1 = 1 << (l / 2);
+    c->fs_n2 = c->n[0] / c->fs_n1;
+  }
   if (hipGetDevice(&c->device) != hipSuccess) return bail(ATUS_ERR_HIP, "no HIP device");
   int lds = 0;
   if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) == hipSuccess && lds > 0)
@@ -448,6 +552,9 @@ int ctx_create(atus_ctx** out, int dim, const long long n[3], const double dx[3]
                                     " bytes)");
   if (hipMemset(c->psi, 0, sizeof(double2) * c->npts * ncomp) != hipSuccess) return bail(ATUS_ERR_HIP, "memset");
   if (hipMalloc(&c->kin, sizeof(double2) * 2 * 3 * c->kin_nmax) != hipSuccess) return bail(ATUS_ERR_NOMEM, "tables");
+  if (c->fs && (hipMalloc(&c->kinz, sizeof(double2) * 2 * c->npts) != hipSuccess ||
+                ensure_twiddles(c, c->fs_n1) != ATUS_OK || ensure_twiddles(c, c->fs_n2) != ATUS_OK))
+    return bail(ATUS_ERR_NOMEM, "four-step tables");
   if (hipMalloc(&c->red_part, sizeof(double) * RED_MAX_BLOCKS * RED_MAX_NV) != hipSuccess ||
       hipMalloc(&c->red_out, sizeof(double) * RED_MAX_NV) != hipSuccess ||
       hipHostMalloc(&c->red_host, sizeof(double) * RED_MAX_NV, hipHostMallocDefault) != hipSuccess)
@@ -489,6 +596,8 @@ int atus_ctx_destroy(atus_ctx* c) {
   (void)hipFree(c->grad);
   (void)hipFree(c->runs);
   (void)hipFree(c->ftab);
+  (void)hipFree(c->kinz);
+  (void)hipFree(c->tbuf);
   (void)hipFree(c->sred_part);
   (void)hipFree(c->sred_out);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -643,7 +752,10 @@ int atus_run_block(atus_ctx* c, const atus_step_params* sp, int Nk, double* t) {
 
   double tt = *t;
   // half kinetic step
-  if (fused && c->dim >= 2) {
+  if (fused && c->fs) {
+    ATUS_TRY(fs_col(c, FS_ENTER, c->psi, c->ncomp, c->npts, V_NONE, 1, false, base, 1.0));
+    ATUS_TRY(fs_row_kin(c, 0, base));
+  } else if (fused && c->dim >= 2) {
     ATUS_TRY(pass(c, c->dim - 1, -1, c->psi, c->psi, c->ncomp, 1.0));
     if (c->dim == 3) ATUS_TRY(pass(c, 1, -1, c->psi, c->psi, c->ncomp, 1.0));
     {
@@ -668,6 +780,9 @@ int atus_run_block(atus_ctx* c, const atus_step_params* sp, int Nk, double* t) {
         HIP_TRY(launch_point(vk, c->ncomp, c->psi, c->grid(), va, c->stream));
       }
       ATUS_TRY(kinetic_unfused(c, f));
+    } else if (c->fs) {
+      ATUS_TRY(fs_col(c, FS_STEP, c->psi, 1, c->npts, vk, c->ncomp, false, va, 1.0));
+      ATUS_TRY(fs_row_kin(c, f, va));
     } else if (c->dim == 1) {
       ProfScope ps(c, ATUS_KFAM_KIN);
       HIP_TRY(launch_kin(N0, c->rm[ATUS_KFAM_KIN], true, vk, c->ncomp, c->psi, c->npts, c->ncomp, 1, c->tw[N0], kin_tabs(c, f), va,
@@ -690,7 +805,11 @@ int atus_run_block(atus_ctx* c, const atus_step_params* sp, int Nk, double* t) {
     }
     tt += (m < Nk) ? dt : 0.5 * dt;
   }
-  if (fused && c->dim >= 2) ATUS_TRY(pass(c, c->dim - 1, +1, c->psi, c->psi, c->ncomp, 1.0));
+  if (fused && c->fs) {
+    ATUS_TRY(fs_col(c, FS_EXIT, c->psi, c->ncomp, c->npts, V_NONE, 1, false, base, 1.0));
+  } else if (fused && c->dim >= 2) {
+    ATUS_TRY(pass(c, c->dim - 1, +1, c->psi, c->psi, c->ncomp, 1.0));
+  }
   if (c->timing) {
     HIP_TRY(hipEventRecord(c->ev1, c->stream));
     HIP_TRY(hipEventSynchronize(c->ev1));
@@ -841,6 +960,7 @@ struct RedMu {
 int atus_sob_setup(atus_ctx* c, const double* alpha_iso, double stepsize) {
   ATUS_TRY(check_ctx(c));
   ATUS_TRY(check_single(c, "atus_sob_setup"));
+  if (c->fs) return fail(ATUS_ERR_UNSUPPORTED, "the Sobolev minimiser covers axes up to 4096 points");
   if (c->ncomp > 2) return fail(ATUS_ERR_UNSUPPORTED, "Sobolev minimiser supports 1 or 2 wavefunctions");
   if (!alpha_iso) return fail(ATUS_ERR_ARG, "null alpha");
   if (!c->pot) return fail(ATUS_ERR_STATE, "set the potential before atus_sob_setup");
@@ -1087,7 +1207,10 @@ int atus_batch_run_block(atus_ctx* c, int kind, double dt, int Nk, double* t, co
   if (c->timing) HIP_TRY(hipEventRecord(c->ev0, c->stream));
 
   double tt = *t;
-  if (c->dim >= 2) {
+  if (c->fs) {
+    ATUS_TRY(fs_col(c, FS_ENTER, c->psi, c->ncomp, c->npts, V_NONE, 1, false, base, 1.0));
+    ATUS_TRY(fs_row_kin(c, 0, base));
+  } else if (c->dim >= 2) {
     ATUS_TRY(pass(c, c->dim - 1, -1, c->psi, c->psi, c->ncomp, 1.0));
     if (c->dim == 3) ATUS_TRY(pass(c, 1, -1, c->psi, c->psi, c->ncomp, 1.0));
     {
@@ -1105,7 +1228,10 @@ int atus_batch_run_block(atus_ctx* c, int kind, double dt, int Nk, double* t, co
     va.t1 = tt + 0.5 * dt;
     va.Fm = c->ftab + (size_t)(m - 1) * B;
     const int f = (m < Nk) ? 1 : 0;
-    if (c->dim == 1) {
+    if (c->fs) {
+      ATUS_TRY(fs_col(c, FS_STEP, c->psi, B, c->npts, vk, 2, true, va, 1.0));
+      ATUS_TRY(fs_row_kin(c, f, va));
+    } else if (c->dim == 1) {
       ProfScope ps(c, ATUS_KFAM_KIN);
       HIP_TRY(launch_vstep_batch(N0, c->rm[ATUS_KFAM_KIN], true, vk, c->psi, c->npts, B, c->tw[N0], kin_tabs(c, f), va,
                                  1, N0, c->dx[0], c->max_lds, c->stream));
@@ -1127,7 +1253,11 @@ int atus_batch_run_block(atus_ctx* c, int kind, double dt, int Nk, double* t, co
     }
     tt += (m < Nk) ? dt : 0.5 * dt;
   }
-  if (c->dim >= 2) ATUS_TRY(pass(c, c->dim - 1, +1, c->psi, c->psi, c->ncomp, 1.0));
+  if (c->fs) {
+    ATUS_TRY(fs_col(c, FS_EXIT, c->psi, c->ncomp, c->npts, V_NONE, 1, false, base, 1.0));
+  } else if (c->dim >= 2) {
+    ATUS_TRY(pass(c, c->dim - 1, +1, c->psi, c->psi, c->ncomp, 1.0));
+  }
   if (c->timing) {
     HIP_TRY(hipEventRecord(c->ev1, c->stream));
     HIP_TRY(hipEventSynchronize(c->ev1));

@@ -19,14 +19,14 @@ static hipError_t kin_go(double2* psi, long long cs, int ycount, long long inner
 template <int N, int RM>
 static hipError_t kin_nr(bool contig, int vk, int nc, double2* psi, long long cs, int ncomp, long long inner,
                         const double2* tw, const KinTabs& kt, const VArgs& va, double dx0, size_t max_lds,
-                        hipStream_t s) {
+                        hipStream_t s, long long rows) {
   if (!contig) {
     // 2D/3D: axis 0 is strided, one component per blockIdx.y, no V prologue
     if (vk != V_NONE) return hipErrorInvalidValue;
     return kin_go<N, RM, false, V_NONE, 1>(psi, cs, ncomp, inner, 1, tw, kt, va, dx0, max_lds, s);
   }
   // 1D: the pencil is the whole grid
-  if (vk == V_NONE) return kin_go<N, RM, true, V_NONE, 1>(psi, cs, ncomp, 1, 1, tw, kt, va, dx0, max_lds, s);
+  if (vk == V_NONE) return kin_go<N, RM, true, V_NONE, 1>(psi, cs, ncomp, 1, rows, tw, kt, va, dx0, max_lds, s);
   if (nc == 1) {
     if (vk == V_NL) return kin_go<N, RM, true, V_NL, 1>(psi, cs, 1, 1, 1, tw, kt, va, dx0, max_lds, s);
     if (vk == V_NL_IF) return kin_go<N, RM, true, V_NL_IF, 1>(psi, cs, 1, 1, 1, tw, kt, va, dx0, max_lds, s);
@@ -41,17 +41,18 @@ static hipError_t kin_nr(bool contig, int vk, int nc, double2* psi, long long cs
 template <int N>
 static hipError_t kin_n(int rm, bool contig, int vk, int nc, double2* psi, long long cs, int ncomp, long long inner,
                         const double2* tw, const KinTabs& kt, const VArgs& va, double dx0, size_t max_lds,
-                        hipStream_t s) {
-  if (rm == 8 && N > 8 && N / 8 <= 256) return kin_nr<N, 8>(contig, vk, nc, psi, cs, ncomp, inner, tw, kt, va, dx0, max_lds, s);
-  return kin_nr<N, 16>(contig, vk, nc, psi, cs, ncomp, inner, tw, kt, va, dx0, max_lds, s);
+                        hipStream_t s, long long rows) {
+  if (rm == 8 && N > 8 && N / 8 <= 256)
+    return kin_nr<N, 8>(contig, vk, nc, psi, cs, ncomp, inner, tw, kt, va, dx0, max_lds, s, rows);
+  return kin_nr<N, 16>(contig, vk, nc, psi, cs, ncomp, inner, tw, kt, va, dx0, max_lds, s, rows);
 }
 
 hipError_t launch_kin(int N, int rm, bool contig, int vk, int nc, double2* psi, long long comp_stride, int ncomp,
                       long long inner, const double2* tw, const KinTabs& kt, const VArgs& va, double dx0,
-                      size_t max_lds, hipStream_t s) {
+                      size_t max_lds, hipStream_t s, long long rows) {
   switch (N) {
 #define ATUS_CASE(n) \
-  case n: return kin_n<n>(rm, contig, vk, nc, psi, comp_stride, ncomp, inner, tw, kt, va, dx0, max_lds, s);
+  case n: return kin_n<n>(rm, contig, vk, nc, psi, comp_stride, ncomp, inner, tw, kt, va, dx0, max_lds, s, rows);
     ATUS_CASE(2) ATUS_CASE(4) ATUS_CASE(8) ATUS_CASE(16) ATUS_CASE(32) ATUS_CASE(64) ATUS_CASE(128)
     ATUS_CASE(256) ATUS_CASE(512) ATUS_CASE(1024) ATUS_CASE(2048) ATUS_CASE(4096)
 #undef ATUS_CASE

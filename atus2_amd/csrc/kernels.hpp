@@ -329,6 +329,10 @@ struct KinTabs {
   const double2* e1;  // n1 (NULL in 1D)
   const double2* e2;  // n2 (NULL in 1D/2D)
   int n2;
+  // contiguous pencils only: a phase per pencil element and pencil row,
+  // ez[(pid % ez_rows) * N + j] (the four-step 1D spectrum), else NULL
+  const double2* ez;
+  long long ez_rows;
 };
 
 template <int N, int RM, bool CONTIG, int VK, int NC, bool BATCH = false>
@@ -382,10 +386,18 @@ __global__ __launch_bounds__(256) void k_kin(double2* __restrict__ psi, long lon
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     fft_pencil<N, RM, -1, WAVE>(v[c], m.t, lds, twl);
+    const double2* ezr = nullptr;
+    if constexpr (CONTIG) {
+      if (kt.ez) ezr = kt.ez + (m.pid % kt.ez_rows) * N;
+    }
 #pragma unroll
     for (int i = 0; i < R; ++i) {
-      double2 e = e0l[m.t + P * i];
-      if constexpr (!CONTIG) e = cmul(e, cp);
+      double2 e;
+      if constexpr (CONTIG) {
+        e = ezr ? ezr[m.t + P * i] : e0l[m.t + P * i];
+      } else {
+        e = cmul(e0l[m.t + P * i], cp);
+      }
       v[c][i] = cmul(v[c][i], e);
     }
     fft_pencil<N, RM, +1, WAVE>(v[c], m.t, lds, twl);
@@ -438,6 +450,88 @@ __global__ __launch_bounds__(256) void k_vfuse(double2* __restrict__ psi, long l
   for (int c = 0; c < NC; ++c) {
     fft_pencil<N, RM, -1, WAVE>(v[c], m.t, lds, twl);
     store_pencil<N, RM>(v[c], psi + c * comp_stride, m.base, 1, m.t, m.valid);
+  }
+}
+
+// ------------------------------------------------------------ four-step 1D
+// A 1D grid of N = N1 * N2 points is the matrix [N1][N2], j = j1 N2 + j2.
+// Forward transform: DFT_N1 down every column j2, times W_N^(j2 k1)
+// (W_N = exp(-2 pi i / N); j2 k1 < N), then DFT_N2 along every row k1, after
+// which row k1, column k2 holds frequency k1 + N1 k2.  The inverse runs
+// the same steps backwards with conjugate factors.  k_fs_col is the column
+// half; the row half is a contiguous pass.
+//   FS_ENTER: x -> Y (DFT_N1, twiddle)
+//   FS_EXIT:  Y -> x (conjugate twiddle, inverse DFT_N1, times scale)
+//   FS_STEP:  Y -> Y with the V step on the x-space values in between
+// Components: NC == 1 and no batch: blockIdx.y is the component; otherwise
+// blockIdx.y is the run (batch) or 0, and each thread holds NC components.
+enum { FS_ENTER = 0, FS_EXIT = 1, FS_STEP = 2 };
+
+template <int N1, int RM, int MODE, int VK, int NC, bool BATCH>
+__global__ __launch_bounds__(256) void k_fs_col(double2* __restrict__ psi, long long comp_stride, AxisGeo g,
+                                                const double2* __restrict__ tw1, const double2* __restrict__ wN,
+                                                VArgs va, double dx, double scale) {
+  extern __shared__ double2 lds_all[];
+  constexpr int R = Plan<N1, RM>::R, P = Plan<N1, RM>::P;
+  PencilMap<N1, RM, false> m(g);
+  double2* lds = lds_all + m.p * lds_stride<N1>();
+  const double2* twl = stage_table<N1>(lds_all + (size_t)g.T * lds_stride<N1>(), tw1);
+  const long long N2 = g.inner;
+  const long long N = N2 * N1;
+  const long long j2 = m.pid % N2;
+  long long c0 = 0;
+  if constexpr (BATCH) {
+    static_assert(MODE == FS_STEP && NC == 2, "batches carry the V step of their run");
+    bind_run(va, blockIdx.y);
+    c0 = (long long)blockIdx.y * (va.run_shift + comp_stride);
+  } else if constexpr (NC == 1) {
+    c0 = (long long)blockIdx.y * comp_stride;
+  }
+  double2 v[NC][R];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) load_pencil<N1, RM>(v[c], psi + c0 + c * comp_stride, m.base, m.stride, m.t, m.valid);
+  if constexpr (tables_in_lds<N1>()) __syncthreads();
+  if constexpr (MODE != FS_ENTER) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+#pragma unroll
+      for (int i = 0; i < R; ++i) v[c][i] = cmulc(v[c][i], wN[j2 * (m.t + P * i)]);
+      fft_pencil<N1, RM, +1, false>(v[c], m.t, lds, twl);
+    }
+  }
+  if constexpr (MODE == FS_EXIT) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      if (scale != 1.0) {
+#pragma unroll
+        for (int i = 0; i < R; ++i) v[c][i] = cscale(v[c][i], scale);
+      }
+      store_pencil<N1, RM>(v[c], psi + c0 + c * comp_stride, m.base, m.stride, m.t, m.valid);
+    }
+    return;
+  }
+  if constexpr (MODE == FS_STEP) {
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      const long long j = (long long)(m.t + P * i) * N2 + j2;
+      const double x0 = (double)(j - N / 2) * dx;
+      double2 a[NC];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) a[c] = v[c][i];
+      double Om = 0.0;
+      double2 eta = make_double2(1.0, 0.0);
+      if constexpr (VK == V_BRAGG) bragg_field(x0, va, Om, eta);
+      if (m.valid) v_point<VK, NC>(a, x0, j, va, Om, eta);
+#pragma unroll
+      for (int c = 0; c < NC; ++c) v[c][i] = a[c];
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    fft_pencil<N1, RM, -1, false>(v[c], m.t, lds, twl);
+#pragma unroll
+    for (int i = 0; i < R; ++i) v[c][i] = cmul(v[c][i], wN[j2 * (m.t + P * i)]);
+    store_pencil<N1, RM>(v[c], psi + c0 + c * comp_stride, m.base, m.stride, m.t, m.valid);
   }
 }
 

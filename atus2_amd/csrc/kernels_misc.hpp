@@ -111,4 +111,33 @@ __global__ __launch_bounds__(256) void k_scale(double2* __restrict__ psi, long l
   for (long long l = (long long)blockIdx.x * blockDim.x + threadIdx.x; l < n; l += stride) psi[l] = cscale(psi[l], f);
 }
 
+
+// dst = scale * transpose(src) for `ncomp` matrices of rows x cols, each
+// comp_stride elements apart in both buffers; 32 x 32 tiles through LDS,
+// one tile per workgroup (blockIdx.x = (comp * row_tiles + i) * col_tiles + j)
+__global__ __launch_bounds__(256) void k_transpose(const double2* __restrict__ src, double2* __restrict__ dst,
+                                                   long long rows, long long cols, long long comp_stride,
+                                                   double scale) {
+  __shared__ double2 tile[32][33];
+  const long long ct = (cols + 31) / 32, rt = (rows + 31) / 32;
+  const long long b = blockIdx.x;
+  const long long comp = b / (rt * ct), rem = b % (rt * ct);
+  const long long i0 = (rem / ct) * 32, j0 = (rem % ct) * 32;
+  src += comp * comp_stride;
+  dst += comp * comp_stride;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int k = ty; k < 32; k += 8) {
+    const long long i = i0 + k, j = j0 + tx;
+    if (i < rows && j < cols) tile[k][tx] = src[i * cols + j];
+  }
+  __syncthreads();
+  for (int k = ty; k < 32; k += 8) {
+    const long long j = j0 + k, i = i0 + tx;
+    if (i < rows && j < cols) {
+      const double2 a = tile[tx][k];
+      dst[j * rows + i] = make_double2(a.x * scale, a.y * scale);
+    }
+  }
+}
+
 }  // namespace atus

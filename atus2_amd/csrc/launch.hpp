@@ -45,7 +45,7 @@ hipError_t launch_pass(int N, int rm, bool contig, int dir, const double2* src, 
 // kinetic step along axis 0; vk/nc select the fused V prologue (1D only)
 hipError_t launch_kin(int N, int rm, bool contig, int vk, int nc, double2* psi, long long comp_stride, int ncomp,
                       long long inner, const double2* tw, const KinTabs& kt, const VArgs& va, double dx0,
-                      size_t max_lds, hipStream_t s);
+                      size_t max_lds, hipStream_t s, long long rows = 1);
 
 // inverse / V / forward along the last (contiguous) axis of a 2D/3D grid
 hipError_t launch_vfuse(int N, int rm, int vk, int nc, double2* psi, long long comp_stride, long long outer,
@@ -60,6 +60,13 @@ hipError_t launch_vfuse(int N, int rm, int vk, int nc, double2* psi, long long c
 hipError_t launch_vstep_batch(int N, int rm, bool one_d, int vk, double2* psi, long long comp_stride, long long count,
                               const double2* tw, const KinTabs& kt, const VArgs& va, long long pencils_per_x0, int n0,
                               double dx0, size_t max_lds, hipStream_t s);
+
+// column pass of the four-step 1D transform (k_fs_col): n1 in 64..1024,
+// mode FS_ENTER / FS_EXIT / FS_STEP; ycount components (NC == 1, no
+// batch), runs (batch) or 1
+hipError_t launch_fs_col(int n1, int rm, int mode, int vk, int nc, bool batch, double2* psi, long long comp_stride,
+                         int ycount, long long n2, const double2* tw1, const double2* wN, const VArgs& va, double dx,
+                         double scale, size_t max_lds, hipStream_t s);
 
 // Sobolev filters along axis 0
 hipError_t launch_sobf(int N, int rm, bool contig, const double2* src, double2* out_sob, double2* out_lap,

@@ -13,7 +13,8 @@
  * optional potentials and one HIP stream.  Calls on one context must come
  * from one thread at a time (as the reference's solver objects, which are
  * not thread-safe).  Grid axes must be powers of two between 2 and 4096
- * (1 for unused axes).
+ * (1 for unused axes); a 1D grid may have up to 2^20 points (four-step
+ * transform).  The Sobolev minimiser covers axes up to 4096.
  *
  * Each declaration names the reference interface it replaces.
  */

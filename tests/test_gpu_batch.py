@@ -44,8 +44,8 @@ def _initial(orc, g, r):
     return base, np.zeros_like(base)
 
 
-@pytest.mark.parametrize("shape,rm", [((1024,), 16), ((256,), 8), ((64, 128), 16), ((32, 64), 8),
-                                      ((16, 8, 32), 16)])
+@pytest.mark.parametrize("shape,rm", [((1024,), 16), ((256,), 8), ((16384,), 16), ((8192,), 8), ((64, 128), 16),
+                                      ((32, 64), 8), ((16, 8, 32), 16)])
 @pytest.mark.parametrize("kind", ["bragg_ad", "freeprop_if"])
 def test_batch_matches_single_runs_and_oracle(hip, orc, shape, rm, kind):
     dim = len(shape)

@@ -28,8 +28,9 @@ def rand_field(shape, seed, smooth=True):
     return psi
 
 
-SHAPES = [(2,), (64,), (4096,), (16, 32), (128, 8), (2048, 4), (8, 16, 32), (32, 32, 32), (4, 256, 16),
-          (64, 4, 512)]
+# 1D lengths above 4096 take the four-step transform
+SHAPES = [(2,), (64,), (4096,), (8192,), (16384,), (1 << 20,), (16, 32), (128, 8), (2048, 4), (8, 16, 32),
+          (32, 32, 32), (4, 256, 16), (64, 4, 512)]
 
 
 @pytest.mark.parametrize("rm", [16, 8])
@@ -51,7 +52,7 @@ def test_ft_matches_oracle(hip, orc, shape, rm):
     assert rel_l2(back, psi) < 1e-13
 
 
-@pytest.mark.parametrize("shape", [(64,), (4096,), (32, 64), (8, 16, 32), (64, 64, 64)])
+@pytest.mark.parametrize("shape", [(64,), (4096,), (8192,), (65536,), (32, 64), (8, 16, 32), (64, 64, 64)])
 @pytest.mark.parametrize("frac", [0.5, 1.0])
 @pytest.mark.parametrize("rm", [16, 8])
 def test_kinetic_matches_oracle(hip, orc, shape, frac, rm):
@@ -118,8 +119,8 @@ def _state(orc, g, seed, ncomp):
     return out
 
 
-@pytest.mark.parametrize("shape,rm", [((4096,), 16), ((256,), 8), ((64, 128), 16), ((32, 16, 64), 16),
-                                      ((32, 16, 64), 8)])
+@pytest.mark.parametrize("shape,rm", [((4096,), 16), ((256,), 8), ((16384,), 16), ((8192,), 8), ((64, 128), 16),
+                                      ((32, 16, 64), 16), ((32, 16, 64), 8)])
 @pytest.mark.parametrize("kind", ["freeprop", "freeprop_if", "bragg_ad"])
 def test_run_block_matches_oracle(hip, orc, shape, rm, kind):
     dim = len(shape)
