@@ -129,3 +129,15 @@ def test_batch_rejects_single_run_calls(hip):
             B.run_batch_block(hip.STEP_FREEPROP, 1e-3, 2, 0.0)
         with pytest.raises(ValueError):
             B.run_batch_block(hip.STEP_BRAGG_AD, 1e-3, 2, 0.0, np.ones((3, 3)))
+
+
+def test_scan_batched_matches_one_after_another(hip):
+    from atus2_amd.scan import run_batched, run_single, scan_runs
+    runs = scan_runs(seed=7, n1d=6, n2d=2)
+    rb, _ = run_batched(runs, steps=300)
+    rs, _ = run_single(runs, steps=300)
+    for i in range(len(runs)):
+        assert rb[i].shape == (3, 3)
+        # populations agree; the amplitudes come from numpy's and libm's exp
+        # (a last-bit difference, amplified near the nodes of Omega)
+        assert np.max(np.abs(rb[i] - rs[i])) < 1e-11, (i, runs[i].shape)
