@@ -11,31 +11,16 @@
 #include <vector>
 
 #include "../../include/atus_hip.h"
+#include "ctx.hpp"
 #include "kernels_misc.hpp"
-#include "launch.hpp"
 
 using namespace atus;
+using atus_host::fail;
+using atus_host::g_err;
+
+thread_local std::string atus_host::g_err;
 
 namespace {
-
-thread_local std::string g_err;
-
-int fail(int code, const std::string& msg) {
-  g_err = msg;
-  return code;
-}
-
-#define HIP_TRY(expr)                                                                          \
-  do {                                                                                         \
-    hipError_t e_ = (expr);                                                                    \
-    if (e_ != hipSuccess) return fail(ATUS_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-  } while (0)
-
-#define ATUS_TRY(expr)     \
-  do {                     \
-    int r_ = (expr);       \
-    if (r_ != ATUS_OK) return r_; \
-  } while (0)
 
 // the scale() factor of cft_Nd (cft_1d.cpp:193, cft_2d.cpp:451, cft_3d.cpp:629)
 double ft_factor(int dim, const double* s) {
@@ -62,101 +47,7 @@ unsigned ew_blocks(long long n) {
 
 }  // namespace
 
-struct atus_ctx {
-  int dim = 0;
-  int n[3] = {1, 1, 1};
-  double dx[3] = {0, 0, 0};
-  double dk[3] = {0, 0, 0};
-  int ncomp = 0;            // all components held (nbatch * nc_run)
-  long long npts = 0;
-  int nbatch = 1;           // independent runs (atus_batch_create)
-  int nc_run = 0;           // components per run
-  // 1D grids longer than 4096: four-step transform on [fs_n1][fs_n2]
-  bool fs = false;
-  int fs_n1 = 0, fs_n2 = 0;
-  double2* kinz = nullptr;  // [2 fracs][N] kinetic phases in [k1][k2] order
-  double2* tbuf = nullptr;  // ncomp * npts, the transposes of full transforms
-  int device = 0;
-  hipStream_t stream = nullptr;
-  size_t max_lds = 65536;
 
-  double2* psi = nullptr;   // ncomp * npts
-  double* pot = nullptr;    // ncomp * npts, or null
-  double2* scratch = nullptr;
-  std::map<int, double2*> tw;
-
-  double2* kin = nullptr;   // [2 fracs][3 axes][nmax]
-  int kin_nmax = 0;
-  double kin_dt = NAN;
-  bool kin_valid = false;
-  double alpha[3] = {0, 0, 0};
-  double gs[16] = {0};
-
-  double* red_part = nullptr;
-  double* red_out = nullptr;
-  double* red_host = nullptr;
-
-  // batches: per-run parameters, the amplitude table of one block and
-  // the segmented-reduction buffers
-  RunParams* runs = nullptr;
-  bool runs_set = false;
-  double* ftab = nullptr;
-  size_t ftab_cap = 0;
-  double* sred_part = nullptr;
-  double* sred_out = nullptr;
-  size_t sred_cap = 0;
-
-  bool sob_ready = false;
-  double sob_alpha[2] = {0, 0};
-  double sob_step = 0;
-  double2 *sob = nullptr, *lap = nullptr, *grad = nullptr;
-
-  bool timing = false;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  double last_ms = 0;
-
-  // values per thread of the FFT kernels, per kernel family (ATUS_KFAM_*)
-  int rm[ATUS_NKFAM] = {16, 16, 16, 16, 16};
-
-  // per-launch event pairs while profiling
-  bool prof = false;
-  std::vector<hipEvent_t> prof_ev;
-  std::vector<int> prof_fam;
-
-  Grid grid() const {
-    Grid g{};
-    g.dim = dim;
-    for (int a = 0; a < 3; ++a) {
-      g.n[a] = n[a];
-      g.dx[a] = dx[a];
-      g.dk[a] = dk[a];
-    }
-    g.npts = npts;
-    return g;
-  }
-  double ar() const {
-    double a = dx[0];
-    if (dim > 1) a *= dx[1];
-    if (dim > 2) a *= dx[2];
-    return a;
-  }
-  double ar_k() const {
-    double a = dk[0];
-    if (dim > 1) a *= dk[1];
-    if (dim > 2) a *= dk[2];
-    return a;
-  }
-  long long inner_of(int a) const {
-    long long v = 1;
-    for (int b = a + 1; b < dim; ++b) v *= n[b];
-    return v;
-  }
-  long long outer_of(int a) const {
-    long long v = 1;
-    for (int b = 0; b < a; ++b) v *= n[b];
-    return v;
-  }
-};
 
 namespace {
 
@@ -1055,6 +946,22 @@ int atus_sob_mu(atus_ctx* c, double* mu) {
     ATUS_TRY(reduce(c, fn, c->npts, &s));
     mu[i] = c->ar() * s;
   }
+  return ATUS_OK;
+}
+
+int atus_sob_run(atus_ctx* c, double eps, long long maxit, long long* iters, double* res_tot) {
+  ATUS_TRY(check_ctx(c));
+  ATUS_TRY(check_single(c, "atus_sob_run"));
+  if (!(eps >= 0.0)) return fail(ATUS_ERR_ARG, "eps must be >= 0");
+  long long it = 0;
+  double r = 0.0;
+  do {
+    int rn = 0;
+    ATUS_TRY(atus_sob_iter(c, &r, &rn, nullptr));
+    ++it;
+  } while (r > eps && (maxit <= 0 || it < maxit));
+  if (iters) *iters = it;
+  if (res_tot) *res_tot = r;
   return ATUS_OK;
 }
 

@@ -24,7 +24,9 @@ EXPORTED = [
     "atus_rabi_integrals", "atus_energy", "atus_sob_setup", "atus_sob_renormalize", "atus_sob_iter",
     "atus_sob_mu", "atus_set_timing", "atus_last_block_ms", "atus_run_block_profiled", "atus_copy_bandwidth", "atus_tune",
     "atus_batch_create", "atus_batch_set_runs", "atus_batch_run_block", "atus_batch_particle_numbers",
-    "atus_batch_rabi_integrals",
+    "atus_batch_rabi_integrals", "atus_sob_run",
+    "atus_net_create", "atus_net_destroy", "atus_net_param_count", "atus_net_get_params", "atus_net_set_params",
+    "atus_net_forward", "atus_net_gradients", "atus_net_set_data", "atus_net_train", "atus_net_warm_start",
 ]
 
 KFAM = ["pass_contig", "pass_strided", "kin", "vfuse", "point"]
@@ -105,6 +107,17 @@ def load():
         "atus_batch_run_block": (I, [P, I, D, I, C.POINTER(D), P]),
         "atus_batch_particle_numbers": (I, [P, P]),
         "atus_batch_rabi_integrals": (I, [P, I, P, I, D, P]),
+        "atus_sob_run": (I, [P, D, C.c_longlong, C.POINTER(C.c_longlong), C.POINTER(D)]),
+        "atus_net_create": (I, [pctx, I, I, I, I, I, I, C.c_ulonglong]),
+        "atus_net_destroy": (I, [P]),
+        "atus_net_param_count": (I, [P, C.POINTER(C.c_longlong)]),
+        "atus_net_get_params": (I, [P, P]),
+        "atus_net_set_params": (I, [P, P]),
+        "atus_net_forward": (I, [P, P, I, P]),
+        "atus_net_gradients": (I, [P, P, P, I, C.POINTER(C.c_float), P]),
+        "atus_net_set_data": (I, [P, P, P, I]),
+        "atus_net_train": (I, [P, I, I, C.c_float, C.c_float, C.c_ulonglong, P]),
+        "atus_net_warm_start": (I, [P, P, C.POINTER(D), P, I]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -302,6 +315,14 @@ class Context:
         _check(_lib.atus_sob_iter(self._h, C.byref(r), C.byref(rn), N))
         return r.value, bool(rn.value), list(N[:self.ncomp])
 
+    def sob_run(self, eps, maxit=0):
+        """CSOB_Base::run: iterations until res_tot <= eps (maxit > 0 caps
+        them); returns (iterations, res_tot)."""
+        it = C.c_longlong()
+        r = C.c_double()
+        _check(_lib.atus_sob_run(self._h, float(eps), int(maxit), C.byref(it), C.byref(r)))
+        return it.value, r.value
+
     def sob_mu(self):
         out = np.zeros(self.ncomp)
         _check(_lib.atus_sob_mu(self._h, _dp(out)))
@@ -408,3 +429,75 @@ class BatchContext(Context):
         _check(_lib.atus_batch_rabi_integrals(self._h, comp, _dp(np.ascontiguousarray(k0)), len(k0list),
                                               float(threshold), _dp(out)))
         return out
+
+
+class Net:
+    """The warm-start perceptron (atus_net_*): dims = (n_in, h1, h2, h3, n_out)."""
+
+    def __init__(self, dims, max_batch=256, seed=1):
+        L = load()
+        h = C.c_void_p()
+        _check(L.atus_net_create(C.byref(h), *[int(d) for d in dims], int(max_batch), int(seed)))
+        self._h = h
+        self.dims = tuple(int(d) for d in dims)
+        n = C.c_longlong()
+        _check(L.atus_net_param_count(h, C.byref(n)))
+        self.nparams = n.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.atus_net_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def get_params(self):
+        out = np.empty(self.nparams, np.float32)
+        _check(_lib.atus_net_get_params(self._h, _dp(out)))
+        return out
+
+    def set_params(self, p):
+        a = np.ascontiguousarray(p, dtype=np.float32)
+        if a.size != self.nparams:
+            raise ValueError(f"{a.size} parameters, expected {self.nparams}")
+        _check(_lib.atus_net_set_params(self._h, _dp(a)))
+
+    def forward(self, x):
+        a = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, self.dims[0])
+        y = np.empty((a.shape[0], self.dims[4]), np.float32)
+        _check(_lib.atus_net_forward(self._h, _dp(a), a.shape[0], _dp(y)))
+        return y
+
+    def gradients(self, x, t):
+        a = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, self.dims[0])
+        b = np.ascontiguousarray(t, dtype=np.float32).reshape(a.shape[0], self.dims[4])
+        g = np.empty(self.nparams, np.float32)
+        loss = C.c_float()
+        _check(_lib.atus_net_gradients(self._h, _dp(a), _dp(b), a.shape[0], C.byref(loss), _dp(g)))
+        return loss.value, g
+
+    def set_data(self, x, t):
+        a = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, self.dims[0])
+        b = np.ascontiguousarray(t, dtype=np.float32).reshape(a.shape[0], self.dims[4])
+        _check(_lib.atus_net_set_data(self._h, _dp(a), _dp(b), a.shape[0]))
+
+    def train(self, steps, batch, lr, lr_final=0.0, seed=1):
+        losses = np.empty(steps, np.float32)
+        _check(_lib.atus_net_train(self._h, int(steps), int(batch), float(lr), float(lr_final), int(seed),
+                                   _dp(losses)))
+        return losses
+
+    def warm_start(self, x, half_width, ctx, comp=0):
+        a = np.ascontiguousarray(x, dtype=np.float32).reshape(self.dims[0])
+        hw = (C.c_double * 3)(*[float(v) for v in half_width])
+        _check(_lib.atus_net_warm_start(self._h, _dp(a), hw, ctx._h, int(comp)))

@@ -146,6 +146,10 @@ int atus_sob_renormalize(atus_ctx *ctx);
 int atus_sob_iter(atus_ctx *ctx, double *res_tot, int *renormalised, double *N);
 /* Compute_mu (CSOB_Base.h:400-424, with Get_Ar() = m_ar): ncomp values */
 int atus_sob_mu(atus_ctx *ctx, double *mu);
+/* CSOB_Base::run (CSOB_Base.h:427-505) after Renormalize_All_Psi: loop
+ * bodies until res_tot <= eps; maxit > 0 caps the count (the reference
+ * loops without a cap).  Outputs: iterations done, last res_tot. */
+int atus_sob_run(atus_ctx *ctx, double eps, long long maxit, long long *iters, double *res_tot);
 
 /* ---- timing helper: average device time (ms) of the last atus_run_block
  * call's kernels, measured with HIP events on the context's stream; 0 if
@@ -202,6 +206,37 @@ int atus_batch_particle_numbers(atus_ctx *ctx, double *out);
 /* compute_rabi_integrals of component `comp` of every run, transformed in
  * place and back as in CRT_Base_IF.h:208-281: out[r*n + q] */
 int atus_batch_rabi_integrals(atus_ctx *ctx, int comp, const double *k0, int n, double threshold, double *out);
+
+/* ---- learned warm start of the Sobolev minimiser (no reference
+ * counterpart: the network only proposes the start state, which
+ * atus_sob_run then polishes to the same residual).  A perceptron
+ * n_in -> h1 -> h2 -> h3 -> n_out, tanh hidden layers, linear output, fp32
+ * parameters laid out as [W1 (n_in x h1, row-major), b1, W2, b2, W3, b3,
+ * W4, b4]; matrix products round their operands to bf16 and accumulate in
+ * fp32 on the matrix cores; training minimises the mean squared error with
+ * Adam (beta1 0.9, beta2 0.999, eps 1e-8). */
+typedef struct atus_net atus_net;
+int atus_net_create(atus_net **out, int n_in, int h1, int h2, int h3, int n_out, int max_batch,
+                    unsigned long long seed);
+int atus_net_destroy(atus_net *net);
+int atus_net_param_count(atus_net *net, long long *count);
+int atus_net_get_params(atus_net *net, float *host);
+/* also resets the optimiser state */
+int atus_net_set_params(atus_net *net, const float *host);
+/* y (batch x n_out) from x (batch x n_in), host arrays */
+int atus_net_forward(atus_net *net, const float *x, int batch, float *y);
+/* loss and, if grads != NULL, its gradient w.r.t. every parameter */
+int atus_net_gradients(atus_net *net, const float *x, const float *t, int batch, float *loss, float *grads);
+/* device-resident training set: count samples of inputs and targets */
+int atus_net_set_data(atus_net *net, const float *x, const float *t, int count);
+/* `steps` Adam steps on shuffled batches of the resident set, learning
+ * rate decaying from lr to lr_final on a cosine; losses[step] if non-NULL */
+int atus_net_train(atus_net *net, int steps, int batch, float lr, float lr_final, unsigned long long seed,
+                   float *losses);
+/* component comp of a 3D context <- the network's 32^3 output for input x,
+ * read as values at x_j = (j - 16) half_width[a] / 16 per axis and
+ * interpolated trilinearly onto the context's grid (0 outside) */
+int atus_net_warm_start(atus_net *net, const float *x, const double half_width[3], atus_ctx *ctx, int comp);
 
 #ifdef __cplusplus
 }

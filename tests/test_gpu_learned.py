@@ -1,0 +1,122 @@
+"""The warm-start network kernels (BF16 MFMA products, FP32 accumulation)
+against the CPU restatement, the warm-start interpolation, and the
+minimiser's converged chemical potential against the oracle."""
+import numpy as np
+import pytest
+
+from oracle import mlp_oracle as M
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def hip():
+    from atus2_amd import hip as H
+    H.load()
+    return H
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / np.linalg.norm(np.ravel(b)))
+
+
+@pytest.mark.parametrize("dims,batch", [((4, 32, 48, 16, 96), 40), ((4, 128, 128, 32, 4096), 64)])
+def test_net_forward_and_gradients_match_oracle(hip, dims, batch):
+    rng = np.random.default_rng(3)
+    with hip.Net(dims, max_batch=batch, seed=9) as net:
+        p = net.get_params()
+        _, n = M.layout(dims)
+        assert p.size == n
+        # biases nonzero too
+        p = (p + 0.05 * rng.standard_normal(n)).astype(np.float32)
+        net.set_params(p)
+        x = rng.uniform(-1, 1, (batch, dims[0])).astype(np.float32)
+        t = rng.standard_normal((batch, dims[4])).astype(np.float32)
+        y = net.forward(x)
+        ref = M.forward(p, dims, x, bf16=True)[4]
+        assert rel(y, ref) < 1e-5
+        loss, g = net.gradients(x, t)
+        lref, gref = M.loss_and_gradients(p, dims, x, t, bf16=True)
+        assert abs(loss / lref - 1) < 1e-5
+        assert rel(g, gref) < 1e-3
+        # per layer, so that a transposed or misplaced block cannot hide
+        lay, _ = M.layout(dims)
+        for wo, ws, bo, bn in lay:
+            assert rel(g[wo:wo + ws[0] * ws[1]], gref[wo:wo + ws[0] * ws[1]]) < 2e-3
+            assert rel(g[bo:bo + bn], gref[bo:bo + bn]) < 2e-3
+
+
+def test_net_training_fits_a_smooth_family(hip):
+    rng = np.random.default_rng(4)
+    dims = (4, 64, 64, 16, 256)
+    x = rng.uniform(-1, 1, (1024, 4)).astype(np.float32)
+    grid = np.linspace(-1, 1, 256)
+    t = np.exp(-((grid[None, :] - 0.5 * x[:, :1]) ** 2) / (0.2 + 0.1 * (x[:, 1:2] + 1))).astype(np.float32)
+    with hip.Net(dims, max_batch=128, seed=2) as net:
+        net.set_data(x, t)
+        losses = net.train(600, 128, 3e-3, 3e-5, seed=1)
+    assert np.all(np.isfinite(losses))
+    assert losses[-20:].mean() < 0.05 * losses[:5].mean()
+
+
+def test_warm_start_interpolation(hip):
+    rng = np.random.default_rng(5)
+    dims = (4, 8, 8, 8, 32 ** 3)
+    with hip.Net(dims, max_batch=1, seed=3) as net:
+        x = rng.uniform(-1, 1, 4).astype(np.float32)
+        y = net.forward(x[None])[0].reshape(32, 32, 32).astype(np.float64)
+        # box of the context equal to the network's box: the identity
+        L = 12.0
+        with hip.Context((32, 32, 32), [L / 32] * 3, [2 * np.pi / L] * 3, 1) as ctx:
+            net.warm_start(x, (L / 2,) * 3, ctx, 0)
+            psi = ctx.download(0)
+        assert np.array_equal(psi.real, y) and not np.any(psi.imag)
+        # a finer grid of a larger box: trilinear weights, zero outside
+        n, Lb, half = 64, 16.0, (5.0, 6.0, 4.0)
+        with hip.Context((n, n, n), [Lb / n] * 3, [2 * np.pi / Lb] * 3, 1) as ctx:
+            net.warm_start(x, half, ctx, 0)
+            psi = ctx.download(0).real
+    xs = (np.arange(n) - n // 2) * (Lb / n)
+    u = [xs * 16 / h + 16 for h in half]
+    j0 = [np.floor(v).astype(int) for v in u]
+    f = [v - q for v, q in zip(u, j0)]
+    ok = [(q >= 0) & (q <= 31) for q in j0]
+    ypad = np.zeros((33, 33, 33))
+    ypad[:32, :32, :32] = y
+    ref = np.zeros((n, n, n))
+    for c in range(8):
+        b = [(c >> a) & 1 for a in range(3)]
+        ws = [f[a] if b[a] else 1 - f[a] for a in range(3)]
+        idx = [np.clip(j0[a] + b[a], 0, 32) for a in range(3)]
+        ref += (ws[0][:, None, None] * ws[1][None, :, None] * ws[2][None, None, :]
+                * ypad[np.ix_(idx[0], idx[1], idx[2])])
+    ref *= ok[0][:, None, None] & ok[1][None, :, None] & ok[2][None, None, :]
+    assert np.max(np.abs(psi - ref)) < 1e-12 * np.max(np.abs(y))
+
+
+def test_sob_run_converged_mu_matches_oracle(hip, orc):
+    # anisotropic trap, interacting: both minimisers run the same iterations
+    n, L = 32, 16.0
+    g = orc.make_grid((n, n, n), L=L)
+    xs = orc.coords_x(g)
+    w = (0.8, 1.1, 1.5)
+    V = 0.5 * sum((w[a] * xs[a]) ** 2 for a in range(3)) * np.ones((n, n, n))
+    guess = np.exp(-0.5 * sum(w[a] * xs[a] ** 2 for a in range(3))).astype(np.complex128) * np.ones((n, n, n))
+    gs, step = [50.0], 1e-2
+    with hip.Context.from_grid(g, 1) as ctx:
+        ctx.upload(0, guess)
+        ctx.set_g(gs)
+        ctx.set_potential(0, V)
+        ctx.set_alpha([0.5] * 3)
+        ctx.sob_setup([0.5], step)
+        ctx.sob_renormalize()
+        its, res = ctx.sob_run(1e-10, 100000)
+        mu = ctx.sob_mu()[0]
+        psi = ctx.download(0)
+    assert res <= 1e-10
+    S = orc.SobSolver(g, [guess], [V], gs, [0.5], stepsize=step)
+    S.renormalize()
+    for _ in range(its):
+        S.iterate()
+    assert abs(mu - S.mu()[0]) <= 1e-9, (mu, S.mu()[0])
+    assert np.linalg.norm(psi - S.psi[0]) / np.linalg.norm(S.psi[0]) < 1e-10
