@@ -35,6 +35,7 @@ struct atus_net {
   int ndata = 0;
   int* idx = nullptr;
   long long step = 0;
+  bool split = true;  // matrix products as hi/lo bf16 pairs (atus_net_set_precision)
 };
 
 namespace {
@@ -57,7 +58,7 @@ int forward(atus_net* n, int B) {
   for (int l = 0; l < 4; ++l) {
     const int in = n->dims[l], out = n->dims[l + 1];
     HIP_TRY(mlp_gemm(n->x[l], in, 1, n->p + n->off_w[l], out, 1, n->x[l + 1], B, out, in, n->p + n->off_b[l],
-                     l < 3 ? 1 : 0, n->work, n->work_floats, n->stream));
+                     l < 3 ? 1 : 0, n->work, n->work_floats, n->stream, n->split));
   }
   return ATUS_OK;
 }
@@ -72,11 +73,11 @@ int backward(atus_net* n, int B, double* loss) {
     if (l < 3) HIP_TRY(mlp_tanh_back(dz, n->x[l + 1], dz, (long long)B * out, n->stream));
     // dW = x^T dZ, db = column sums of dZ, dX = dZ W^T
     HIP_TRY(mlp_gemm(n->x[l], 1, in, dz, out, 1, n->g + n->off_w[l], in, out, B, nullptr, 0, n->work,
-                     n->work_floats, n->stream));
+                     n->work_floats, n->stream, n->split));
     HIP_TRY(mlp_colsum(dz, B, out, n->g + n->off_b[l], n->stream));
     if (l > 0)
       HIP_TRY(mlp_gemm(dz, out, 1, n->p + n->off_w[l], 1, out, n->d[l], B, in, out, nullptr, 0, n->work,
-                       n->work_floats, n->stream));
+                       n->work_floats, n->stream, n->split));
   }
   HIP_TRY(hipMemcpyAsync(n->part_host.data(), n->part, sizeof(float) * n->nblk, hipMemcpyDeviceToHost, n->stream));
   HIP_TRY(hipStreamSynchronize(n->stream));
@@ -176,6 +177,13 @@ int atus_net_destroy(atus_net* n) {
   (void)hipFree(n->idx);
   if (n->stream) (void)hipStreamDestroy(n->stream);
   delete n;
+  return ATUS_OK;
+}
+
+int atus_net_set_precision(atus_net* n, int bf16_terms) {
+  ATUS_TRY(check_net(n));
+  if (bf16_terms != 1 && bf16_terms != 3) return fail(ATUS_ERR_ARG, "bf16_terms must be 1 or 3");
+  n->split = bf16_terms == 3;
   return ATUS_OK;
 }
 
@@ -293,13 +301,55 @@ int atus_net_warm_start(atus_net* n, const float* x, const double half_width[3],
   if (comp < 0 || comp >= c->ncomp) return fail(ATUS_ERR_ARG, "component out of range");
   for (int a = 0; a < 3; ++a)
     if (!(half_width[a] > 0.0)) return fail(ATUS_ERR_ARG, "half widths must be positive");
+  // per axis: the trigonometric interpolant of the 32 periodic samples at
+  // x_j = (j - 16) h, h = s/16, evaluated at the context's grid points;
+  // zero outside [-s, s).  Weight of sample j at t = (x - x_j)/h:
+  // sin(pi t) / (32 tan(pi t / 32)), 1 at t = 0.
+  std::vector<double> E[3];
+  for (int a = 0; a < 3; ++a) {
+    const int na = c->n[a];
+    const double s = half_width[a], h = s / 16.0;
+    E[a].assign((size_t)na * 32, 0.0);
+    for (int i = 0; i < na; ++i) {
+      const double xi = (double)(i - na / 2) * c->dx[a];
+      if (xi < -s || xi >= s) continue;
+      for (int j = 0; j < 32; ++j) {
+        const double t = (xi - (double)(j - 16) * h) / h;
+        const double rt = std::nearbyint(t);
+        double w;
+        if (std::fabs(t - rt) < 1e-12)
+          w = ((long long)rt % 32 == 0) ? 1.0 : 0.0;
+        else
+          w = std::sin(M_PI * t) / (32.0 * std::tan(M_PI * t / 32.0));
+        E[a][(size_t)i * 32 + j] = w;
+      }
+    }
+  }
   HIP_TRY(hipMemcpyAsync(n->x[0], x, sizeof(float) * n->dims[0], hipMemcpyHostToDevice, n->stream));
   ATUS_TRY(forward(n, 1));
   HIP_TRY(hipStreamSynchronize(n->stream));
-  ATUS_TRY(hipSetDevice(c->device) == hipSuccess ? ATUS_OK : fail(ATUS_ERR_HIP, "hipSetDevice failed"));
-  HIP_TRY(mlp_upsample32(n->x[4], c->psi + (long long)comp * c->npts, c->grid(), half_width, c->stream));
-  HIP_TRY(hipStreamSynchronize(c->stream));
-  return ATUS_OK;
+  if (hipSetDevice(c->device) != hipSuccess) return fail(ATUS_ERR_HIP, "hipSetDevice failed");
+  const size_t ne = (size_t)32 * (c->n[0] + c->n[1] + c->n[2]);
+  const size_t nt = (size_t)c->n[0] * 32 * 32 + (size_t)c->n[0] * c->n[1] * 32;
+  double* buf = nullptr;
+  if (hipMalloc(&buf, sizeof(double) * (ne + nt)) != hipSuccess) return fail(ATUS_ERR_NOMEM, "interpolation buffers");
+  double* e0 = buf;
+  double* e1 = e0 + (size_t)32 * c->n[0];
+  double* e2 = e1 + (size_t)32 * c->n[1];
+  double* t1 = e2 + (size_t)32 * c->n[2];
+  double* t2 = t1 + (size_t)c->n[0] * 32 * 32;
+  int rc = ATUS_OK;
+  if (hipMemcpyAsync(e0, E[0].data(), sizeof(double) * E[0].size(), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+      hipMemcpyAsync(e1, E[1].data(), sizeof(double) * E[1].size(), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+      hipMemcpyAsync(e2, E[2].data(), sizeof(double) * E[2].size(), hipMemcpyHostToDevice, c->stream) != hipSuccess)
+    rc = fail(ATUS_ERR_HIP, "interpolation weights");
+  if (rc == ATUS_OK) {
+    hipError_t e = mlp_interp32(n->x[4], e0, e1, e2, t1, t2, c->psi + (long long)comp * c->npts, c->n, c->stream);
+    if (e != hipSuccess) rc = fail(ATUS_ERR_HIP, std::string("interpolation: ") + hipGetErrorString(e));
+  }
+  if (hipStreamSynchronize(c->stream) != hipSuccess && rc == ATUS_OK) rc = fail(ATUS_ERR_HIP, "synchronize");
+  (void)hipFree(buf);
+  return rc;
 }
 
 }  // extern "C"

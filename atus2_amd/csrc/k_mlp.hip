@@ -1,7 +1,7 @@
 // k_mlp.hip -- kernels of the warm-start network: BF16 MFMA matrix
 // products with FP32 accumulation (v_mfma_f32_16x16x32_bf16), their
-// epilogues, the loss, the optimiser, and the map of the network's 32^3
-// output onto a solver grid.
+// epilogues, the loss, the optimiser, and the separable interpolation of
+// the network's 32^3 output onto a solver grid.
 #include "mlp.hpp"
 
 namespace atus {
@@ -10,10 +10,14 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // P[z][m][n] = sum over k in [z*kchunk, (z+1)*kchunk) of A(m,k) B(k,n), with
-// A(m,k) = A[m*sam + k*sak] and B(k,n) = B[k*sbk + n*sbn] rounded to bf16.
+// A(m,k) = A[m*sam + k*sak] and B(k,n) = B[k*sbk + n*sbn] rounded to bf16
+// (SPLIT false), or split as a = hi + lo, hi = bf16(a), lo = bf16(a - hi),
+// with a b ~ hi_a hi_b + hi_a lo_b + lo_a hi_b (SPLIT true: three products
+// per slice, about 16 significant bits per operand).
 // One wavefront per 16x16 tile; lane l holds A rows l%16 and B columns
 // l%16 at k = 8*(l/16) + 0..7 of each 32-wide slice, and receives rows
 // 4*(l/16) + 0..3 of column l%16 of the tile.
+template <bool SPLIT>
 __global__ __launch_bounds__(64) void k_gemm_bf16(const float* __restrict__ A, long long sam, long long sak,
                                                   const float* __restrict__ B, long long sbk, long long sbn,
                                                   float* __restrict__ P, int M, int N, int K, int kchunk) {
@@ -26,13 +30,23 @@ __global__ __launch_bounds__(64) void k_gemm_bf16(const float* __restrict__ A, l
   const bool mok = m < M, nok = n < N;
   f32x4 acc = {0.f, 0.f, 0.f, 0.f};
   for (int k0 = kb; k0 < ke; k0 += 32) {
-    bf16x8 a, b;
+    bf16x8 a, b, al, bl;
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       const int k = k0 + 8 * grp + j;
       const bool kin = k < ke;
-      a[j] = (__bf16)((kin && mok) ? A[(long long)m * sam + (long long)k * sak] : 0.f);
-      b[j] = (__bf16)((kin && nok) ? B[(long long)k * sbk + (long long)n * sbn] : 0.f);
+      const float fa = (kin && mok) ? A[(long long)m * sam + (long long)k * sak] : 0.f;
+      const float fb = (kin && nok) ? B[(long long)k * sbk + (long long)n * sbn] : 0.f;
+      a[j] = (__bf16)fa;
+      b[j] = (__bf16)fb;
+      if constexpr (SPLIT) {
+        al[j] = (__bf16)(fa - (float)a[j]);
+        bl[j] = (__bf16)(fb - (float)b[j]);
+      }
+    }
+    if constexpr (SPLIT) {
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, b, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bl, acc, 0, 0, 0);
     }
     acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
   }
@@ -115,52 +129,27 @@ __global__ __launch_bounds__(256) void k_gather(const float* __restrict__ src, c
     dst[e] = src[(long long)idx[e / w] * w + e % w];
 }
 
-// psi(x) = trilinear interpolation of the 32^3 values y at grid
-// coordinates u_a = x_a * 16 / s_a + 16 (the 32^3 grid spans [-s_a, s_a)
-// with x_j = (j - 16) s_a / 16), 0 outside it; imaginary part 0
-__global__ __launch_bounds__(256) void k_upsample32(const float* __restrict__ y, double2* __restrict__ psi, Grid g,
-                                                    double s0, double s1, double s2) {
-  const double s[3] = {s0, s1, s2};
-  for (long long l = (long long)blockIdx.x * blockDim.x + threadIdx.x; l < g.npts;
-       l += (long long)gridDim.x * blockDim.x) {
-    long long rem = l;
-    int idx[3] = {0, 0, 0};
-    for (int a = g.dim - 1; a >= 0; --a) {
-      idx[a] = (int)(rem % g.n[a]);
-      rem /= g.n[a];
-    }
-    int j0[3];
-    double f[3];
-    bool inside = true;
-    for (int a = 0; a < 3; ++a) {
-      if (a >= g.dim) {
-        j0[a] = 16;
-        f[a] = 0.0;
-        continue;
-      }
-      const double x = (double)(idx[a] - g.n[a] / 2) * g.dx[a];
-      const double u = x * 16.0 / s[a] + 16.0;
-      const double fl = floor(u);
-      j0[a] = (int)fl;
-      f[a] = u - fl;
-      if (j0[a] < 0 || j0[a] > 31) inside = false;
-    }
-    double val = 0.0;
-    if (inside) {
-      for (int c = 0; c < 8; ++c) {
-        double w = 1.0;
-        int jj[3];
-        for (int a = 0; a < 3; ++a) {
-          const int bit = (c >> a) & 1;
-          jj[a] = j0[a] + bit;
-          w *= bit ? f[a] : 1.0 - f[a];
-        }
-        if (w == 0.0) continue;
-        if (jj[0] > 31 || jj[1] > 31 || jj[2] > 31) continue;
-        val += w * (double)y[(jj[0] * 32 + jj[1]) * 32 + jj[2]];
-      }
-    }
-    psi[l] = make_double2(val, 0.0);
+// one axis of a separable interpolation: out[p][i][q] = sum_j E[i][j] in[p][j][q]
+// (p < outer, i < nout, j < nin, q < inner); `in` is fp32 (the network's
+// output) or fp64, out is fp64 or, for the last axis, the complex field
+template <typename Tin, bool TO_FIELD>
+__global__ __launch_bounds__(256) void k_interp_axis(const Tin* __restrict__ in, const double* __restrict__ E,
+                                                     void* __restrict__ out, long long outer, int nin, int nout,
+                                                     long long inner) {
+  const long long tot = outer * nout * inner;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (long long)gridDim.x * blockDim.x) {
+    const long long q = e % inner;
+    const long long r = e / inner;
+    const int i = (int)(r % nout);
+    const long long p = r / nout;
+    const double* Ei = E + (long long)i * nin;
+    const Tin* src = in + p * nin * inner + q;
+    double acc = 0.0;
+    for (int j = 0; j < nin; ++j) acc += Ei[j] * (double)src[(long long)j * inner];
+    if constexpr (TO_FIELD)
+      static_cast<double2*>(out)[e] = make_double2(acc, 0.0);
+    else
+      static_cast<double*>(out)[e] = acc;
   }
 }
 
@@ -195,13 +184,16 @@ size_t mlp_gemm_work(int M, int N, int K) {
 
 hipError_t mlp_gemm(const float* A, long long sam, long long sak, const float* B, long long sbk, long long sbn,
                     float* C, int M, int N, int K, const float* bias, int act, float* work, size_t work_floats,
-                    hipStream_t s) {
+                    hipStream_t s, bool split) {
   if (M < 1 || N < 1 || K < 1) return hipErrorInvalidValue;
   int splits, kchunk;
   split_plan(M, N, K, &splits, &kchunk);
   if ((size_t)splits * M * N > work_floats) return hipErrorInvalidValue;
   dim3 grid((unsigned)((N + 15) / 16), (unsigned)((M + 15) / 16), (unsigned)splits);
-  hipLaunchKernelGGL(k_gemm_bf16, grid, dim3(64), 0, s, A, sam, sak, B, sbk, sbn, work, M, N, K, kchunk);
+  if (split)
+    hipLaunchKernelGGL(k_gemm_bf16<true>, grid, dim3(64), 0, s, A, sam, sak, B, sbk, sbn, work, M, N, K, kchunk);
+  else
+    hipLaunchKernelGGL(k_gemm_bf16<false>, grid, dim3(64), 0, s, A, sam, sak, B, sbk, sbn, work, M, N, K, kchunk);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_gemm_sum, dim3(ew_grid((long long)M * N)), dim3(256), 0, s, work, splits, M, N, bias, act, C);
@@ -234,8 +226,19 @@ hipError_t mlp_gather(const float* src, const int* idx, int rows, long long w, f
   return hipGetLastError();
 }
 
-hipError_t mlp_upsample32(const float* y, double2* psi, const Grid& g, const double sc[3], hipStream_t s) {
-  hipLaunchKernelGGL(k_upsample32, dim3(ew_grid(g.npts)), dim3(256), 0, s, y, psi, g, sc[0], sc[1], sc[2]);
+hipError_t mlp_interp32(const float* y, const double* E0, const double* E1, const double* E2, double* t1, double* t2,
+                        double2* psi, const int n[3], hipStream_t s) {
+  // [32][32][32] -> [n0][32][32] -> [n0][n1][32] -> [n0][n1][n2]
+  hipLaunchKernelGGL((k_interp_axis<float, false>), dim3(ew_grid((long long)n[0] * 32 * 32)), dim3(256), 0, s, y, E0,
+                     (void*)t1, 1LL, 32, n[0], 32LL * 32);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_interp_axis<double, false>), dim3(ew_grid((long long)n[0] * n[1] * 32)), dim3(256), 0, s, t1,
+                     E1, (void*)t2, (long long)n[0], 32, n[1], 32LL);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_interp_axis<double, true>), dim3(ew_grid((long long)n[0] * n[1] * n[2])), dim3(256), 0, s, t2,
+                     E2, (void*)psi, (long long)n[0] * n[1], 32, n[2], 1LL);
   return hipGetLastError();
 }
 

@@ -25,7 +25,7 @@ EXPORTED = [
     "atus_sob_mu", "atus_set_timing", "atus_last_block_ms", "atus_run_block_profiled", "atus_copy_bandwidth", "atus_tune",
     "atus_batch_create", "atus_batch_set_runs", "atus_batch_run_block", "atus_batch_particle_numbers",
     "atus_batch_rabi_integrals", "atus_sob_run",
-    "atus_net_create", "atus_net_destroy", "atus_net_param_count", "atus_net_get_params", "atus_net_set_params",
+    "atus_net_create", "atus_net_destroy", "atus_net_set_precision", "atus_net_param_count", "atus_net_get_params", "atus_net_set_params",
     "atus_net_forward", "atus_net_gradients", "atus_net_set_data", "atus_net_train", "atus_net_warm_start",
 ]
 
@@ -110,6 +110,7 @@ def load():
         "atus_sob_run": (I, [P, D, C.c_longlong, C.POINTER(C.c_longlong), C.POINTER(D)]),
         "atus_net_create": (I, [pctx, I, I, I, I, I, I, C.c_ulonglong]),
         "atus_net_destroy": (I, [P]),
+        "atus_net_set_precision": (I, [P, I]),
         "atus_net_param_count": (I, [P, C.POINTER(C.c_longlong)]),
         "atus_net_get_params": (I, [P, P]),
         "atus_net_set_params": (I, [P, P]),
@@ -460,6 +461,10 @@ class Net:
 
     def __exit__(self, *a):
         self.close()
+
+    def set_precision(self, bf16_terms):
+        """1: bf16 operands; 3: hi/lo bf16 pairs (three products)."""
+        _check(_lib.atus_net_set_precision(self._h, int(bf16_terms)))
 
     def get_params(self):
         out = np.empty(self.nparams, np.float32)

@@ -8,11 +8,12 @@ CSOB_Base conventions with Alpha = 1/2) on 128^3 points of [-16, 16)^3.
   32^3 grid whose box [-s_a, s_a) is fitted to the cloud (half widths from
   the Thomas-Fermi radius and the oscillator length), from the cold start.
   The targets are those 32^3 states divided by their peak.
-* Network: (log w, log g) scaled to about [-1, 1] -> 128 -> 128 -> 32 ->
-  32^3 (tanh hidden layers, linear output; 1.1M parameters), trained with
-  the library's BF16 MFMA kernels (atus_net_*).
-* Warm start: the network's 32^3 output, read on the fitted box and
-  interpolated trilinearly onto the 128^3 grid (atus_net_warm_start); the
+* Network: (log w, log g) scaled to about [-1, 1] -> 256 -> 256 -> 64 ->
+  32^3 (tanh hidden layers, linear output; 2.2M parameters), trained with
+  the library's BF16 MFMA kernels (atus_net_*), 60000 Adam steps.
+* Warm start: the network's 32^3 output, read as periodic samples on the
+  fitted box and carried onto the 128^3 grid by the trigonometric
+  interpolant of each axis (atus_net_warm_start); the
   minimiser then polishes it to the same residual as a cold start.  The
   cold start is the Gaussian or the Thomas-Fermi profile, whichever has
   the lower energy.
@@ -37,7 +38,7 @@ from . import hip
 
 N_GRID, L_BOX = 128, 32.0
 N_SMALL = 32
-DIMS = (4, 128, 128, 32, N_SMALL ** 3)
+DIMS = (4, 256, 256, 64, N_SMALL ** 3)
 ALPHA = 0.5
 STEP_128 = 1e-3        # stable for every case of the range on [-16, 16)^3
 EPS = 1e-10
@@ -179,7 +180,7 @@ def main(argv=None):
     sub = ap.add_subparsers(dest="cmd", required=True)
     tr = sub.add_parser("train")
     tr.add_argument("--cases", type=int, default=4096)
-    tr.add_argument("--steps", type=int, default=6000)
+    tr.add_argument("--steps", type=int, default=60000)
     tr.add_argument("--batch", type=int, default=256)
     tr.add_argument("--lr", type=float, default=2e-3)
     tr.add_argument("--out", default=WEIGHTS)

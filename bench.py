@@ -115,6 +115,28 @@ def measured_traffic(n, family, values_per_thread):
     return tab["families"].get(family, {}).get(str(values_per_thread))
 
 
+def ground_state_run(ncases):
+    """Sobolev ground states of held-out traps at 128^3 (res <= 1e-10):
+    GPU wall time from the cold start and from the learned warm start."""
+    from atus2_amd import learned as Lr
+    net = Lr.load_net()
+    rows = []
+    try:
+        for c in Lr.cases(ncases, 55):
+            ic, tc, _, _ = Lr.solve_large(c, "cold")
+            iw, tw, _, _ = Lr.solve_large(c, "warm", net)
+            rows.append((ic, tc, iw, tw))
+            log(f"[bench] ground state w={tuple(round(v, 2) for v in c.w)} g={c.g:.0f}: "
+                f"cold {ic} its {tc:.2f} s, warm {iw} its {tw:.2f} s")
+    finally:
+        net.close()
+    cold = sum(r[1] for r in rows)
+    warm = sum(r[3] for r in rows)
+    return {"grid": [128, 128, 128], "eps": 1e-10, "cases": ncases, "cold_s": cold, "warm_s": warm,
+            "warm_over_cold": warm / cold, "cold_iters": [r[0] for r in rows], "warm_iters": [r[2] for r in rows],
+            "sample": "held-out (w, g) of seed 55; warm time includes the network and the interpolation"}
+
+
 def cpu_baseline(wl, steps):
     """The oracle on the same 256^3 Bragg workload, `steps` steps."""
     from oracle import oracle as O
@@ -143,6 +165,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tuning", default="", help="fixed values per thread, e.g. pass_strided=8,vfuse=8")
+    ap.add_argument("--gs-cases", type=int, default=4, help="ground states timed cold and warm (0: none)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -154,6 +177,7 @@ def main():
     npts = float(math.prod(wl.shape))
     value = npts * args.steps / r["elapsed"] / 1e9
     traffic = measured_traffic(wl.n, r["dominant"], r["tuning"].get(r["dominant"], 16))
+    gs = ground_state_run(args.gs_cases) if args.gs_cases > 0 else None
     cpu = None
     if not args.no_cpu_baseline:
         log(f"[bench] CPU baseline: oracle, {args.cpu_steps} steps")
@@ -181,6 +205,7 @@ def main():
                      "algorithmic_bytes_per_launch": r["bytes_per_launch"],
                      "measured_copy_GBps": r["copy_gbps"]},
         "cpu_baseline": cpu,
+        "ground_state": gs,
         "kernels": r["kernels"],
         "tuning": r["tuning"],
         "checks": {"norm_drift_per_step": r["norm_drift_per_step"], "rabi_last_block": r["rabi"]},

@@ -212,13 +212,17 @@ int atus_batch_rabi_integrals(atus_ctx *ctx, int comp, const double *k0, int n, 
  * atus_sob_run then polishes to the same residual).  A perceptron
  * n_in -> h1 -> h2 -> h3 -> n_out, tanh hidden layers, linear output, fp32
  * parameters laid out as [W1 (n_in x h1, row-major), b1, W2, b2, W3, b3,
- * W4, b4]; matrix products round their operands to bf16 and accumulate in
- * fp32 on the matrix cores; training minimises the mean squared error with
+ * W4, b4]; matrix products take bf16 operands (atus_net_set_precision) and
+ * accumulate in fp32 on the matrix cores; training minimises the mean squared error with
  * Adam (beta1 0.9, beta2 0.999, eps 1e-8). */
 typedef struct atus_net atus_net;
 int atus_net_create(atus_net **out, int n_in, int h1, int h2, int h3, int n_out, int max_batch,
                     unsigned long long seed);
 int atus_net_destroy(atus_net *net);
+/* operands of every matrix product: 1 = one bf16 value each; 3 (the
+ * default) = a bf16 pair hi + lo each and the three products hi*hi +
+ * hi*lo + lo*hi, about 16 significant bits, still on the bf16 matrix cores */
+int atus_net_set_precision(atus_net *net, int bf16_terms);
 int atus_net_param_count(atus_net *net, long long *count);
 int atus_net_get_params(atus_net *net, float *host);
 /* also resets the optimiser state */
@@ -234,8 +238,9 @@ int atus_net_set_data(atus_net *net, const float *x, const float *t, int count);
 int atus_net_train(atus_net *net, int steps, int batch, float lr, float lr_final, unsigned long long seed,
                    float *losses);
 /* component comp of a 3D context <- the network's 32^3 output for input x,
- * read as values at x_j = (j - 16) half_width[a] / 16 per axis and
- * interpolated trilinearly onto the context's grid (0 outside) */
+ * read as periodic samples at x_j = (j - 16) half_width[a] / 16 per axis
+ * and interpolated onto the context's grid with the trigonometric
+ * interpolant of each axis (0 outside [-half_width, half_width)) */
 int atus_net_warm_start(atus_net *net, const float *x, const double half_width[3], atus_ctx *ctx, int comp);
 
 #ifdef __cplusplus

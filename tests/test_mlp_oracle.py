@@ -22,7 +22,7 @@ def test_gradients_match_central_differences():
     p = (rng.standard_normal(n) * 0.5).astype(np.float32)
     x = rng.standard_normal((5, 4))
     t = rng.standard_normal((5, 7))
-    loss, g = M.loss_and_gradients(p, dims, x, t, bf16=False)
+    loss, g = M.loss_and_gradients(p, dims, x, t, terms=None)
     h = 1e-3
     num = np.zeros(n)
     pp = p.astype(np.float64)
@@ -43,3 +43,13 @@ def _loss64(p64, dims, x, t):
         z = a @ p64[wo:wo + ws[0] * ws[1]].reshape(ws) + p64[bo:bo + bn]
         a = np.tanh(z) if l < 3 else z
     return float(np.sum((a - t) ** 2) / a.size)
+
+
+def test_split_products_are_closer_to_float64_than_bf16():
+    rng = np.random.default_rng(1)
+    a = rng.standard_normal((16, 64)).astype(np.float32)
+    b = rng.standard_normal((64, 8)).astype(np.float32)
+    exact = a.astype(np.float64) @ b.astype(np.float64)
+    e1 = np.abs(M.matmul(a, b, 1) - exact).max()
+    e3 = np.abs(M.matmul(a, b, 3) - exact).max()
+    assert e3 < e1 / 100
