@@ -624,23 +624,28 @@ int atus_set_momentum(atus_ctx* c, int comp, const double p[3]) {
   return ATUS_OK;
 }
 
-int atus_run_block(atus_ctx* c, const atus_step_params* sp, int Nk, double* t) {
-  ATUS_TRY(check_ctx(c));
-  ATUS_TRY(check_single(c, "atus_run_block"));
-  if (!sp || !t) return fail(ATUS_ERR_ARG, "null argument");
-  if (Nk < 1) return fail(ATUS_ERR_ARG, "Nk must be >= 1");
-  const int vk = vkind_of(sp->kind);
-  if (vk < 0) return fail(ATUS_ERR_ARG, "unknown step kind " + std::to_string(sp->kind));
-  if (vk == V_BRAGG && c->ncomp != 2) return fail(ATUS_ERR_STATE, "bragg_ad needs exactly two components");
-  const double dt = sp->dt;
-  ATUS_TRY(ensure_kin(c, dt));
-  VArgs base;
-  fill_vargs(c, base, dt, &sp->laser);
+}  // extern "C"
+
+namespace {
+
+// per-step time-dependent arguments of the V steps of one block: the
+// amplitude callback (single runs), the device amplitude table of a batch
+// ([Nk][nbatch]), and for replayed graphs device tables of t1 ([Nk]) and of
+// the single-run amplitude ([Nk])
+struct StepFeed {
+  const atus_step_params* sp = nullptr;
+  const double* fbatch = nullptr;
+  const double* t1dev = nullptr;
+  const double* fdev = nullptr;
+};
+
+// the launches of one block of a single run on the context's stream (half
+// kinetic, (V, full) x (Nk-1), V, half), without synchronisation; *t
+// advances with the reference's bookkeeping
+int single_block(atus_ctx* c, int vk, const VArgs& base, int Nk, double dt, double* t, const StepFeed& fd) {
   const bool fused = c->ncomp <= 2;
   const int N0 = c->n[0], NL = c->n[c->dim - 1];
   const long long inner0 = c->inner_of(0);
-  if (c->timing) HIP_TRY(hipEventRecord(c->ev0, c->stream));
-
   double tt = *t;
   // half kinetic step
   if (fused && c->fs) {
@@ -662,8 +667,13 @@ int atus_run_block(atus_ctx* c, const atus_step_params* sp, int Nk, double* t) {
   for (int m = 1; m <= Nk; ++m) {
     VArgs va = base;
     va.t1 = tt + 0.5 * dt;
-    va.F = sp->amp ? sp->amp(va.t1, sp->amp_user) : 1.0;
-    if (!std::isfinite(va.F)) return fail(ATUS_ERR_NUMERIC, "amplitude callback returned a non-finite value");
+    if (fd.t1dev) {
+      va.t1p = fd.t1dev + (m - 1);
+      va.Fp = fd.fdev + (m - 1);
+    } else {
+      va.F = (fd.sp && fd.sp->amp) ? fd.sp->amp(va.t1, fd.sp->amp_user) : 1.0;
+      if (!std::isfinite(va.F)) return fail(ATUS_ERR_NUMERIC, "amplitude callback returned a non-finite value");
+    }
     const int f = (m < Nk) ? 1 : 0;
     if (!fused) {
       {
@@ -701,6 +711,81 @@ int atus_run_block(atus_ctx* c, const atus_step_params* sp, int Nk, double* t) {
   } else if (fused && c->dim >= 2) {
     ATUS_TRY(pass(c, c->dim - 1, +1, c->psi, c->psi, c->ncomp, 1.0));
   }
+  *t = tt;
+  return ATUS_OK;
+}
+
+void batch_vargs(atus_ctx* c, VArgs& base, double dt) {
+  fill_vargs(c, base, dt, nullptr);
+  base.runs = c->runs;
+  base.run_shift = (long long)(c->nc_run - 1) * c->npts;
+  base.pencils_per_run = c->dim == 1 ? 1 : c->npts / c->n[c->dim - 1];
+}
+
+// the launches of one block of every run of a batch, without synchronisation
+int batch_block(atus_ctx* c, int vk, const VArgs& base, int Nk, double dt, double* t, const StepFeed& fd) {
+  const int B = c->nbatch;
+  const int N0 = c->n[0], NL = c->n[c->dim - 1];
+  const long long inner0 = c->inner_of(0);
+  double tt = *t;
+  if (c->fs) {
+    ATUS_TRY(fs_col(c, FS_ENTER, c->psi, c->ncomp, c->npts, V_NONE, 1, false, base, 1.0));
+    ATUS_TRY(fs_row_kin(c, 0, base));
+  } else if (c->dim >= 2) {
+    ATUS_TRY(pass(c, c->dim - 1, -1, c->psi, c->psi, c->ncomp, 1.0));
+    if (c->dim == 3) ATUS_TRY(pass(c, 1, -1, c->psi, c->psi, c->ncomp, 1.0));
+    {
+      ProfScope ps(c, ATUS_KFAM_KIN);
+      HIP_TRY(launch_kin(N0, c->rm[ATUS_KFAM_KIN], false, V_NONE, 1, c->psi, c->npts, c->ncomp, inner0, c->tw[N0],
+                         kin_tabs(c, 0), base, c->dx[0], c->max_lds, c->stream));
+    }
+    if (c->dim == 3) ATUS_TRY(pass(c, 1, +1, c->psi, c->psi, c->ncomp, 1.0));
+  } else {
+    ATUS_TRY(kinetic_unfused(c, 0));
+  }
+  tt += 0.5 * dt;
+  for (int m = 1; m <= Nk; ++m) {
+    VArgs va = base;
+    va.t1 = tt + 0.5 * dt;
+    va.Fm = fd.fbatch + (size_t)(m - 1) * B;
+    if (fd.t1dev) va.t1p = fd.t1dev + (m - 1);
+    const int f = (m < Nk) ? 1 : 0;
+    if (c->fs) {
+      ATUS_TRY(fs_col(c, FS_STEP, c->psi, B, c->npts, vk, 2, true, va, 1.0));
+      ATUS_TRY(fs_row_kin(c, f, va));
+    } else if (c->dim == 1) {
+      ProfScope ps(c, ATUS_KFAM_KIN);
+      HIP_TRY(launch_vstep_batch(N0, c->rm[ATUS_KFAM_KIN], true, vk, c->psi, c->npts, B, c->tw[N0], kin_tabs(c, f), va,
+                                 1, N0, c->dx[0], c->max_lds, c->stream));
+    } else {
+      const long long ppx = c->dim == 3 ? c->n[1] : 1;
+      {
+        ProfScope ps(c, ATUS_KFAM_VFUSE);
+        HIP_TRY(launch_vstep_batch(NL, c->rm[ATUS_KFAM_VFUSE], false, vk, c->psi, c->npts,
+                                   (long long)B * base.pencils_per_run, c->tw[NL], kin_tabs(c, f), va, ppx, c->n[0],
+                                   c->dx[0], c->max_lds, c->stream));
+      }
+      if (c->dim == 3) ATUS_TRY(pass(c, 1, -1, c->psi, c->psi, c->ncomp, 1.0));
+      {
+        ProfScope ps(c, ATUS_KFAM_KIN);
+        HIP_TRY(launch_kin(N0, c->rm[ATUS_KFAM_KIN], false, V_NONE, 1, c->psi, c->npts, c->ncomp, inner0, c->tw[N0],
+                           kin_tabs(c, f), va, c->dx[0], c->max_lds, c->stream));
+      }
+      if (c->dim == 3) ATUS_TRY(pass(c, 1, +1, c->psi, c->psi, c->ncomp, 1.0));
+    }
+    tt += (m < Nk) ? dt : 0.5 * dt;
+  }
+  if (c->fs) {
+    ATUS_TRY(fs_col(c, FS_EXIT, c->psi, c->ncomp, c->npts, V_NONE, 1, false, base, 1.0));
+  } else if (c->dim >= 2) {
+    ATUS_TRY(pass(c, c->dim - 1, +1, c->psi, c->psi, c->ncomp, 1.0));
+  }
+  *t = tt;
+  return ATUS_OK;
+}
+
+// end of a block run eagerly: event timing or a plain synchronisation
+int finish_block(atus_ctx* c) {
   if (c->timing) {
     HIP_TRY(hipEventRecord(c->ev1, c->stream));
     HIP_TRY(hipEventSynchronize(c->ev1));
@@ -710,8 +795,29 @@ int atus_run_block(atus_ctx* c, const atus_step_params* sp, int Nk, double* t) {
   } else {
     HIP_TRY(hipStreamSynchronize(c->stream));
   }
-  *t = tt;
   return ATUS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int atus_run_block(atus_ctx* c, const atus_step_params* sp, int Nk, double* t) {
+  ATUS_TRY(check_ctx(c));
+  ATUS_TRY(check_single(c, "atus_run_block"));
+  if (!sp || !t) return fail(ATUS_ERR_ARG, "null argument");
+  if (Nk < 1) return fail(ATUS_ERR_ARG, "Nk must be >= 1");
+  const int vk = vkind_of(sp->kind);
+  if (vk < 0) return fail(ATUS_ERR_ARG, "unknown step kind " + std::to_string(sp->kind));
+  if (vk == V_BRAGG && c->ncomp != 2) return fail(ATUS_ERR_STATE, "bragg_ad needs exactly two components");
+  ATUS_TRY(ensure_kin(c, sp->dt));
+  VArgs base;
+  fill_vargs(c, base, sp->dt, &sp->laser);
+  if (c->timing) HIP_TRY(hipEventRecord(c->ev0, c->stream));
+  StepFeed fd;
+  fd.sp = sp;
+  ATUS_TRY(single_block(c, vk, base, Nk, sp->dt, t, fd));
+  return finish_block(c);
 }
 
 int atus_particle_number(atus_ctx* c, int comp, double* out) {
@@ -1105,77 +1211,12 @@ int atus_batch_run_block(atus_ctx* c, int kind, double dt, int Nk, double* t, co
   HIP_TRY(hipMemcpyAsync(c->ftab, F, sizeof(double) * nf, hipMemcpyHostToDevice, c->stream));
   ATUS_TRY(ensure_kin(c, dt));
   VArgs base;
-  fill_vargs(c, base, dt, nullptr);
-  base.runs = c->runs;
-  base.run_shift = (long long)(c->nc_run - 1) * c->npts;
-  const int N0 = c->n[0], NL = c->n[c->dim - 1];
-  const long long inner0 = c->inner_of(0);
-  base.pencils_per_run = c->dim == 1 ? 1 : c->npts / NL;
+  batch_vargs(c, base, dt);
   if (c->timing) HIP_TRY(hipEventRecord(c->ev0, c->stream));
-
-  double tt = *t;
-  if (c->fs) {
-    ATUS_TRY(fs_col(c, FS_ENTER, c->psi, c->ncomp, c->npts, V_NONE, 1, false, base, 1.0));
-    ATUS_TRY(fs_row_kin(c, 0, base));
-  } else if (c->dim >= 2) {
-    ATUS_TRY(pass(c, c->dim - 1, -1, c->psi, c->psi, c->ncomp, 1.0));
-    if (c->dim == 3) ATUS_TRY(pass(c, 1, -1, c->psi, c->psi, c->ncomp, 1.0));
-    {
-      ProfScope ps(c, ATUS_KFAM_KIN);
-      HIP_TRY(launch_kin(N0, c->rm[ATUS_KFAM_KIN], false, V_NONE, 1, c->psi, c->npts, c->ncomp, inner0, c->tw[N0],
-                         kin_tabs(c, 0), base, c->dx[0], c->max_lds, c->stream));
-    }
-    if (c->dim == 3) ATUS_TRY(pass(c, 1, +1, c->psi, c->psi, c->ncomp, 1.0));
-  } else {
-    ATUS_TRY(kinetic_unfused(c, 0));
-  }
-  tt += 0.5 * dt;
-  for (int m = 1; m <= Nk; ++m) {
-    VArgs va = base;
-    va.t1 = tt + 0.5 * dt;
-    va.Fm = c->ftab + (size_t)(m - 1) * B;
-    const int f = (m < Nk) ? 1 : 0;
-    if (c->fs) {
-      ATUS_TRY(fs_col(c, FS_STEP, c->psi, B, c->npts, vk, 2, true, va, 1.0));
-      ATUS_TRY(fs_row_kin(c, f, va));
-    } else if (c->dim == 1) {
-      ProfScope ps(c, ATUS_KFAM_KIN);
-      HIP_TRY(launch_vstep_batch(N0, c->rm[ATUS_KFAM_KIN], true, vk, c->psi, c->npts, B, c->tw[N0], kin_tabs(c, f), va,
-                                 1, N0, c->dx[0], c->max_lds, c->stream));
-    } else {
-      const long long ppx = c->dim == 3 ? c->n[1] : 1;
-      {
-        ProfScope ps(c, ATUS_KFAM_VFUSE);
-        HIP_TRY(launch_vstep_batch(NL, c->rm[ATUS_KFAM_VFUSE], false, vk, c->psi, c->npts,
-                                   (long long)B * base.pencils_per_run, c->tw[NL], kin_tabs(c, f), va, ppx, c->n[0],
-                                   c->dx[0], c->max_lds, c->stream));
-      }
-      if (c->dim == 3) ATUS_TRY(pass(c, 1, -1, c->psi, c->psi, c->ncomp, 1.0));
-      {
-        ProfScope ps(c, ATUS_KFAM_KIN);
-        HIP_TRY(launch_kin(N0, c->rm[ATUS_KFAM_KIN], false, V_NONE, 1, c->psi, c->npts, c->ncomp, inner0, c->tw[N0],
-                           kin_tabs(c, f), va, c->dx[0], c->max_lds, c->stream));
-      }
-      if (c->dim == 3) ATUS_TRY(pass(c, 1, +1, c->psi, c->psi, c->ncomp, 1.0));
-    }
-    tt += (m < Nk) ? dt : 0.5 * dt;
-  }
-  if (c->fs) {
-    ATUS_TRY(fs_col(c, FS_EXIT, c->psi, c->ncomp, c->npts, V_NONE, 1, false, base, 1.0));
-  } else if (c->dim >= 2) {
-    ATUS_TRY(pass(c, c->dim - 1, +1, c->psi, c->psi, c->ncomp, 1.0));
-  }
-  if (c->timing) {
-    HIP_TRY(hipEventRecord(c->ev1, c->stream));
-    HIP_TRY(hipEventSynchronize(c->ev1));
-    float ms = 0;
-    HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    c->last_ms = ms;
-  } else {
-    HIP_TRY(hipStreamSynchronize(c->stream));
-  }
-  *t = tt;
-  return ATUS_OK;
+  StepFeed fd;
+  fd.fbatch = c->ftab;
+  ATUS_TRY(batch_block(c, vk, base, Nk, dt, t, fd));
+  return finish_block(c);
 }
 
 int atus_batch_particle_numbers(atus_ctx* c, double* out) {
@@ -1212,6 +1253,120 @@ int atus_batch_rabi_integrals(atus_ctx* c, int comp, const double* k0, int n, do
   HIP_TRY(hipStreamSynchronize(c->stream));
   for (int r = 0; r < c->nbatch; ++r)
     for (int q = 0; q < n; ++q) out[(size_t)r * n + q] = c->ar_k() * s[(size_t)r * RED_MAX_NV + q];
+  return ATUS_OK;
+}
+
+
+// ---------------------------------------------------------------- graphs
+
+}  // extern "C"
+
+struct atus_graph {
+  atus_ctx* c = nullptr;
+  bool batch = false;
+  int vk = 0, Nk = 0;
+  double dt = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  double* dev = nullptr;   // t1 of each step [Nk], then the amplitudes [Nk][nbatch]
+  double* host = nullptr;  // pinned staging of the same
+  size_t nf = 0;           // amplitudes per block
+};
+
+extern "C" {
+
+int atus_graph_destroy(atus_graph* g) {
+  if (!g) return ATUS_OK;
+  if (g->c) {
+    (void)hipSetDevice(g->c->device);
+    (void)hipStreamSynchronize(g->c->stream);
+  }
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  (void)hipFree(g->dev);
+  if (g->host) (void)hipHostFree(g->host);
+  delete g;
+  return ATUS_OK;
+}
+
+int atus_graph_create(atus_ctx* c, int kind, double dt, const atus_laser* laser, int Nk, atus_graph** out) {
+  ATUS_TRY(check_ctx(c));
+  if (!out) return fail(ATUS_ERR_ARG, "null output pointer");
+  *out = nullptr;
+  if (Nk < 1) return fail(ATUS_ERR_ARG, "Nk must be >= 1");
+  const int vk = vkind_of(kind);
+  if (vk < 0) return fail(ATUS_ERR_ARG, "unknown step kind " + std::to_string(kind));
+  const bool batch = c->runs != nullptr;
+  if (batch) {
+    if (!c->runs_set) return fail(ATUS_ERR_STATE, "run parameters not set (atus_batch_set_runs)");
+    if (vk != V_BRAGG && vk != V_NL_IF) return fail(ATUS_ERR_ARG, "batches run bragg_ad or freeprop (IF) steps");
+  } else if (vk == V_BRAGG && c->ncomp != 2) {
+    return fail(ATUS_ERR_STATE, "bragg_ad needs exactly two components");
+  }
+  ATUS_TRY(ensure_kin(c, dt));
+  atus_graph* g = new atus_graph();
+  g->c = c;
+  g->batch = batch;
+  g->vk = vk;
+  g->Nk = Nk;
+  g->dt = dt;
+  g->nf = (size_t)Nk * (batch ? c->nbatch : 1);
+  auto bail = [&](int code, const std::string& m) {
+    atus_graph_destroy(g);
+    return fail(code, m);
+  };
+  if (hipMalloc(&g->dev, sizeof(double) * (Nk + g->nf)) != hipSuccess ||
+      hipHostMalloc(&g->host, sizeof(double) * (Nk + g->nf), hipHostMallocDefault) != hipSuccess)
+    return bail(ATUS_ERR_NOMEM, "graph step tables");
+  VArgs base;
+  StepFeed fd;
+  fd.t1dev = g->dev;
+  if (batch) {
+    batch_vargs(c, base, dt);
+    fd.fbatch = g->dev + Nk;
+  } else {
+    fill_vargs(c, base, dt, laser);
+    fd.fdev = g->dev + Nk;
+  }
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return bail(ATUS_ERR_HIP, "synchronize");
+  const bool prof = c->prof;
+  c->prof = false;
+  if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed) != hipSuccess)
+    return bail(ATUS_ERR_HIP, "stream capture");
+  double t = 0.0;
+  int rc = batch ? batch_block(c, vk, base, Nk, dt, &t, fd) : single_block(c, vk, base, Nk, dt, &t, fd);
+  const std::string err = g_err;
+  hipGraph_t graph = nullptr;
+  const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
+  c->prof = prof;
+  g->graph = graph;
+  if (rc != ATUS_OK) return bail(rc, "capturing the block: " + err);
+  if (ec != hipSuccess || !graph) return bail(ATUS_ERR_HIP, "stream capture did not produce a graph");
+  if (hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0) != hipSuccess)
+    return bail(ATUS_ERR_HIP, "graph instantiation");
+  *out = g;
+  return ATUS_OK;
+}
+
+int atus_graph_launch(atus_graph* g, double* t, const double* F) {
+  if (!g || !t) return fail(ATUS_ERR_ARG, "null argument");
+  atus_ctx* c = g->c;
+  ATUS_TRY(check_ctx(c));
+  // t1 of each step with the block runner's bookkeeping, then amplitudes
+  double tt = *t + 0.5 * g->dt;
+  for (int m = 1; m <= g->Nk; ++m) {
+    g->host[m - 1] = tt + 0.5 * g->dt;
+    tt += (m < g->Nk) ? g->dt : 0.5 * g->dt;
+  }
+  for (size_t i = 0; i < g->nf; ++i) {
+    const double f = F ? F[i] : 1.0;
+    if (!std::isfinite(f)) return fail(ATUS_ERR_NUMERIC, "non-finite amplitude in the table");
+    g->host[g->Nk + i] = f;
+  }
+  HIP_TRY(hipMemcpyAsync(g->dev, g->host, sizeof(double) * (g->Nk + g->nf), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipGraphLaunch(g->exec, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  *t = tt;
   return ATUS_OK;
 }
 

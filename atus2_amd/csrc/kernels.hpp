@@ -69,6 +69,10 @@ struct VArgs {
   const double* Fm;         // Amplitude_at_time() of each run at this step
   long long pencils_per_run;
   long long run_shift;      // run_elems - npts
+  // replayed blocks (hipGraph): t1 and, for single runs, F of this step
+  // come from device memory the host refills before each replay
+  const double* t1p;
+  const double* Fp;
 };
 
 // the parameters of run r replace the shared ones
@@ -220,8 +224,9 @@ __device__ __forceinline__ void nl_point(double2 (&a)[NC], const double (&extra)
 
 // Rabi frequency Omega(x0) and eta(x0) of Do_Bragg_ad (bragg.cpp:224,240)
 __device__ __forceinline__ void bragg_field(double x0, const VArgs& p, double& Om, double2& eta) {
-  const double t1 = p.t1;
-  Om = p.F * p.amp0 *
+  const double t1 = p.t1p ? *p.t1p : p.t1;
+  const double F = p.Fp ? *p.Fp : p.F;
+  Om = F * p.amp0 *
        cos(p.laser_k * x0 - (p.laser_domh + p.chirp * t1 + p.chirp_rate0 * t1) * t1 + p.phase0 / 2);
   double s, c;
   sincos(-0.5 * (p.laser_dk * x0), &s, &c);

@@ -24,7 +24,7 @@ EXPORTED = [
     "atus_rabi_integrals", "atus_energy", "atus_sob_setup", "atus_sob_renormalize", "atus_sob_iter",
     "atus_sob_mu", "atus_set_timing", "atus_last_block_ms", "atus_run_block_profiled", "atus_copy_bandwidth", "atus_tune",
     "atus_batch_create", "atus_batch_set_runs", "atus_batch_run_block", "atus_batch_particle_numbers",
-    "atus_batch_rabi_integrals", "atus_sob_run",
+    "atus_batch_rabi_integrals", "atus_sob_run", "atus_graph_create", "atus_graph_launch", "atus_graph_destroy",
     "atus_net_create", "atus_net_destroy", "atus_net_set_precision", "atus_net_param_count", "atus_net_get_params", "atus_net_set_params",
     "atus_net_forward", "atus_net_gradients", "atus_net_set_data", "atus_net_train", "atus_net_warm_start",
 ]
@@ -108,6 +108,9 @@ def load():
         "atus_batch_particle_numbers": (I, [P, P]),
         "atus_batch_rabi_integrals": (I, [P, I, P, I, D, P]),
         "atus_sob_run": (I, [P, D, C.c_longlong, C.POINTER(C.c_longlong), C.POINTER(D)]),
+        "atus_graph_create": (I, [P, I, D, C.POINTER(Laser), I, pctx]),
+        "atus_graph_launch": (I, [P, C.POINTER(D), P]),
+        "atus_graph_destroy": (I, [P]),
         "atus_net_create": (I, [pctx, I, I, I, I, I, I, C.c_ulonglong]),
         "atus_net_destroy": (I, [P]),
         "atus_net_set_precision": (I, [P, I]),
@@ -506,3 +509,44 @@ class Net:
         a = np.ascontiguousarray(x, dtype=np.float32).reshape(self.dims[0])
         hw = (C.c_double * 3)(*[float(v) for v in half_width])
         _check(_lib.atus_net_warm_start(self._h, _dp(a), hw, ctx._h, int(comp)))
+
+
+class Graph:
+    """One block of `ctx` (Context or BatchContext) captured as a HIP graph
+    (atus_graph_*); launch() replays it with the block's amplitudes."""
+
+    def __init__(self, ctx, kind, dt, Nk, las: Laser | None = None):
+        L = load()
+        h = C.c_void_p()
+        _check(L.atus_graph_create(ctx._h, int(kind), float(dt), C.byref(las) if las is not None else None, int(Nk),
+                                   C.byref(h)))
+        self._h = h
+        self.Nk = int(Nk)
+        self.dt = float(dt)
+
+    def launch(self, t, F=None):
+        """F: amplitudes of V steps 1..Nk ((Nk,) or (Nk, nbatch)) or None."""
+        tt = C.c_double(t)
+        if F is None:
+            _check(_lib.atus_graph_launch(self._h, C.byref(tt), None))
+        else:
+            a = np.ascontiguousarray(F, dtype=np.float64)
+            _check(_lib.atus_graph_launch(self._h, C.byref(tt), _dp(a)))
+        return tt.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.atus_graph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

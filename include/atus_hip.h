@@ -207,6 +207,18 @@ int atus_batch_particle_numbers(atus_ctx *ctx, double *out);
  * place and back as in CRT_Base_IF.h:208-281: out[r*n + q] */
 int atus_batch_rabi_integrals(atus_ctx *ctx, int comp, const double *k0, int n, double threshold, double *out);
 
+/* ---- one block of the sequence hot loop (CRT_Base.h:776-786,
+ * CRT_Base_IF.h:779-788) captured once as a HIP graph and replayed: the
+ * launches of atus_run_block (single runs; the laser parameters are fixed at
+ * capture) or atus_batch_run_block (batches; the run parameters stay those
+ * of atus_batch_set_runs).  Each launch takes the amplitudes of the block,
+ * F[m-1] (single) or F[(m-1)*nbatch + r] (batch), for V step m = 1..Nk at
+ * the t1 the block runner would use (NULL: all 1), and advances *t. */
+typedef struct atus_graph atus_graph;
+int atus_graph_create(atus_ctx *ctx, int kind, double dt, const atus_laser *laser, int Nk, atus_graph **out);
+int atus_graph_launch(atus_graph *graph, double *t, const double *F);
+int atus_graph_destroy(atus_graph *graph);
+
 /* ---- learned warm start of the Sobolev minimiser (no reference
  * counterpart: the network only proposes the start state, which
  * atus_sob_run then polishes to the same residual).  A perceptron

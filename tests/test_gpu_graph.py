@@ -1,0 +1,114 @@
+"""Blocks captured as HIP graphs and replayed (atus_graph_*) against the same
+blocks run eagerly: every replay must leave exactly the eager state, with
+the program's own eager work (reductions, eager blocks, host copies)
+between the replays."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def hip():
+    from atus2_amd import hip as H
+    H.load()
+    return H
+
+
+def _state(orc, g, ncomp, seed):
+    xs = orc.coords_x(g)
+    rng = np.random.default_rng(seed)
+    base = np.exp(-0.5 * sum((x / (2.0 + a)) ** 2 for a, x in enumerate(xs)))
+    out = []
+    for c in range(ncomp):
+        psi = (base * (1.0 if c == 0 else 0.4) + 1e-3 * (rng.standard_normal(base.shape)
+                                                          + 1j * rng.standard_normal(base.shape)))
+        out.append(psi.astype(np.complex128))
+    return out
+
+
+def _amp(t):
+    return float(np.exp(-0.5 * ((t - 0.05) / 0.03) ** 2))
+
+
+@pytest.mark.parametrize("shape,kind", [((4096,), "freeprop"), ((8192,), "bragg_ad"), ((64, 128), "bragg_ad"),
+                                        ((16, 32, 64), "freeprop_if"), ((32, 16, 64), "bragg_ad")])
+def test_graph_replays_equal_eager_blocks(hip, orc, shape, kind):
+    dim = len(shape)
+    g = orc.make_grid(shape, L=[40.0, 30.0, 36.0][:dim])
+    ncomp = 1 if kind == "freeprop" else 2
+    psis = _state(orc, g, ncomp, 5)
+    hk = {"freeprop": hip.STEP_FREEPROP, "freeprop_if": hip.STEP_FREEPROP_IF, "bragg_ad": hip.STEP_BRAGG_AD}[kind]
+    las = hip.laser(amp0=5.0, laser_k=3 * 2 * np.pi / 40.0, laser_domh=8.0, beta0=0.01, deltaL=(0.0, 0.3))
+    dt, Nk = 1e-3, 20
+    ctxs = []
+    for _ in range(2):
+        ctx = hip.Context.from_grid(g, ncomp)
+        for c in range(ncomp):
+            ctx.upload(c, psis[c])
+        ctx.set_alpha([0.5] * dim)
+        ctx.set_g([10.0] if ncomp == 1 else [100.0, 95.0, 95.0, 100.0])
+        if kind == "freeprop":
+            ctx.set_potential(0, 0.5 * sum(x ** 2 for x in orc.coords_x(g)) * np.ones(shape))
+        ctxs.append(ctx)
+    eager, rep = ctxs
+    amp = _amp if kind == "bragg_ad" else None
+    try:
+        with hip.Graph(rep, hk, dt, Nk, las) as G:
+            te = tr = 0.0
+            for blk in range(4):
+                te = eager.run_block(hk, dt, Nk, te, las, amp)
+                F = None if amp is None else np.array([amp(t1) for t1 in hip.BatchContext.step_times(tr, dt, Nk)])
+                tr = G.launch(tr, F)
+                assert tr == te
+                for c in range(ncomp):
+                    assert np.array_equal(rep.download(c), eager.download(c)), (blk, c)
+                # the program's own eager work between replays
+                assert rep.particle_number(0) == eager.particle_number(0)
+                if blk == 1:
+                    te = eager.run_block(hk, dt, 3, te, las, amp)
+                    tr = rep.run_block(hk, dt, 3, tr, las, amp)
+    finally:
+        for c in ctxs:
+            c.close()
+
+
+@pytest.mark.parametrize("shape", [(1024,), (16384,), (32, 64)])
+def test_graph_replays_equal_eager_batch_blocks(hip, orc, shape):
+    dim = len(shape)
+    g = orc.make_grid(shape, L=[40.0, 30.0][:dim])
+    nb, dt, Nk = 4, 1e-3, 10
+    dx = [g.dx[a] for a in range(dim)]
+    dk = [g.dk[a] for a in range(dim)]
+    rng = np.random.default_rng(2)
+    gsl = [[gg, 0.95 * gg, 0.95 * gg, gg] for gg in rng.uniform(1, 200, nb)]
+    lasers = [hip.laser(amp0=a, laser_k=3 * 2 * np.pi / 40.0, laser_domh=d)
+              for a, d in zip(rng.uniform(1, 10, nb), rng.uniform(6, 10, nb))]
+    sig = rng.uniform(0.02, 0.05, nb)
+    bs = []
+    for _ in range(2):
+        B = hip.BatchContext(shape, dx, dk, nb)
+        for r in range(nb):
+            p = _state(orc, g, 2, 10 + r)
+            B.upload(2 * r, p[0])
+            B.upload(2 * r + 1, p[1])
+        B.set_alpha([0.5] * dim)
+        B.set_runs(gsl, lasers)
+        bs.append(B)
+    eager, rep = bs
+    try:
+        with hip.Graph(rep, hip.STEP_BRAGG_AD, dt, Nk) as G:
+            t = 0.02
+            for blk in range(3):
+                t1 = np.array(eager.step_times(t, dt, Nk))[:, None]
+                F = np.exp(-0.5 * ((t1 - 0.04) / sig[None, :]) ** 2)
+                te = eager.run_batch_block(hip.STEP_BRAGG_AD, dt, Nk, t, F)
+                tr = G.launch(t, F)
+                assert tr == te
+                for c in range(2 * nb):
+                    assert np.array_equal(rep.download(c), eager.download(c)), (blk, c)
+                assert np.array_equal(rep.particle_numbers(), eager.particle_numbers())
+                t = te
+    finally:
+        for B in bs:
+            B.close()
