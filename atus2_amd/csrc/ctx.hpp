@@ -89,8 +89,10 @@ struct atus_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms = 0;
 
-  // values per thread of the FFT kernels, per kernel family (ATUS_KFAM_*)
-  int rm[ATUS_NKFAM] = {16, 16, 16, 16, 16};
+  // values per thread of the FFT kernels, per kernel family (ATUS_KFAM_*);
+  // defaults: what the autotuner picks at 256^3 (8 for the strided passes
+  // and for the register-heavy V pass, which spills at 16)
+  int rm[ATUS_NKFAM] = {16, 8, 16, 8, 16};
 
   // per-launch event pairs while profiling
   bool prof = false;

@@ -23,6 +23,8 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # HBM bytes per launch of each kernel family at 256^3, from rocprofv3
 # FETCH_SIZE / WRITE_SIZE passes scaled by a copy kernel of known size
 TRAFFIC_TABLE = os.path.join(ROOT, "profiles", "hbm_traffic_bragg256.json")
+# the library's values per thread when a family is not tuned
+DEFAULT_RM = {"pass_contig": 16, "pass_strided": 8, "kin": 16, "vfuse": 8, "point": 16}
 
 
 def log(msg):
@@ -176,7 +178,7 @@ def main():
     r = gpu_run(args, wl)
     npts = float(math.prod(wl.shape))
     value = npts * args.steps / r["elapsed"] / 1e9
-    traffic = measured_traffic(wl.n, r["dominant"], r["tuning"].get(r["dominant"], 16))
+    traffic = measured_traffic(wl.n, r["dominant"], r["tuning"].get(r["dominant"], DEFAULT_RM[r["dominant"]]))
     gs = ground_state_run(args.gs_cases) if args.gs_cases > 0 else None
     cpu = None
     if not args.no_cpu_baseline:

@@ -171,7 +171,9 @@ int atus_run_block_profiled(atus_ctx *ctx, const atus_step_params *sp, int Nk, d
 
 /* tuning: values per thread (8 or 16) of the FFT kernels of one kernel
  * family (ATUS_KFAM_*; the Sobolev filter follows ATUS_KFAM_KIN).  16 halves
- * the LDS exchanges, 8 halves the registers per thread.  Default 16. */
+ * the LDS exchanges, 8 halves the registers per thread.  Defaults: 16 for
+ * the contiguous passes and the kinetic step, 8 for the strided passes and
+ * the fused V pass. */
 int atus_tune(atus_ctx *ctx, int family, int values_per_thread);
 
 /* HBM copy probe: copies all components (ncomp * npts complex128) into a
