@@ -113,10 +113,9 @@ int ensure_twiddles(atus_ctx* c, int N) {
 //   E_a[j] = exp(i * s * phi_a(j)),  phi_a = -dt * (k_a alpha_a) k_a
 // (CRT_Base::Init, CRT_Base.h:253-274, factorised over the axes); axis 0
 // carries the ft(-1) * ft(+1) scale.
-int ensure_kin(atus_ctx* c, double dt) {
-  if (c->kin_valid && c->kin_dt == dt) return ATUS_OK;
+void kin_tables(const atus_ctx* c, const double* alpha, double dt, std::vector<double2>& h) {
   const int nm = c->kin_nmax;
-  std::vector<double2> h((size_t)2 * 3 * nm, make_double2(1.0, 0.0));
+  h.assign((size_t)2 * 3 * nm, make_double2(1.0, 0.0));
   const double scale = ft_factor(c->dim, c->dx) * ft_factor(c->dim, c->dk);
   const double mdt = -dt;
   for (int f = 0; f < 2; ++f) {
@@ -124,7 +123,7 @@ int ensure_kin(atus_ctx* c, double dt) {
       const int N = c->n[a], sh = N / 2;
       for (int j = 0; j < N; ++j) {
         const double k = c->dk[a] * (double)(((j + sh) % N) - sh);
-        const double phi = mdt * ((k * c->alpha[a]) * k);
+        const double phi = mdt * ((k * alpha[a]) * k);
         const double ph = (f == 0) ? 0.5 * phi : phi;
         double2 e = make_double2(cos(ph), sin(ph));
         if (a == 0) e = make_double2(e.x * scale, e.y * scale);
@@ -132,7 +131,21 @@ int ensure_kin(atus_ctx* c, double dt) {
       }
     }
   }
+}
+
+int ensure_kin(atus_ctx* c, double dt) {
+  if (c->kin_valid && c->kin_dt == dt) return ATUS_OK;
+  const int nm = c->kin_nmax;
+  std::vector<double2> h, h2;
+  kin_tables(c, c->alpha, dt, h);
   HIP_TRY(hipMemcpyAsync(c->kin, h.data(), sizeof(double2) * h.size(), hipMemcpyHostToDevice, c->stream));
+  if (c->alpha2_set) {
+    // CRT_Base_2::Init (CRT_Base_2.h:195-221): the tables of Alpha_2
+    if (!c->kin2 && hipMalloc(&c->kin2, sizeof(double2) * 2 * 3 * nm) != hipSuccess)
+      return fail(ATUS_ERR_NOMEM, "second-species tables");
+    kin_tables(c, c->alpha2, dt, h2);
+    HIP_TRY(hipMemcpyAsync(c->kin2, h2.data(), sizeof(double2) * h2.size(), hipMemcpyHostToDevice, c->stream));
+  }
   std::vector<double2> hz;
   if (c->fs) {
     // four-step spectrum: row k1, column k2 holds frequency k1 + N1 k2
@@ -159,6 +172,12 @@ KinTabs kin_tabs(atus_ctx* c, int f) {
   if (c->fs) {
     kt.ez = c->kinz + (size_t)f * c->npts;
     kt.ez_rows = c->fs_n1;
+  }
+  if (c->alpha2_set && c->kin2) {
+    kt.f0 = c->kin2 + ((size_t)f * 3 + 0) * nm;
+    kt.f1 = c->dim > 1 ? c->kin2 + ((size_t)f * 3 + 1) * nm : nullptr;
+    kt.f2 = c->dim > 2 ? c->kin2 + ((size_t)f * 3 + 2) * nm : nullptr;
+    kt.split = c->ncomp / 2;
   }
   return kt;
 }
@@ -489,6 +508,7 @@ int atus_ctx_destroy(atus_ctx* c) {
   (void)hipFree(c->ftab);
   (void)hipFree(c->kinz);
   (void)hipFree(c->tbuf);
+  (void)hipFree(c->kin2);
   (void)hipFree(c->sred_part);
   (void)hipFree(c->sred_out);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -554,6 +574,18 @@ int atus_set_g(atus_ctx* c, const double* gs) {
 int atus_set_alpha(atus_ctx* c, const double alpha[3]) {
   ATUS_TRY(check_ctx(c));
   for (int a = 0; a < 3; ++a) c->alpha[a] = a < c->dim ? alpha[a] : 0.0;
+  c->kin_valid = false;
+  return ATUS_OK;
+}
+
+int atus_set_alpha2(atus_ctx* c, const double alpha2[3]) {
+  ATUS_TRY(check_ctx(c));
+  ATUS_TRY(check_single(c, "atus_set_alpha2"));
+  if (!alpha2) return fail(ATUS_ERR_ARG, "null alpha2");
+  if (c->ncomp < 2 || c->ncomp % 2) return fail(ATUS_ERR_STATE, "two species need an even number of components");
+  if (c->fs) return fail(ATUS_ERR_UNSUPPORTED, "two species on 1D grids longer than 4096 are not covered");
+  for (int a = 0; a < 3; ++a) c->alpha2[a] = a < c->dim ? alpha2[a] : 0.0;
+  c->alpha2_set = true;
   c->kin_valid = false;
   return ATUS_OK;
 }
@@ -634,6 +666,7 @@ namespace {
 // the single-run amplitude ([Nk])
 struct StepFeed {
   const atus_step_params* sp = nullptr;
+  const atus_ds_params* ds = nullptr;  // pulse gates of the two-species Bragg step
   const double* fbatch = nullptr;
   const double* t1dev = nullptr;
   const double* fdev = nullptr;
@@ -643,7 +676,10 @@ struct StepFeed {
 // kinetic, (V, full) x (Nk-1), V, half), without synchronisation; *t
 // advances with the reference's bookkeeping
 int single_block(atus_ctx* c, int vk, const VArgs& base, int Nk, double dt, double* t, const StepFeed& fd) {
-  const bool fused = c->ncomp <= 2;
+  // the fused kernels cover one species of up to two components; two
+  // species (per-component kinetic tables, four-component steps) run the
+  // V step and the kinetic step as separate passes
+  const bool fused = c->ncomp <= 2 && !c->alpha2_set && vk != V_NL_IF2 && vk != V_BRAGG_DS;
   const int N0 = c->n[0], NL = c->n[c->dim - 1];
   const long long inner0 = c->inner_of(0);
   double tt = *t;
@@ -673,6 +709,12 @@ int single_block(atus_ctx* c, int vk, const VArgs& base, int Nk, double dt, doub
     } else {
       va.F = (fd.sp && fd.sp->amp) ? fd.sp->amp(va.t1, fd.sp->amp_user) : 1.0;
       if (!std::isfinite(va.F)) return fail(ATUS_ERR_NUMERIC, "amplitude callback returned a non-finite value");
+    }
+    if (fd.ds) {
+      // bragg_ds.cpp:120-128: a pulse is off once t1 - seq.time exceeds its duration
+      const double time = va.t1 - fd.ds->seq_time;
+      va.mode1 = time > fd.ds->duration[0] ? 0.0 : 1.0;
+      va.mode2 = time > fd.ds->duration[1] ? 0.0 : 1.0;
     }
     const int f = (m < Nk) ? 1 : 0;
     if (!fused) {
@@ -820,6 +862,39 @@ int atus_run_block(atus_ctx* c, const atus_step_params* sp, int Nk, double* t) {
   return finish_block(c);
 }
 
+int atus_run_block_ds(atus_ctx* c, const atus_step_params* sp, const atus_ds_params* ds, int Nk, double* t) {
+  ATUS_TRY(check_ctx(c));
+  ATUS_TRY(check_single(c, "atus_run_block_ds"));
+  if (!sp || !ds || !t) return fail(ATUS_ERR_ARG, "null argument");
+  if (Nk < 1) return fail(ATUS_ERR_ARG, "Nk must be >= 1");
+  if (c->ncomp < 2 || c->ncomp % 2) return fail(ATUS_ERR_STATE, "two species need an even number of components");
+  int vk;
+  if (sp->kind == ATUS_STEP_FREEPROP_IF) {
+    vk = V_NL_IF2;
+  } else if (sp->kind == ATUS_STEP_BRAGG_AD) {
+    if (c->ncomp != 4) return fail(ATUS_ERR_STATE, "the two-species Bragg step needs four components");
+    vk = V_BRAGG_DS;
+  } else {
+    return fail(ATUS_ERR_ARG, "two-species blocks run freeprop (IF) or bragg_ad steps");
+  }
+  ATUS_TRY(ensure_kin(c, sp->dt));
+  VArgs base;
+  fill_vargs(c, base, sp->dt, &sp->laser);
+  const atus_laser& L2 = ds->laser2;
+  base.amp2 = L2.amp0;
+  base.laser_k2 = L2.laser_k;
+  base.laser_domh2 = L2.laser_domh;
+  base.laser_dk2 = L2.laser_dk;
+  base.chirp_rate2 = L2.chirp_rate0;
+  base.phase2 = L2.phase0;
+  base.beta2 = L2.beta0;
+  if (c->timing) HIP_TRY(hipEventRecord(c->ev0, c->stream));
+  StepFeed fd;
+  fd.ds = ds;
+  ATUS_TRY(single_block(c, vk, base, Nk, sp->dt, t, fd));
+  return finish_block(c);
+}
+
 int atus_particle_number(atus_ctx* c, int comp, double* out) {
   ATUS_TRY(check_ctx(c));
   ATUS_TRY(check_comp(c, comp));
@@ -884,7 +959,8 @@ int atus_energy(atus_ctx* c, double* out) {
     RedEkin fn{};
     fn.psik = c->scratch;
     fn.gr = c->grid();
-    for (int a = 0; a < 3; ++a) fn.alpha[a] = c->alpha[a];
+    const double* al = (c->alpha2_set && i >= c->ncomp / 2) ? c->alpha2 : c->alpha;
+    for (int a = 0; a < 3; ++a) fn.alpha[a] = al[a];
     double s;
     ATUS_TRY(reduce(c, fn, c->npts, &s));
     ekin += s;

@@ -64,6 +64,10 @@ struct atus_ctx {
   double kin_dt = NAN;
   bool kin_valid = false;
   double alpha[3] = {0, 0, 0};
+  // second species (CRT_Base_2): components [ncomp/2, ncomp) use alpha2
+  bool alpha2_set = false;
+  double alpha2[3] = {0, 0, 0};
+  double2* kin2 = nullptr;  // same layout as kin
   double gs[16] = {0};
 
   double* red_part = nullptr;

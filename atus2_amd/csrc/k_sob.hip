@@ -57,6 +57,7 @@ hipError_t launch_point(int vk, int nc, double2* psi, const Grid& gr, const VArg
   ATUS_PT(V_NL, 1) ATUS_PT(V_NL, 2) ATUS_PT(V_NL, 3) ATUS_PT(V_NL, 4)
   ATUS_PT(V_NL_IF, 1) ATUS_PT(V_NL_IF, 2) ATUS_PT(V_NL_IF, 3) ATUS_PT(V_NL_IF, 4)
   ATUS_PT(V_BRAGG, 2)
+  ATUS_PT(V_NL_IF2, 2) ATUS_PT(V_NL_IF2, 4) ATUS_PT(V_BRAGG_DS, 4)
 #undef ATUS_PT
   return hipErrorInvalidValue;
 }

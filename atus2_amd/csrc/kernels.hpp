@@ -23,7 +23,8 @@
 
 namespace atus {
 
-enum VKind { V_NONE = 0, V_NL = 1, V_NL_IF = 2, V_BRAGG = 3 };
+// V_NL_IF2 / V_BRAGG_DS: two species, components [NC/2, NC) are species 2
+enum VKind { V_NONE = 0, V_NL = 1, V_NL_IF = 2, V_BRAGG = 3, V_NL_IF2 = 4, V_BRAGG_DS = 5 };
 
 struct Grid {
   int dim;
@@ -73,6 +74,10 @@ struct VArgs {
   // come from device memory the host refills before each replay
   const double* t1p;
   const double* Fp;
+  // second species (CRT_Base_IF_2): its laser and gravity, and the pulse
+  // gates of bragg_ds (1: on, 0: off) for this step
+  double amp2, laser_k2, laser_domh2, laser_dk2, chirp_rate2, phase2, beta2;
+  double mode1, mode2;
 };
 
 // the parameters of run r replace the shared ones
@@ -279,6 +284,83 @@ __device__ __forceinline__ void bragg_point(double2& a, double2& b, double bx, d
   b.y = (O21r * ai + O21i * ar) + (O22r * bi + O22i * br);
 }
 
+// the 2x2 exponential of one species of bragg_ds (bragg_ds.cpp:174-206):
+// std::complex arithmetic of the reference written out; gamma divides by
+// |d^2 + Omega^2| as the reference does
+__device__ __forceinline__ void ds_pair(double2& a, double2& b, double V11, double V22, double Om, double theta,
+                                        double dt) {
+  double ei, er;
+  sincos(theta, &ei, &er);
+  const double tmp1 = sqrt((V11 - V22) * (V11 - V22) + 4.0 * Om * Om);
+  const double Ep = 0.5 * (V11 + V22 + tmp1);
+  const double Em = 0.5 * (V11 + V22 - tmp1);
+  const double d0 = V11 - Ep, d1 = V11 - Em;
+  const double den0 = fabs(d0 * d0 + Om * Om), den1 = fabs(d1 * d1 + Om * Om);
+  double s0, c0, s1, c1;
+  sincos(-dt * Ep, &s0, &c0);
+  sincos(-dt * Em, &s1, &c1);
+  const double g0r = c0 / den0, g0i = s0 / den0, g1r = c1 / den1, g1i = s1 / den1;
+  const double tr = d0 * g0r + d1 * g1r, ti = d0 * g0i + d1 * g1i;
+  const double o2 = Om * Om;
+  const double M00r = o2 * (g0r + g1r), M00i = o2 * (g0i + g1i);
+  const double ar = -Om * er, ai = -Om * ei;
+  const double M01r = ar * tr - ai * ti, M01i = ar * ti + ai * tr;
+  const double cr = -Om * er, ci = -Om * (-ei);
+  const double M10r = cr * tr - ci * ti, M10i = cr * ti + ci * tr;
+  const double dd0 = d0 * d0, dd1 = d1 * d1;
+  const double M11r = dd0 * g0r + dd1 * g1r, M11i = dd0 * g0i + dd1 * g1i;
+  const double p0r = a.x, p0i = a.y, p1r = b.x, p1i = b.y;
+  a.x = (M00r * p0r - M00i * p0i) + (M01r * p1r - M01i * p1i);
+  a.y = (M00r * p0i + M00i * p0r) + (M01r * p1i + M01i * p1r);
+  b.x = (M10r * p0r - M10i * p0i) + (M11r * p1r - M11i * p1i);
+  b.y = (M10r * p0i + M10i * p0r) + (M11r * p1i + M11i * p1r);
+}
+
+__device__ __forceinline__ void ds_diag(double2& a, double phi, double dt) {
+  double im1, re1;
+  sincos(-phi * dt, &im1, &re1);
+  const double tmp1 = a.x;
+  a.x = tmp1 * re1 - a.y * im1;
+  a.y = tmp1 * im1 + a.y * re1;
+}
+
+// Bragg_single::Do_Bragg_ad of bragg_ds (bragg_ds.cpp:112-300): two species
+// of two components, couplings through all four densities, one laser per
+// species switched off by its gate
+__device__ __forceinline__ void bragg_ds_point(double2 (&a)[4], double x0, const VArgs& p) {
+  double n[4], phi[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) n[j] = a[j].x * a[j].x + a[j].y * a[j].y;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    double ph = 0.0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) ph += p.gs[j + 4 * i] * n[j];
+    ph += (i < 2 ? p.beta0 : p.beta2) * x0 - p.deltaL[i];
+    phi[i] = ph;
+  }
+  const double t1 = p.t1p ? *p.t1p : p.t1;
+  const double Om = p.mode1 * p.amp0 * cos(p.laser_k * x0 - (p.laser_domh + p.chirp_rate0 * t1) * t1 + p.phase0 / 2);
+  const double Om2 = p.mode2 * p.amp2 * cos(p.laser_k2 * x0 - (p.laser_domh2 + p.chirp_rate2 * t1) * t1 + p.phase2 / 2);
+  const double th1 = -0.5 * (p.laser_dk * x0 + p.phase0);
+  const double th2 = -0.5 * (p.laser_dk2 * x0 + p.phase2);
+  if (Om == 0.0 && Om2 == 0.0) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) ds_diag(a[i], phi[i], p.dt);
+  } else if (Om == 0.0) {
+    ds_diag(a[0], phi[0], p.dt);
+    ds_diag(a[1], phi[1], p.dt);
+    ds_pair(a[2], a[3], phi[2], phi[3], Om2, th2, p.dt);
+  } else if (Om2 == 0.0) {
+    ds_diag(a[2], phi[2], p.dt);
+    ds_diag(a[3], phi[3], p.dt);
+    ds_pair(a[0], a[1], phi[0], phi[1], Om, th1, p.dt);
+  } else {
+    ds_pair(a[0], a[1], phi[0], phi[1], Om, th1, p.dt);
+    ds_pair(a[2], a[3], phi[2], phi[3], Om2, th2, p.dt);
+  }
+}
+
 // V step on the registers of NC component pencils at one element.
 // x0: first-axis coordinate of the element; l: linear point index.
 template <int VK, int NC>
@@ -297,6 +379,15 @@ __device__ __forceinline__ void v_point(double2 (&a)[NC], double x0, long long l
   } else if constexpr (VK == V_BRAGG) {
     static_assert(NC == 2, "Bragg coupling needs two components");
     bragg_point(a[0], a[1], p.beta0 * x0, Om, eta, p);
+  } else if constexpr (VK == V_NL_IF2) {
+    // CRT_Base_IF_2::Do_NL_Step (CRT_Base_IF_2.h:324-366)
+    double extra[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) extra[c] = (c < NC / 2 ? p.beta0 : p.beta2) * x0 - p.deltaL[c];
+    nl_point<NC, false>(a, extra, p);
+  } else if constexpr (VK == V_BRAGG_DS) {
+    static_assert(NC == 4, "the two-species Bragg step has four components");
+    bragg_ds_point(a, x0, p);
   }
 }
 
@@ -338,6 +429,12 @@ struct KinTabs {
   // ez[(pid % ez_rows) * N + j] (the four-step 1D spectrum), else NULL
   const double2* ez;
   long long ez_rows;
+  // second species (CRT_Base_2): components blockIdx.y >= split use these
+  // tables instead (one component per blockIdx.y); f0 NULL: one species
+  const double2* f0;
+  const double2* f1;
+  const double2* f2;
+  int split;
 };
 
 template <int N, int RM, bool CONTIG, int VK, int NC, bool BATCH = false>
@@ -349,15 +446,21 @@ __global__ __launch_bounds__(256) void k_kin(double2* __restrict__ psi, long lon
   PencilMap<N, RM, CONTIG> m(g);
   double2* lds = lds_all + m.p * lds_stride<N>();
   const double2* twl = stage_table<N>(lds_all + (size_t)g.T * lds_stride<N>(), tw);
-  const double2* e0l = stage_table<N>(lds_all + (size_t)g.T * lds_stride<N>() + N, kt.e0);
+  // the component of this block (one per blockIdx.y) may belong to the
+  // second species, with its own tables (workgroup-uniform choice)
+  const bool sp2 = NC == 1 && kt.f0 != nullptr && (int)blockIdx.y >= kt.split;
+  const double2* t0 = sp2 ? kt.f0 : kt.e0;
+  const double2* t1t = sp2 ? kt.f1 : kt.e1;
+  const double2* t2t = sp2 ? kt.f2 : kt.e2;
+  const double2* e0l = stage_table<N>(lds_all + (size_t)g.T * lds_stride<N>() + N, t0);
   // pencil constant of the phase (axes 1, 2)
   double2 cp = make_double2(1.0, 0.0);
   if constexpr (!CONTIG) {
     const long long col = m.pid % g.inner;
-    if (kt.e1) {
+    if (t1t) {
       const int i1 = (int)(col / kt.n2), i2 = (int)(col % kt.n2);
-      cp = kt.e1[i1];
-      if (kt.e2) cp = cmul(cp, kt.e2[i2]);
+      cp = t1t[i1];
+      if (t2t) cp = cmul(cp, t2t[i2]);
     }
   }
   double2 v[NC][R];

@@ -25,6 +25,7 @@ EXPORTED = [
     "atus_sob_mu", "atus_set_timing", "atus_last_block_ms", "atus_run_block_profiled", "atus_copy_bandwidth", "atus_tune",
     "atus_batch_create", "atus_batch_set_runs", "atus_batch_run_block", "atus_batch_particle_numbers",
     "atus_batch_rabi_integrals", "atus_sob_run", "atus_graph_create", "atus_graph_launch", "atus_graph_destroy",
+    "atus_set_alpha2", "atus_run_block_ds",
     "atus_net_create", "atus_net_destroy", "atus_net_set_precision", "atus_net_param_count", "atus_net_get_params", "atus_net_set_params",
     "atus_net_forward", "atus_net_gradients", "atus_net_set_data", "atus_net_train", "atus_net_warm_start",
 ]
@@ -43,6 +44,10 @@ class Laser(C.Structure):
 
 
 AMP_FN = C.CFUNCTYPE(C.c_double, C.c_double, C.c_void_p)
+
+
+class DsParams(C.Structure):
+    _fields_ = [("laser2", Laser), ("seq_time", C.c_double), ("duration", C.c_double * 2)]
 
 
 class BatchRun(C.Structure):
@@ -111,6 +116,8 @@ def load():
         "atus_graph_create": (I, [P, I, D, C.POINTER(Laser), I, pctx]),
         "atus_graph_launch": (I, [P, C.POINTER(D), P]),
         "atus_graph_destroy": (I, [P]),
+        "atus_set_alpha2": (I, [P, C.POINTER(D)]),
+        "atus_run_block_ds": (I, [P, C.POINTER(StepParams), C.POINTER(DsParams), I, C.POINTER(D)]),
         "atus_net_create": (I, [pctx, I, I, I, I, I, I, C.c_ulonglong]),
         "atus_net_destroy": (I, [P]),
         "atus_net_set_precision": (I, [P, I]),
@@ -318,6 +325,25 @@ class Context:
         N = (C.c_double * 4)()
         _check(_lib.atus_sob_iter(self._h, C.byref(r), C.byref(rn), N))
         return r.value, bool(rn.value), list(N[:self.ncomp])
+
+    def set_alpha2(self, alpha2):
+        """Kinetic coefficients of species 2 (components [ncomp/2, ncomp))."""
+        a = (C.c_double * 3)(*(list(alpha2) + [0.0] * (3 - len(alpha2))))
+        _check(_lib.atus_set_alpha2(self._h, a))
+
+    def run_block_ds(self, kind, dt, Nk, t, las: Laser, las2: Laser, seq_time=0.0, duration=(1e300, 1e300)):
+        """One two-species block (CRT_Base_IF_2 / bragg_ds); returns the new t."""
+        sp = StepParams()
+        sp.kind = kind
+        sp.dt = dt
+        sp.laser = las
+        ds = DsParams()
+        ds.laser2 = las2
+        ds.seq_time = float(seq_time)
+        ds.duration[0], ds.duration[1] = float(duration[0]), float(duration[1])
+        tt = C.c_double(t)
+        _check(_lib.atus_run_block_ds(self._h, C.byref(sp), C.byref(ds), int(Nk), C.byref(tt)))
+        return tt.value
 
     def sob_run(self, eps, maxit=0):
         """CSOB_Base::run: iterations until res_tot <= eps (maxit > 0 caps

@@ -119,6 +119,25 @@ typedef struct {
 
 int atus_run_block(atus_ctx *ctx, const atus_step_params *sp, int Nk, double *t);
 
+/* ---- two species (CRT_Base_2, CRT_Base_IF_2, the bragg_ds driver):
+ * components [ncomp/2, ncomp) are species 2 */
+/* m_alpha2 (CRT_Base_2.h:195-221, Alpha_2): kinetic coefficients of species 2 */
+int atus_set_alpha2(atus_ctx *ctx, const double alpha2[3]);
+typedef struct {
+  atus_laser laser2;  /* species 2 (CRT_Base_IF_2.h:104-123): amp0 = Amp_2[0], laser_k =
+                         laser_k_2, laser_dk = laser_dk_2, laser_domh = laser_domh_2,
+                         chirp_rate0 = chirp_rate2[0], phase0 = phase2[0], beta0 =
+                         Beta_2[0]; chirp and deltaL are not read */
+  double seq_time;    /* sequence_item::time, start of the pulses */
+  double duration[2]; /* sequence_item::duration[0..1]: pulse lengths of species 1 and 2 */
+} atus_ds_params;
+/* one block of CRT_Base_IF_2::run_sequence (CRT_Base_IF_2.h:439-600) with
+ * kind ATUS_STEP_FREEPROP_IF (CRT_Base_IF_2::Do_NL_Step, :324-366) or
+ * ATUS_STEP_BRAGG_AD (Do_Bragg_ad of bragg_ds.cpp:112-300, four
+ * components); sp->laser is species 1 (bragg_ds reads neither its chirp
+ * nor AMP_T); the kinetic steps use alpha and alpha2 per species */
+int atus_run_block_ds(atus_ctx *ctx, const atus_step_params *sp, const atus_ds_params *ds, int Nk, double *t);
+
 /* ---- reductions (CRT_Base.h:481-599, CRT_Base_IF.h:208-281) */
 /* Get_Particle_Number: ar * sum |psi_comp|^2 */
 int atus_particle_number(atus_ctx *ctx, int comp, double *out);

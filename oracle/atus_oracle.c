@@ -543,10 +543,143 @@ void orc_rt_set_dt(orc_rt *s, double dt) {
   s->dt = dt;
   orc_kinetic_table(&s->g, s->alpha, dt, 0.5, s->half_tab);
   orc_kinetic_table(&s->g, s->alpha, dt, 1.0, s->full_tab);
+  if (s->two_species) {
+    /* CRT_Base_2::Init (CRT_Base_2.h:195-221), second pair of tables from Alpha_2 */
+    orc_kinetic_table(&s->g, s->alpha2, dt, 0.5, s->half_tab2);
+    orc_kinetic_table(&s->g, s->alpha2, dt, 1.0, s->full_tab2);
+  }
   s->tab_dt = dt;
 }
 
+void orc_ft_step2(const orc_grid *g, int ncomp, double **psi, const double *tab1, const double *tab2) {
+  long long np = orc_npts(g);
+  for (int c = 0; c < ncomp; c++) orc_ft(g, psi[c], -1);
+  for (int c = 0; c < ncomp; c++) {
+    double *p = psi[c];
+    const double *tab = c < ncomp / 2 ? tab1 : tab2;
+#pragma omp parallel for schedule(static)
+    for (long long l = 0; l < np; l++) {
+      double re = p[2 * l], im = p[2 * l + 1];
+      p[2 * l] = re * tab[2 * l] - im * tab[2 * l + 1];
+      p[2 * l + 1] = im * tab[2 * l] + re * tab[2 * l + 1];
+    }
+  }
+  for (int c = 0; c < ncomp; c++) orc_ft(g, psi[c], 1);
+}
+
+void orc_nl_if2_step(const orc_grid *g, int ncomp, double **psi, const double *gs, double dt, double beta,
+                     double beta2, const double *deltaL) {
+  long long np = orc_npts(g);
+  const double mdt = -dt;
+#pragma omp parallel for schedule(static)
+  for (long long l = 0; l < np; l++) {
+    double x[3], phi[4];
+    orc_get_x(g, l, x);
+    for (int i = 0; i < ncomp; i++) {
+      phi[i] = 0;
+      for (int j = 0; j < ncomp; j++) {
+        const double re = psi[j][2 * l], im = psi[j][2 * l + 1];
+        phi[i] += gs[j + ncomp * i] * (re * re + im * im);
+      }
+      phi[i] += (i < ncomp / 2 ? beta : beta2) * x[0] - deltaL[i];
+      phi[i] *= mdt;
+    }
+    for (int i = 0; i < ncomp; i++) {
+      double si, co;
+      sincos(phi[i], &si, &co);
+      const double re = psi[i][2 * l];
+      psi[i][2 * l] = psi[i][2 * l] * co - psi[i][2 * l + 1] * si;
+      psi[i][2 * l + 1] = psi[i][2 * l + 1] * co + re * si;
+    }
+  }
+}
+
+/* psi_i *= exp(-i dt phi_i) as in the Omega == 0 branches of bragg_ds.cpp:152-160 */
+static void ds_diag(double **psi, long long l, int i, double phi, double dt) {
+  double im1, re1;
+  sincos(-phi * dt, &im1, &re1);
+  const double tmp1 = psi[i][2 * l];
+  psi[i][2 * l] = tmp1 * re1 - psi[i][2 * l + 1] * im1;
+  psi[i][2 * l + 1] = tmp1 * im1 + psi[i][2 * l + 1] * re1;
+}
+
+/* the 2x2 exponential of one species (bragg_ds.cpp:174-206), std::complex
+ * arithmetic written out (products with a zero imaginary part are exact) */
+static void ds_pair(double **psi, long long l, int i0, double V11, double V22, double Om, double theta,
+                    double dt) {
+  const double er = cos(theta), ei = sin(theta);
+  const double tmp1 = sqrt((V11 - V22) * (V11 - V22) + 4.0 * Om * Om);
+  const double Ep = 0.5 * (V11 + V22 + tmp1);
+  const double Em = 0.5 * (V11 + V22 - tmp1);
+  const double d0 = V11 - Ep, d1 = V11 - Em;
+  const double den0 = fabs(d0 * d0 + Om * Om), den1 = fabs(d1 * d1 + Om * Om);
+  const double g0r = cos(-dt * Ep) / den0, g0i = sin(-dt * Ep) / den0;
+  const double g1r = cos(-dt * Em) / den1, g1i = sin(-dt * Em) / den1;
+  const double tr = d0 * g0r + d1 * g1r, ti = d0 * g0i + d1 * g1i;
+  const double o2 = Om * Om;
+  const double M00r = o2 * (g0r + g1r), M00i = o2 * (g0i + g1i);
+  const double ar = -Om * er, ai = -Om * ei;          /* -Omega * eta */
+  const double M01r = ar * tr - ai * ti, M01i = ar * ti + ai * tr;
+  const double cr = -Om * er, ci = -Om * (-ei);       /* -Omega * conj(eta) */
+  const double M10r = cr * tr - ci * ti, M10i = cr * ti + ci * tr;
+  const double dd0 = d0 * d0, dd1 = d1 * d1;
+  const double M11r = dd0 * g0r + dd1 * g1r, M11i = dd0 * g0i + dd1 * g1i;
+  const double p0r = psi[i0][2 * l], p0i = psi[i0][2 * l + 1];
+  const double p1r = psi[i0 + 1][2 * l], p1i = psi[i0 + 1][2 * l + 1];
+  psi[i0][2 * l] = (M00r * p0r - M00i * p0i) + (M01r * p1r - M01i * p1i);
+  psi[i0][2 * l + 1] = (M00r * p0i + M00i * p0r) + (M01r * p1i + M01i * p1r);
+  psi[i0 + 1][2 * l] = (M10r * p0r - M10i * p0i) + (M11r * p1r - M11i * p1i);
+  psi[i0 + 1][2 * l + 1] = (M10r * p0i + M10i * p0r) + (M11r * p1i + M11i * p1r);
+}
+
+void orc_bragg_ds(const orc_grid *g, double **psi, const orc_rt *s, double t1, int mode1, int mode2) {
+  long long np = orc_npts(g);
+  const double dt = s->dt;
+#pragma omp parallel for schedule(static)
+  for (long long l = 0; l < np; l++) {
+    double x[3], phi[4];
+    orc_get_x(g, l, x);
+    for (int i = 0; i < 4; i++) {
+      phi[i] = 0;
+      for (int j = 0; j < 4; j++) {
+        const double re = psi[j][2 * l], im = psi[j][2 * l + 1];
+        phi[i] += s->gs[j + 4 * i] * (re * re + im * im);
+      }
+      phi[i] += (i < 2 ? s->beta0 : s->beta2) * x[0] - s->deltaL[i];
+    }
+    const double Om = mode1 * s->amp0 * cos(s->laser_k * x[0] - (s->laser_domh + s->chirp_rate0 * t1) * t1 + s->phase0 / 2);
+    const double Om2 =
+        mode2 * s->amp2 * cos(s->laser_k2 * x[0] - (s->laser_domh2 + s->chirp_rate2 * t1) * t1 + s->phase2 / 2);
+    const double th1 = -0.5 * (s->laser_dk * x[0] + s->phase0);
+    const double th2 = -0.5 * (s->laser_dk2 * x[0] + s->phase2);
+    if (Om == 0 && Om2 == 0) {
+      for (int i = 0; i < 4; i++) ds_diag(psi, l, i, phi[i], dt);
+    } else if (Om == 0.0) {
+      for (int i = 0; i < 2; i++) ds_diag(psi, l, i, phi[i], dt);
+      ds_pair(psi, l, 2, phi[2], phi[3], Om2, th2, dt);
+    } else if (Om2 == 0.0) {
+      for (int i = 2; i < 4; i++) ds_diag(psi, l, i, phi[i], dt);
+      ds_pair(psi, l, 0, phi[0], phi[1], Om, th1, dt);
+    } else {
+      ds_pair(psi, l, 0, phi[0], phi[1], Om, th1, dt);
+      ds_pair(psi, l, 2, phi[2], phi[3], Om2, th2, dt);
+    }
+  }
+}
+
 static void v_step(orc_rt *s, int kind) {
+  if (kind == ORC_FREEPROP_IF2) {
+    orc_nl_if2_step(&s->g, s->ncomp, s->psi, s->gs, s->dt, s->beta0, s->beta2, s->deltaL);
+    return;
+  }
+  if (kind == ORC_BRAGG_DS) {
+    /* pulse gates (bragg_ds.cpp:120-128): a pulse is off once t1 - seq.time
+     * exceeds its duration */
+    const double t1 = s->t + 0.5 * s->dt;
+    const double time = t1 - s->seq_time;
+    orc_bragg_ds(&s->g, s->psi, s, t1, time > s->duration[0] ? 0 : 1, time > s->duration[1] ? 0 : 1);
+    return;
+  }
   if (kind == ORC_FREEPROP) {
     orc_nl_step(&s->g, s->ncomp, s->psi, s->gs, s->dt, s->pot);
   } else if (kind == ORC_FREEPROP_IF) {
@@ -573,16 +706,23 @@ static void v_step(orc_rt *s, int kind) {
 
 /* CRT_Base::run_sequence hot loop (CRT_Base.h:776-786); the time
  * bookkeeping is Do_FT_Step_half: t += 0.5*dt (:391), full: t += dt (:361) */
+static void kin_step(orc_rt *s, int full) {
+  if (s->two_species)
+    orc_ft_step2(&s->g, s->ncomp, s->psi, full ? s->full_tab : s->half_tab, full ? s->full_tab2 : s->half_tab2);
+  else
+    orc_ft_step(&s->g, s->ncomp, s->psi, full ? s->full_tab : s->half_tab);
+}
+
 void orc_rt_block(orc_rt *s, int kind, int Nk) {
-  orc_ft_step(&s->g, s->ncomp, s->psi, s->half_tab);
+  kin_step(s, 0);
   s->t += 0.5 * s->dt;
   for (int j = 2; j <= Nk; j++) {
     v_step(s, kind);
-    orc_ft_step(&s->g, s->ncomp, s->psi, s->full_tab);
+    kin_step(s, 1);
     s->t += s->dt;
   }
   v_step(s, kind);
-  orc_ft_step(&s->g, s->ncomp, s->psi, s->half_tab);
+  kin_step(s, 0);
   s->t += 0.5 * s->dt;
 }
 

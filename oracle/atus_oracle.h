@@ -90,7 +90,7 @@ double orc_energy(const orc_grid *g, int ncomp, double **psi,
 /* ---- sequence block (CRT_Base::run_sequence / CRT_Base_IF::run_sequence) */
 typedef double (*orc_amp_fn)(double t, void *user);
 
-enum { ORC_FREEPROP = 0, ORC_FREEPROP_IF = 1, ORC_BRAGG_AD = 2 };
+enum { ORC_FREEPROP = 0, ORC_FREEPROP_IF = 1, ORC_BRAGG_AD = 2, ORC_FREEPROP_IF2 = 3, ORC_BRAGG_DS = 4 };
 
 typedef struct {
   orc_grid g;
@@ -110,7 +110,27 @@ typedef struct {
   double amp0, laser_k, laser_domh, laser_dk, chirp, chirp_rate0, phase0;
   orc_amp_fn amp;      /* NULL => Amplitude_at_time() == 1 */
   void *amp_user;
+  /* two species (CRT_Base_2 / CRT_Base_IF_2 / bragg_ds): components
+   * [ncomp/2, ncomp) are species 2 with their own kinetic tables and laser */
+  int two_species;
+  double alpha2[3];
+  double *half_tab2, *full_tab2;
+  double beta2;
+  double amp2, laser_k2, laser_domh2, laser_dk2, chirp_rate2, phase2;
+  double seq_time;     /* sequence_item::time, start of the pulses */
+  double duration[2];  /* sequence_item::duration[0..1] (pulse gating of bragg_ds) */
 } orc_rt;
+
+/* CRT_Base_2::Do_FT_Step_*: table 1 on components [0, ncomp/2), table 2 on
+ * the rest (CRT_Base_2.h:274-348) */
+void orc_ft_step2(const orc_grid *g, int ncomp, double **psi, const double *tab1, const double *tab2);
+/* CRT_Base_IF_2::Do_NL_Step (CRT_Base_IF_2.h:324-366): species 1 with beta,
+ * species 2 with beta2 */
+void orc_nl_if2_step(const orc_grid *g, int ncomp, double **psi, const double *gs, double dt, double beta,
+                     double beta2, const double *deltaL);
+/* bragg_ds Bragg_single::Do_Bragg_ad (bragg_ds.cpp:112-300), four components;
+ * t1 = Get_t() + dt/2, mode1/mode2 the pulse gates */
+void orc_bragg_ds(const orc_grid *g, double **psi, const orc_rt *s, double t1, int mode1, int mode2);
 
 /* one outer iteration of the hot loop: half, (V, full) x (Nk-1), V, half
  * (CRT_Base.h:776-786, CRT_Base_IF.h:779-788) */

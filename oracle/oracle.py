@@ -50,7 +50,12 @@ class RT(C.Structure):
                 ("laser_domh", C.c_double), ("laser_dk", C.c_double),
                 ("chirp", C.c_double), ("chirp_rate0", C.c_double),
                 ("phase0", C.c_double),
-                ("amp", AMP_FN), ("amp_user", C.c_void_p)]
+                ("amp", AMP_FN), ("amp_user", C.c_void_p),
+                ("two_species", C.c_int), ("alpha2", C.c_double * 3),
+                ("half_tab2", C.c_void_p), ("full_tab2", C.c_void_p),
+                ("beta2", C.c_double), ("amp2", C.c_double), ("laser_k2", C.c_double),
+                ("laser_domh2", C.c_double), ("laser_dk2", C.c_double), ("chirp_rate2", C.c_double),
+                ("phase2", C.c_double), ("seq_time", C.c_double), ("duration", C.c_double * 2)]
 
 
 class SOB(C.Structure):
@@ -65,7 +70,7 @@ class SOB(C.Structure):
                 ("renormalised", C.c_int)]
 
 
-FREEPROP, FREEPROP_IF, BRAGG_AD = 0, 1, 2
+FREEPROP, FREEPROP_IF, BRAGG_AD, FREEPROP_IF2, BRAGG_DS = 0, 1, 2, 3, 4
 
 _lib = None
 
@@ -87,6 +92,9 @@ def lib():
             "orc_nl_step": (None, [G, I, P, P, D, P]),
             "orc_nl_if_step": (None, [G, I, P, P, D, D, P]),
             "orc_bragg_ad": (None, [G, P, P, C.POINTER(BraggParams)]),
+            "orc_ft_step2": (None, [G, I, P, P, P]),
+            "orc_nl_if2_step": (None, [G, I, P, P, D, D, D, P]),
+            "orc_bragg_ds": (None, [G, P, C.POINTER(RT), D, I, I]),
             "orc_particle_number": (D, [G, P]),
             "orc_expval_x": (None, [G, P, P]), "orc_expval_k": (None, [G, P, P]),
             "orc_rabi_integrals": (None, [G, P, P, I, D, P]),
@@ -299,7 +307,11 @@ def energy(g, psis, gs, alpha, pots=None):
 class RTSolver:
     """Owns the arrays of an orc_rt and runs reference blocks on them."""
 
-    def __init__(self, g, psis, gs, alpha, dt, t=0.0, pots=None, amp=None, **ifp):
+    def __init__(self, g, psis, gs, alpha, dt, t=0.0, pots=None, amp=None, alpha2=None, species2=None, **ifp):
+        """alpha2: kinetic coefficients of species 2 (components [n/2, n),
+        CRT_Base_2); species2: dict of beta2, amp2, laser_k2, laser_domh2,
+        laser_dk2, chirp_rate2, phase2, seq_time, duration (CRT_Base_IF_2,
+        bragg_ds)."""
         self.g = g
         self.psis = [_c128(p).copy() for p in psis]
         self.pots = None
@@ -335,6 +347,21 @@ class RTSolver:
             raise TypeError(f"unknown parameters {sorted(ifp)}")
         self._amp_cb = AMP_FN(lambda t, u: float(amp(t))) if amp else AMP_FN()
         s.amp = self._amp_cb
+        if alpha2 is not None:
+            s.two_species = 1
+            for a in range(len(alpha2)):
+                s.alpha2[a] = float(alpha2[a])
+            self.half2 = np.empty(shape, np.complex128)
+            self.full2 = np.empty(shape, np.complex128)
+            s.half_tab2 = self.half2.ctypes.data
+            s.full_tab2 = self.full2.ctypes.data
+        sp2 = dict(species2 or {})
+        dur = sp2.pop("duration", (0.0, 0.0))
+        s.duration[0], s.duration[1] = float(dur[0]), float(dur[1])
+        for k in ("beta2", "amp2", "laser_k2", "laser_domh2", "laser_dk2", "chirp_rate2", "phase2", "seq_time"):
+            setattr(s, k, float(sp2.pop(k, 0.0)))
+        if sp2:
+            raise TypeError(f"unknown species-2 parameters {sorted(sp2)}")
         self.s = s
         lib().orc_rt_set_dt(C.byref(s), float(dt))
 
@@ -393,3 +420,27 @@ class SobSolver:
         out = np.zeros(self.s.nwf)
         lib().orc_sob_mu(C.byref(self.s), _ptr(out))
         return out
+
+
+def ft_step2(g, psis, tab1, tab2):
+    """CRT_Base_2 kinetic step: tab1 on components [0, n/2), tab2 on the rest."""
+    out = [_c128(p).copy() for p in psis]
+    lib().orc_ft_step2(C.byref(g), len(out), _ptr_array(out), _ptr(_c128(tab1)), _ptr(_c128(tab2)))
+    return out
+
+
+def nl_if2_step(g, psis, gs, dt, beta, beta2, deltaL):
+    out = [_c128(p).copy() for p in psis]
+    gsa = np.ascontiguousarray(np.ravel(gs), dtype=np.float64)
+    dl = np.ascontiguousarray(deltaL, dtype=np.float64)
+    lib().orc_nl_if2_step(C.byref(g), len(out), _ptr_array(out), _ptr(gsa), float(dt), float(beta), float(beta2),
+                          _ptr(dl))
+    return out
+
+
+def bragg_ds(g, psis, solver, t1, mode1, mode2):
+    """One two-species Bragg step (bragg_ds) with the parameters of an
+    RTSolver built with species2=...; returns new arrays."""
+    out = [_c128(p).copy() for p in psis]
+    lib().orc_bragg_ds(C.byref(g), _ptr_array(out), C.byref(solver.s), float(t1), int(mode1), int(mode2))
+    return out

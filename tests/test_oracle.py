@@ -406,3 +406,113 @@ def test_sobolev_projected_gradient_orthogonal(orc):
     # (CSOB_Base.h:340-367): <psi, g - (<psi,g>/<psi,psi_sob>) psi_sob> = 0
     v = np.sum(psi_before.real * S.grad[0].real + psi_before.imag * S.grad[0].imag)
     assert abs(v) < 1e-12 * np.sum(np.abs(S.grad[0]) ** 2) ** 0.5 * np.sum(np.abs(psi_before) ** 2) ** 0.5 + 1e-14
+
+
+# ------------------------------------------------ two species (CRT_Base_2, bragg_ds)
+
+def test_ft_step2_applies_the_species_tables(orc):
+    # plane waves e^{i k x}: species 1 picks up exp(-i dt a1 k^2), species 2 exp(-i dt a2 k^2)
+    n, L = 64, 20.0
+    g = orc.make_grid((n,), L=L)
+    (x,) = orc.coords_x(g)
+    k = 5 * 2 * np.pi / L
+    psis = [np.exp(1j * k * x) * (1 + 0.1 * c) for c in range(4)]
+    dt, a1, a2 = 0.013, 0.5, 0.21
+    out = orc.ft_step2(g, psis, orc.kinetic_table(g, [a1], dt, 1.0), orc.kinetic_table(g, [a2], dt, 1.0))
+    for c in range(4):
+        a = a1 if c < 2 else a2
+        assert np.abs(out[c] - psis[c] * np.exp(-1j * dt * a * k * k)).max() < 1e-13
+
+
+def test_nl_if2_step_per_species_gravity(orc):
+    rng = np.random.default_rng(7)
+    g = orc.make_grid((16, 4), L=[8.0, 2.0])
+    psis = [rng.standard_normal((16, 4)) + 1j * rng.standard_normal((16, 4)) for _ in range(4)]
+    gs = rng.uniform(0.1, 2.0, 16)
+    dt, b1, b2, dL = 0.02, 0.3, -0.7, [0.1, -2.0, 0.5, 1.5]
+    out = orc.nl_if2_step(g, psis, gs, dt, b1, b2, dL)
+    x = orc.coords_x(g)[0]
+    n = [np.abs(p) ** 2 for p in psis]
+    for i in range(4):
+        phi = sum(gs[j + 4 * i] * n[j] for j in range(4)) + (b1 if i < 2 else b2) * x - dL[i]
+        assert np.abs(out[i] - psis[i] * np.exp(-1j * dt * phi)).max() < 1e-13
+
+
+def _ds_solver(orc, g, psis, gs, sp2=None, **sp1):
+    return orc.RTSolver(g, psis, gs, [0.5] * g.dim, dt=0.05, alpha2=[0.3] * g.dim, species2=sp2, **sp1)
+
+
+def _ds_expected(orc, g, psis, gs, S, t1, m1, m2):
+    """Per point: phi_i from all four densities; each species the
+    exponential expm(-i dt [[V11, Om eta], [Om conj(eta), V22]]) with
+    eta = exp(-i (laser_dk x + phase)/2), or diagonal phases when its Omega
+    is 0."""
+    s = S.s
+    x = orc.coords_x(g)[0] * np.ones(psis[0].shape)
+    n = [np.abs(p) ** 2 for p in psis]
+    phi = [sum(gs[j + 4 * i] * n[j] for j in range(4)) + (s.beta0 if i < 2 else s.beta2) * x - s.deltaL[i]
+           for i in range(4)]
+    Om = m1 * s.amp0 * np.cos(s.laser_k * x - (s.laser_domh + s.chirp_rate0 * t1) * t1 + s.phase0 / 2)
+    Om2 = m2 * s.amp2 * np.cos(s.laser_k2 * x - (s.laser_domh2 + s.chirp_rate2 * t1) * t1 + s.phase2 / 2)
+    eta = np.exp(-0.5j * (s.laser_dk * x + s.phase0))
+    eta2 = np.exp(-0.5j * (s.laser_dk2 * x + s.phase2))
+    out = [p.copy() for p in psis]
+    for idx in np.ndindex(x.shape):
+        for i0, om, et in ((0, Om[idx], eta[idx]), (2, Om2[idx], eta2[idx])):
+            if om == 0:
+                for i in (i0, i0 + 1):
+                    out[i][idx] = psis[i][idx] * np.exp(-1j * s.dt * phi[i][idx])
+                continue
+            H = np.array([[phi[i0][idx], om * et], [om * np.conj(et), phi[i0 + 1][idx]]])
+            v = scipy.linalg.expm(-1j * s.dt * H) @ np.array([psis[i0][idx], psis[i0 + 1][idx]])
+            out[i0][idx], out[i0 + 1][idx] = v
+    return out
+
+
+@pytest.mark.parametrize("modes", [(1, 1), (0, 1), (1, 0), (0, 0)])
+def test_bragg_ds_matches_expm_per_species(orc, modes):
+    rng = np.random.default_rng(11)
+    shape = (20, 3)
+    g = orc.make_grid(shape, L=[10.0, 3.0])
+    psis = [rng.standard_normal(shape) + 1j * rng.standard_normal(shape) for _ in range(4)]
+    gs = rng.uniform(0.1, 1.5, 16)
+    S = _ds_solver(orc, g, psis, gs,
+                   sp2=dict(beta2=-0.04, amp2=1.9, laser_k2=2.3, laser_domh2=0.7, laser_dk2=0.35, chirp_rate2=0.15,
+                            phase2=0.9),
+                   beta0=0.05, deltaL=[0.2, -1.2, 0.4, -0.8], amp0=2.5, laser_k=1.7, laser_domh=0.9, laser_dk=0.6,
+                   chirp_rate0=0.1, phase0=0.4)
+    t1 = 0.37
+    out = orc.bragg_ds(g, psis, S, t1, *modes)
+    exp = _ds_expected(orc, g, psis, gs, S, t1, *modes)
+    for c in range(4):
+        assert np.abs(out[c] - exp[c]).max() < 1e-12, (modes, c)
+    # each species keeps its population point by point
+    for i0 in (0, 2):
+        n_in = np.abs(psis[i0]) ** 2 + np.abs(psis[i0 + 1]) ** 2
+        n_out = np.abs(out[i0]) ** 2 + np.abs(out[i0 + 1]) ** 2
+        assert np.abs(n_out - n_in).max() < 1e-12 * n_in.max()
+
+
+def test_bragg_ds_block_with_an_empty_second_species_is_single_bragg(orc):
+    # species 2 empty and uncoupled, alpha2 = alpha: species 1 follows the
+    # single-species bragg_ad block (no chirp, AMP_T = 1, phase 0, DeltaL_0 = 0)
+    n, L = 64, 24.0
+    g = orc.make_grid((n,), L=L)
+    (x,) = orc.coords_x(g)
+    p1 = np.exp(-x ** 2 / 8).astype(np.complex128)
+    p1 /= np.sqrt(orc.particle_number(g, p1))
+    z = np.zeros_like(p1)
+    g2 = [3.0, 2.5, 2.5, 3.1]
+    g4 = np.zeros(16)
+    g4[0], g4[1], g4[4], g4[5] = g2
+    lp = dict(beta0=0.02, amp0=4.0, laser_k=2 * 2 * np.pi / L * 3, laser_domh=1.5, laser_dk=0.2, chirp_rate0=0.05)
+    ds = orc.RTSolver(g, [p1, z, z, z], g4, [0.5], dt=2e-3, alpha2=[0.5], deltaL=[0.0, -0.3, 0.0, 0.0],
+                      species2=dict(amp2=3.0, laser_k2=1.0, duration=(10.0, 10.0)), **lp)
+    single = orc.RTSolver(g, [p1, z], g2, [0.5], dt=2e-3, deltaL=[0.0, -0.3], **lp)
+    for _ in range(3):
+        ds.block(orc.BRAGG_DS, 20)
+        single.block(orc.BRAGG_AD, 20)
+    assert ds.t == single.t
+    err = np.linalg.norm(np.concatenate([ds.psis[0] - single.psis[0], ds.psis[1] - single.psis[1]]))
+    assert err / np.linalg.norm(np.concatenate(single.psis)) < 1e-12
+    assert not np.any(ds.psis[2]) and not np.any(ds.psis[3])
