@@ -862,24 +862,23 @@ int atus_run_block(atus_ctx* c, const atus_step_params* sp, int Nk, double* t) {
   return finish_block(c);
 }
 
-int atus_run_block_ds(atus_ctx* c, const atus_step_params* sp, const atus_ds_params* ds, int Nk, double* t) {
-  ATUS_TRY(check_ctx(c));
-  ATUS_TRY(check_single(c, "atus_run_block_ds"));
-  if (!sp || !ds || !t) return fail(ATUS_ERR_ARG, "null argument");
-  if (Nk < 1) return fail(ATUS_ERR_ARG, "Nk must be >= 1");
+}  // extern "C"
+
+namespace {
+
+// the V-step kind and the species-independent arguments of a two-species
+// step; the per-step t1, F and pulse gates are set by the caller
+int ds_vargs(atus_ctx* c, int kind, double dt, const atus_laser* laser1, const atus_ds_params* ds, int* vk, VArgs& base) {
   if (c->ncomp < 2 || c->ncomp % 2) return fail(ATUS_ERR_STATE, "two species need an even number of components");
-  int vk;
-  if (sp->kind == ATUS_STEP_FREEPROP_IF) {
-    vk = V_NL_IF2;
-  } else if (sp->kind == ATUS_STEP_BRAGG_AD) {
+  if (kind == ATUS_STEP_FREEPROP_IF) {
+    *vk = V_NL_IF2;
+  } else if (kind == ATUS_STEP_BRAGG_AD) {
     if (c->ncomp != 4) return fail(ATUS_ERR_STATE, "the two-species Bragg step needs four components");
-    vk = V_BRAGG_DS;
+    *vk = V_BRAGG_DS;
   } else {
-    return fail(ATUS_ERR_ARG, "two-species blocks run freeprop (IF) or bragg_ad steps");
+    return fail(ATUS_ERR_ARG, "two-species steps are freeprop (IF) or bragg_ad steps");
   }
-  ATUS_TRY(ensure_kin(c, sp->dt));
-  VArgs base;
-  fill_vargs(c, base, sp->dt, &sp->laser);
+  fill_vargs(c, base, dt, laser1);
   const atus_laser& L2 = ds->laser2;
   base.amp2 = L2.amp0;
   base.laser_k2 = L2.laser_k;
@@ -888,11 +887,46 @@ int atus_run_block_ds(atus_ctx* c, const atus_step_params* sp, const atus_ds_par
   base.chirp_rate2 = L2.chirp_rate0;
   base.phase2 = L2.phase0;
   base.beta2 = L2.beta0;
+  return ATUS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int atus_run_block_ds(atus_ctx* c, const atus_step_params* sp, const atus_ds_params* ds, int Nk, double* t) {
+  ATUS_TRY(check_ctx(c));
+  ATUS_TRY(check_single(c, "atus_run_block_ds"));
+  if (!sp || !ds || !t) return fail(ATUS_ERR_ARG, "null argument");
+  if (Nk < 1) return fail(ATUS_ERR_ARG, "Nk must be >= 1");
+  int vk;
+  VArgs base;
+  ATUS_TRY(ds_vargs(c, sp->kind, sp->dt, &sp->laser, ds, &vk, base));
+  ATUS_TRY(ensure_kin(c, sp->dt));
   if (c->timing) HIP_TRY(hipEventRecord(c->ev0, c->stream));
   StepFeed fd;
   fd.ds = ds;
   ATUS_TRY(single_block(c, vk, base, Nk, sp->dt, t, fd));
   return finish_block(c);
+}
+
+int atus_v_step_ds(atus_ctx* c, int kind, double dt, double t, const atus_laser* laser1, const atus_ds_params* ds) {
+  ATUS_TRY(check_ctx(c));
+  ATUS_TRY(check_single(c, "atus_v_step_ds"));
+  if (!laser1 || !ds) return fail(ATUS_ERR_ARG, "null argument");
+  int vk;
+  VArgs va;
+  ATUS_TRY(ds_vargs(c, kind, dt, laser1, ds, &vk, va));
+  // bragg_ds.cpp:116-128: t1 = t + dt/2, a pulse is off once t1 - seq.time
+  // exceeds its duration
+  va.t1 = t + 0.5 * dt;
+  va.F = 1.0;
+  const double time = va.t1 - ds->seq_time;
+  va.mode1 = time > ds->duration[0] ? 0.0 : 1.0;
+  va.mode2 = time > ds->duration[1] ? 0.0 : 1.0;
+  HIP_TRY(launch_point(vk, c->ncomp, c->psi, c->grid(), va, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return ATUS_OK;
 }
 
 int atus_particle_number(atus_ctx* c, int comp, double* out) {

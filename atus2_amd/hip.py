@@ -25,7 +25,7 @@ EXPORTED = [
     "atus_sob_mu", "atus_set_timing", "atus_last_block_ms", "atus_run_block_profiled", "atus_copy_bandwidth", "atus_tune",
     "atus_batch_create", "atus_batch_set_runs", "atus_batch_run_block", "atus_batch_particle_numbers",
     "atus_batch_rabi_integrals", "atus_sob_run", "atus_graph_create", "atus_graph_launch", "atus_graph_destroy",
-    "atus_set_alpha2", "atus_run_block_ds",
+    "atus_set_alpha2", "atus_run_block_ds", "atus_v_step_ds",
     "atus_net_create", "atus_net_destroy", "atus_net_set_precision", "atus_net_param_count", "atus_net_get_params", "atus_net_set_params",
     "atus_net_forward", "atus_net_gradients", "atus_net_set_data", "atus_net_train", "atus_net_warm_start",
 ]
@@ -118,6 +118,7 @@ def load():
         "atus_graph_destroy": (I, [P]),
         "atus_set_alpha2": (I, [P, C.POINTER(D)]),
         "atus_run_block_ds": (I, [P, C.POINTER(StepParams), C.POINTER(DsParams), I, C.POINTER(D)]),
+        "atus_v_step_ds": (I, [P, I, D, D, C.POINTER(Laser), C.POINTER(DsParams)]),
         "atus_net_create": (I, [pctx, I, I, I, I, I, I, C.c_ulonglong]),
         "atus_net_destroy": (I, [P]),
         "atus_net_set_precision": (I, [P, I]),
@@ -344,6 +345,14 @@ class Context:
         tt = C.c_double(t)
         _check(_lib.atus_run_block_ds(self._h, C.byref(sp), C.byref(ds), int(Nk), C.byref(tt)))
         return tt.value
+
+    def v_step_ds(self, kind, dt, t, las: Laser, las2: Laser, seq_time=0.0, duration=(1e300, 1e300)):
+        """One two-species V step at t1 = t + dt/2, no kinetic step."""
+        ds = DsParams()
+        ds.laser2 = las2
+        ds.seq_time = float(seq_time)
+        ds.duration[0], ds.duration[1] = float(duration[0]), float(duration[1])
+        _check(_lib.atus_v_step_ds(self._h, int(kind), float(dt), float(t), C.byref(las), C.byref(ds)))
 
     def sob_run(self, eps, maxit=0):
         """CSOB_Base::run: iterations until res_tot <= eps (maxit > 0 caps

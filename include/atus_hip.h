@@ -137,6 +137,11 @@ typedef struct {
  * components); sp->laser is species 1 (bragg_ds reads neither its chirp
  * nor AMP_T); the kinetic steps use alpha and alpha2 per species */
 int atus_run_block_ds(atus_ctx *ctx, const atus_step_params *sp, const atus_ds_params *ds, int Nk, double *t);
+/* one V step of a two-species run, no kinetic step: kind
+ * ATUS_STEP_FREEPROP_IF (CRT_Base_IF_2::Do_NL_Step, CRT_Base_IF_2.h:324-366)
+ * or ATUS_STEP_BRAGG_AD (Bragg_single::Do_Bragg_ad, bragg_ds.cpp:112-300)
+ * at t1 = t + dt/2, pulse gates from ds; laser1 as sp->laser above */
+int atus_v_step_ds(atus_ctx *ctx, int kind, double dt, double t, const atus_laser *laser1, const atus_ds_params *ds);
 
 /* ---- reductions (CRT_Base.h:481-599, CRT_Base_IF.h:208-281) */
 /* Get_Particle_Number: ar * sum |psi_comp|^2 */

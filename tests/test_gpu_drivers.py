@@ -130,3 +130,70 @@ def test_sobmin_3d_ground_state(orc, tmp_path):
     ref = S.psi[0] * np.sqrt(2.0)             # Save(..., rescale = true) multiplies by sqrt(N)
     assert rel_l2(out.reshape(n, n, n), ref) < 1e-9
     assert res < 1e-18 or iters == 3000
+
+
+def test_bragg_ds_two_species_pulses_then_free_flight(orc, tmp_path):
+    # bragg_ds: four components (two species of two states), pulses of
+    # different lengths, then the two-species free step; .bin outputs and
+    # both species' Rabi files against the oracle from the same inputs
+    n, L, dt = 2048, 80.0, 1e-3
+    g = orc.make_grid((n,), L=L)
+    (x,) = orc.coords_x(g)
+    rng = np.random.default_rng(8)
+    psis = []
+    for c, (w, amp) in enumerate([(6.0, 1.0), (6.0, 0.0), (4.0, 1.0), (4.0, 0.0)]):
+        p = (amp * np.exp(-x ** 2 / (2 * w ** 2)) + 1e-3 * (rng.standard_normal(n) + 1j * rng.standard_normal(n)))
+        p = p.astype(np.complex128)
+        p /= np.sqrt(orc.particle_number(g, p)) if amp else 10.0
+        psis.append(p)
+        write_bin(tmp_path / f"p{c + 1}.bin", p, [L], dt=dt)
+    gs = [2.0, 1.9, 0.5, 0.4, 1.9, 2.0, 0.4, 0.5, 0.5, 0.4, 3.0, 2.8, 0.4, 0.5, 2.8, 3.0]
+    gsx = "".join(f"<GS_{i + 1}>{','.join(str(v) for v in gs[4 * i:4 * i + 4])}</GS_{i + 1}>" for i in range(4))
+    (tmp_path / "ds.xml").write_text(f"""<SIMULATION><DIM>1</DIM>
+      <FILENAME>p1.bin</FILENAME><FILENAME_2>p2.bin</FILENAME_2>
+      <FILENAME_3>p3.bin</FILENAME_3><FILENAME_4>p4.bin</FILENAME_4>
+      <CONSTANTS><laser_k>1.0</laser_k><laser_k_2>0.75</laser_k_2><laser_domh>0.5</laser_domh>
+        <laser_domh_2>0.28</laser_domh_2><laser_dk>0.01</laser_dk><laser_dk_2>0.02</laser_dk_2>
+        <rabi_threshold>0.3</rabi_threshold></CONSTANTS>
+      <VCONSTANTS>{gsx}<Alpha_1>0.5</Alpha_1><Alpha_2>0.35</Alpha_2>
+        <Beta>0.01</Beta><Beta_2>-0.02</Beta_2><Delta_L>0,0.3,0.1,-0.2</Delta_L>
+        <Amp_1>5,5</Amp_1><Amp_2>3.5,3.5</Amp_2></VCONSTANTS>
+      <ALGORITHM><DT>1e-3</DT></ALGORITHM>
+      <SEQUENCE>
+        <bragg_ad dt="1e-3" Nk="50" rabi_output_freq="each" output_freq="last" pn_freq="last">0.2,0.13025</bragg_ad>
+        <freeprop dt="1e-3" Nk="25" output_freq="last">0.05</freeprop>
+      </SEQUENCE></SIMULATION>""")
+    r = subprocess.run([tool("bragg_ds"), "ds.xml"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    # pn_freq defaults to "last": four lines after each sequence
+    assert r.stdout.count("N[") == 8 and "FYI: Na          : 4" in r.stdout
+    sp2 = dict(beta2=-0.02, amp2=3.5, laser_k2=0.75, laser_domh2=0.28, laser_dk2=0.02, seq_time=0.0,
+               duration=(0.2, 0.13025))
+    l1 = dict(beta0=0.01, deltaL=[0.0, 0.3, 0.1, -0.2], amp0=5.0, laser_k=1.0, laser_domh=0.5, laser_dk=0.01)
+    S = orc.RTSolver(g, psis, gs, [0.5], dt=dt, alpha2=[0.35], species2=sp2, **l1)
+    # the noise floor of the reference formula near its Omega = 0 branches
+    pert = [p * (1 + 2e-16 * np.random.default_rng(99).standard_normal(n)) for p in psis]
+    Sp = orc.RTSolver(g, pert, gs, [0.5], dt=dt, alpha2=[0.35], species2=sp2, **l1)
+    rabi1, rabi2 = [], []
+    for _ in range(4):
+        S.block(orc.BRAGG_DS, 50)
+        Sp.block(orc.BRAGG_DS, 50)
+        rabi1.append(orc.rabi_integrals(g, S.psis[0], [(0.0,), (2.0,), (-2.0,)], 0.3)[0])
+        rabi2.append(orc.rabi_integrals(g, S.psis[2], [(0.0,), (1.5,), (-1.5,)], 0.3)[0])
+    tol = max(1e-11, 20 * rel_l2(np.concatenate(Sp.psis), np.concatenate(S.psis)))
+    out = np.concatenate([read_bin(tmp_path / f"0.200_{c}.bin")[1] for c in range(1, 5)])
+    assert read_bin(tmp_path / "0.200_1.bin")[0]["t"] == pytest.approx(0.2, abs=1e-12)
+    assert rel_l2(out, np.concatenate(S.psis)) < tol
+    for name, ref, ks in (("Spezie_1_Rabi_1_0.txt", rabi1, "0\t2\t-2"), ("Spezie_2_Rabi_1_0.txt", rabi2, "0\t1.5\t-1.5")):
+        lines = (tmp_path / name).read_text().splitlines()
+        assert lines[0].startswith("# time \t" + ks), lines[0]
+        rows = [[float(v) for v in ln.split("\t") if v] for ln in lines[1:]]
+        assert [r_[0] for r_ in rows] == pytest.approx([0.0, 0.05, 0.1, 0.15])
+        for r_, rf in zip(rows, ref):
+            assert np.allclose(r_[1:], rf, rtol=1e-5, atol=1e-9)
+    # the free flight of both species from the end of the pulses
+    S.s.seq_time = S.t
+    for _ in range(2):
+        S.block(orc.FREEPROP_IF2, 25)
+    out = np.concatenate([read_bin(tmp_path / f"0.250_{c}.bin")[1] for c in range(1, 5)])
+    assert rel_l2(out, np.concatenate(S.psis)) < tol + 1e-11

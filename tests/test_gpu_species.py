@@ -124,3 +124,49 @@ def test_two_species_rejections(hip):
     with hip.Context((8192,), [0.01], [2 * np.pi / 81.92], 2) as ctx:
         with pytest.raises(hip.AtusError, match="not covered"):
             ctx.set_alpha2([0.3])
+
+
+@pytest.mark.parametrize("shape", [(512,), (32, 16, 16)])
+@pytest.mark.parametrize("kind", ["freeprop_if", "bragg_ds"])
+def test_single_v_steps_compose_to_the_block(hip, orc, shape, kind):
+    # half, (V, full) x (Nk-1), V, half from single device steps, as the
+    # host templates call them when the kinetic steps are replaced
+    dim = len(shape)
+    L = [40.0, 30.0, 36.0][:dim]
+    g = orc.make_grid(shape, L=L)
+    psis = _states(orc, g, 4, 23)
+    gs = list(np.random.default_rng(4).uniform(5.0, 60.0, 16))
+    a1, a2 = [0.5] * dim, [0.35] * dim
+    _, _, las, las2 = _laser_pair(hip, L[0])
+    dt, Nk, t0 = 2e-3, 6, 0.1
+    seq_time, duration = 0.05, (1.0, 0.05 + 3.5 * dt)
+    hk = hip.STEP_FREEPROP_IF if kind == "freeprop_if" else hip.STEP_BRAGG_AD
+    outs = []
+    for fused in (True, False):
+        with hip.Context.from_grid(g, 4) as ctx:
+            for c in range(4):
+                ctx.upload(c, psis[c])
+            ctx.set_g(gs)
+            ctx.set_alpha(a1)
+            ctx.set_alpha2(a2)
+            if fused:
+                t = ctx.run_block_ds(hk, dt, Nk, t0, las, las2, seq_time, duration)
+            else:
+                t = t0
+                ctx.kinetic(0.5, dt)
+                t += 0.5 * dt
+                for m in range(1, Nk + 1):
+                    ctx.v_step_ds(hk, dt, t, las, las2, seq_time, duration)
+                    ctx.kinetic(1.0 if m < Nk else 0.5, dt)
+                    t += dt if m < Nk else 0.5 * dt
+            outs.append(([ctx.download(c) for c in range(4)], t))
+    assert outs[0][1] == outs[1][1]
+    assert rel_l2(outs[1][0], outs[0][0]) <= 1e-15
+
+
+def test_single_v_step_rejections(hip):
+    with hip.Context((64,), [0.5], [2 * np.pi / 32], 2) as ctx:
+        with pytest.raises(hip.AtusError, match="four components"):
+            ctx.v_step_ds(hip.STEP_BRAGG_AD, 1e-3, 0.0, hip.laser(), hip.laser())
+        with pytest.raises(hip.AtusError, match="freeprop"):
+            ctx.v_step_ds(hip.STEP_FREEPROP, 1e-3, 0.0, hip.laser(), hip.laser())
