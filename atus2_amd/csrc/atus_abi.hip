@@ -494,6 +494,7 @@ int atus_ctx_destroy(atus_ctx* c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   (void)hipFree(c->psi);
+  for (double2* p : c->snap) (void)hipFree(p);
   (void)hipFree(c->pot);
   (void)hipFree(c->scratch);
   for (auto& kv : c->tw) (void)hipFree(kv.second);
@@ -539,6 +540,43 @@ int atus_download(atus_ctx* c, int comp, double* host) {
   if (!host) return fail(ATUS_ERR_ARG, "null host pointer");
   HIP_TRY(hipMemcpyAsync(host, c->psi + comp * c->npts, sizeof(double2) * c->npts, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
+  return ATUS_OK;
+}
+
+int atus_snapshot_save(atus_ctx* c, int slot) {
+  ATUS_TRY(check_ctx(c));
+  if (slot < 0 || slot >= 4) return fail(ATUS_ERR_ARG, "snapshot slot must be 0..3");
+  const size_t bytes = sizeof(double2) * c->npts * c->ncomp;
+  if (!c->snap[slot] && hipMalloc(&c->snap[slot], bytes) != hipSuccess) {
+    c->snap[slot] = nullptr;
+    return fail(ATUS_ERR_NOMEM, "snapshot allocation");
+  }
+  HIP_TRY(hipMemcpyAsync(c->snap[slot], c->psi, bytes, hipMemcpyDeviceToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return ATUS_OK;
+}
+
+int atus_snapshot_restore(atus_ctx* c, int slot) {
+  ATUS_TRY(check_ctx(c));
+  if (slot < 0 || slot >= 4) return fail(ATUS_ERR_ARG, "snapshot slot must be 0..3");
+  if (!c->snap[slot]) return fail(ATUS_ERR_STATE, "snapshot slot " + std::to_string(slot) + " holds no snapshot");
+  HIP_TRY(hipMemcpyAsync(c->psi, c->snap[slot], sizeof(double2) * c->npts * c->ncomp, hipMemcpyDeviceToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return ATUS_OK;
+}
+
+int atus_copy_components(atus_ctx* dst, int dst_comp, atus_ctx* src, int src_comp, int n) {
+  ATUS_TRY(check_ctx(dst));
+  ATUS_TRY(check_ctx(src));
+  if (dst->device != src->device) return fail(ATUS_ERR_ARG, "contexts on different devices");
+  if (dst->npts != src->npts) return fail(ATUS_ERR_ARG, "contexts of different grid sizes");
+  if (n < 1 || src_comp < 0 || dst_comp < 0 || src_comp + n > src->ncomp || dst_comp + n > dst->ncomp)
+    return fail(ATUS_ERR_ARG, "component range out of bounds");
+  // the source's pending work first; the copy runs on the destination's stream
+  HIP_TRY(hipStreamSynchronize(src->stream));
+  HIP_TRY(hipMemcpyAsync(dst->psi + (size_t)dst_comp * dst->npts, src->psi + (size_t)src_comp * src->npts,
+                         sizeof(double2) * src->npts * n, hipMemcpyDeviceToDevice, dst->stream));
+  HIP_TRY(hipStreamSynchronize(dst->stream));
   return ATUS_OK;
 }
 

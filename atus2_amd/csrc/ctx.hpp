@@ -55,6 +55,7 @@ struct atus_ctx {
   size_t max_lds = 65536;
 
   double2* psi = nullptr;   // ncomp * npts
+  double2* snap[4] = {nullptr, nullptr, nullptr, nullptr};  // device snapshots of psi
   double* pot = nullptr;    // ncomp * npts, or null
   double2* scratch = nullptr;
   std::map<int, double2*> tw;

@@ -26,6 +26,7 @@ EXPORTED = [
     "atus_batch_create", "atus_batch_set_runs", "atus_batch_run_block", "atus_batch_particle_numbers",
     "atus_batch_rabi_integrals", "atus_sob_run", "atus_graph_create", "atus_graph_launch", "atus_graph_destroy",
     "atus_set_alpha2", "atus_run_block_ds", "atus_v_step_ds",
+    "atus_snapshot_save", "atus_snapshot_restore", "atus_copy_components",
     "atus_net_create", "atus_net_destroy", "atus_net_set_precision", "atus_net_param_count", "atus_net_get_params", "atus_net_set_params",
     "atus_net_forward", "atus_net_gradients", "atus_net_set_data", "atus_net_train", "atus_net_warm_start",
 ]
@@ -119,6 +120,9 @@ def load():
         "atus_set_alpha2": (I, [P, C.POINTER(D)]),
         "atus_run_block_ds": (I, [P, C.POINTER(StepParams), C.POINTER(DsParams), I, C.POINTER(D)]),
         "atus_v_step_ds": (I, [P, I, D, D, C.POINTER(Laser), C.POINTER(DsParams)]),
+        "atus_snapshot_save": (I, [P, I]),
+        "atus_snapshot_restore": (I, [P, I]),
+        "atus_copy_components": (I, [P, I, P, I, I]),
         "atus_net_create": (I, [pctx, I, I, I, I, I, I, C.c_ulonglong]),
         "atus_net_destroy": (I, [P]),
         "atus_net_set_precision": (I, [P, I]),
@@ -211,6 +215,16 @@ class Context:
         a = np.empty(self.shape, np.complex128)
         _check(_lib.atus_download(self._h, comp, _dp(a)))
         return a
+
+    def snapshot_save(self, slot=0):
+        _check(_lib.atus_snapshot_save(self._h, int(slot)))
+
+    def snapshot_restore(self, slot=0):
+        _check(_lib.atus_snapshot_restore(self._h, int(slot)))
+
+    def copy_components(self, dst_comp, src, src_comp, n):
+        """Device copy of n components of context `src` into this one."""
+        _check(_lib.atus_copy_components(self._h, int(dst_comp), src._h, int(src_comp), int(n)))
 
     def set_potential(self, comp, V):
         if V is None:

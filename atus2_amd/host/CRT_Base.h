@@ -79,6 +79,9 @@ class CRT_Base : public CRT_shared {
 
   std::map<std::string, StepFunction> m_map_stepfcts;
   StepFunction m_custom_fct;
+  // writes per file name by Save / Save_Phi / Append_Phi: a restart file
+  // whose count has not changed since it was written still holds that state
+  std::map<std::string, long long> m_file_writes;
 
   // --- device side
   std::shared_ptr<atus_host::FieldSet> m_set;
@@ -272,9 +275,10 @@ template <class T, int dim, int no_int_states>
 void CRT_Base<T, dim, no_int_states>::Save_Phi(std::string filename, const int comp) {
   if (comp < 0 || comp >= no_int_states)
     throw std::string("Error in " + std::string(__func__) + ": comp out of bounds\n");
+  ++m_file_writes[filename];
   std::ofstream f(filename, std::ofstream::binary);
   f.write(reinterpret_cast<char*>(&m_header), sizeof(generic_header));
-  f.write(reinterpret_cast<char*>(m_fields[comp]->Getp2In()), (size_t)m_no_of_pts * sizeof(fftw_complex));
+  f.write(reinterpret_cast<const char*>(m_set->host_view(comp)), (size_t)m_no_of_pts * sizeof(fftw_complex));
 }
 
 // Append_Phi (CRT_Base.h:626-641)
@@ -282,14 +286,16 @@ template <class T, int dim, int no_int_states>
 void CRT_Base<T, dim, no_int_states>::Append_Phi(std::string filename, const int comp) {
   if (comp < 0 || comp >= no_int_states)
     throw std::string("Error in " + std::string(__func__) + ": comp out of bounds\n");
+  ++m_file_writes[filename];
   std::ofstream f(filename, std::ofstream::binary | std::ofstream::app);
   if (f.fail()) throw std::string("File " + filename + " could not be opened.");
   f.write(reinterpret_cast<char*>(&m_header), sizeof(generic_header));
-  f.write(reinterpret_cast<char*>(m_fields[comp]->Getp2In()), (size_t)m_no_of_pts * sizeof(fftw_complex));
+  f.write(reinterpret_cast<const char*>(m_set->host_view(comp)), (size_t)m_no_of_pts * sizeof(fftw_complex));
 }
 
 template <class T, int dim, int no_int_states>
 void CRT_Base<T, dim, no_int_states>::Save(fftw_complex* data, std::string filename) {
+  ++m_file_writes[filename];
   std::ofstream f(filename, std::ofstream::binary);
   f.write(reinterpret_cast<char*>(&m_header), sizeof(generic_header));
   f.write(reinterpret_cast<char*>(data), (size_t)m_no_of_pts * sizeof(fftw_complex));

@@ -3,6 +3,13 @@
 // parameters, Amplitude_at_time (AMP_T), the gravity/detuning free step,
 // Rabi momentum-bin integrals, chirp / phase scans and their text outputs.
 //
+// Chirp scans restart every chirp from the sequence's start state and hand
+// the end state of chirp 0 to the next sequence.  The restart files are
+// written as in the reference, and the states are also kept as device
+// snapshots, restored without reading the files back unless a later write
+// (an output of the same "%.3f" name) has replaced a file; then the file is
+// read, which is what the reference's reload sees.
+//
 // The GSL diagonalisers registered as "bragg" and "raman"
 // (CRT_Base_IF.h:427-632, 3-state couplings) are not part of this build:
 // selecting them throws.  phase[0] starts at 0 (the reference leaves it
@@ -10,7 +17,9 @@
 #ifndef ATUS_AMD_CRT_BASE_IF_H
 #define ATUS_AMD_CRT_BASE_IF_H
 
+#include <cstdlib>
 #include <list>
+#include <sstream>
 
 #include "CRT_Base.h"
 
@@ -53,6 +62,28 @@ class CRT_Base_IF : public CRT_Base<T, dim, no_int_states> {
 
   void compute_rabi_integrals();
   double Amplitude_at_time();
+
+  // device snapshots beside the chirp scan's restart files "%.3f_<k>.bin"
+  // (CRT_Base_IF.h:757-766 and 835-843 write them, 893-903 and 913-921
+  // read them back): slot 0 the sequence's start, slot 1 the end of chirp 0
+  struct Snapshot {
+    std::vector<std::string> names;
+    std::vector<long long> writes;
+    bool valid = false;
+  };
+  std::array<Snapshot, 2> m_snap;
+  std::vector<std::string> restart_names(double t) const;
+  void take_snapshot(int slot, double t);
+  // the state held by the restart files of time t: the device snapshot
+  // while no write has touched those files since it was taken, else the
+  // files themselves, as the reference reads them
+  void load_restart_files(int slot, double t);
+
+  // phase scans (chirp_mode == 1) as one batch of independent runs, one
+  // per chirp, all started from the sequence's start state; returns the end
+  // time of the chirps
+  bool phase_scan_batchable(const sequence_item& seq, StepFunction step_fct, int Na, int Nk) const;
+  double run_phase_scan_batched(sequence_item& seq, int Na, int Nk, int seq_counter);
 
   // AMP_T evaluator
   atus_host::Expr m_amp_expr;
@@ -199,6 +230,194 @@ void CRT_Base_IF<T, dim, no_int_states>::Output_chirps_list(std::string filename
   }
 }
 
+template <class T, int dim, int no_int_states>
+std::vector<std::string> CRT_Base_IF<T, dim, no_int_states>::restart_names(double t) const {
+  std::vector<std::string> out;
+  char filename[1024];
+  for (int k = 0; k < no_int_states; k++) {
+    snprintf(filename, sizeof(filename), "%.3f_%d.bin", t, k + 1);
+    out.emplace_back(filename);
+  }
+  return out;
+}
+
+template <class T, int dim, int no_int_states>
+void CRT_Base_IF<T, dim, no_int_states>::take_snapshot(int slot, double t) {
+  Snapshot& s = m_snap[slot];
+  s.names = restart_names(t);
+  s.writes.clear();
+  for (auto& n : s.names) s.writes.push_back(this->m_file_writes[n]);
+  atus_host::check(atus_snapshot_save(m_set->device_readonly(), slot), "atus_snapshot_save");
+  s.valid = true;
+}
+
+template <class T, int dim, int no_int_states>
+void CRT_Base_IF<T, dim, no_int_states>::load_restart_files(int slot, double t) {
+  const Snapshot& s = m_snap[slot];
+  const std::vector<std::string> names = restart_names(t);
+  bool current = s.valid && s.names == names;
+  for (size_t k = 0; current && k < names.size(); ++k) current = this->m_file_writes[names[k]] == s.writes[k];
+  if (current) {
+    atus_host::check(atus_snapshot_restore(m_set->device(), slot), "atus_snapshot_restore");
+    return;
+  }
+  for (int k = 0; k < no_int_states; k++) {
+    std::ifstream file1(names[k], std::ifstream::binary);
+    file1.seekg(sizeof(generic_header), std::ifstream::beg);
+    file1.read((char*)m_fields[k]->Getp2In(), sizeof(fftw_complex) * m_no_of_pts);
+  }
+}
+
+// A phase scan records no dphi, so the bisection of the chirp rates
+// (CRT_Base_IF.h:905-912) never swaps and every chirp's rate and phase are
+// known before the first chirp: the chirps are independent runs from the
+// same start state.  They run as one batch when nothing needs the state of
+// a chirp other than 0 between blocks (no custom function, no per-block
+// snapshots whose names could overwrite the restart files) and the batch
+// fits in 64 GiB.  ATUS_CHIRP_BATCH=0 turns this off.
+template <class T, int dim, int no_int_states>
+bool CRT_Base_IF<T, dim, no_int_states>::phase_scan_batchable(const sequence_item& seq, StepFunction step_fct, int Na,
+                                                               int Nk) const {
+  if (no_int_states != 2 || seq.no_of_chirps < 2 || seq.chirp_mode != 1 || Na < 1) return false;
+  if (m_rabi_momentum_list.empty() || m_rabi_momentum_list.size() > 8) return false;
+  auto it = this->m_device_steps.find(step_fct);
+  if (it == this->m_device_steps.end() || it->second != ATUS_STEP_BRAGG_AD) return false;
+  if (m_custom_fct != nullptr && seq.custom_freq != freq::none) return false;
+  if (seq.output_freq == freq::each) return false;
+  // restart files (start) and end files carry different "%.3f" names
+  if (double(Na) * double(Nk) * seq.dt < 0.0011) return false;
+  if (const char* e = std::getenv("ATUS_CHIRP_BATCH"))
+    if (std::string(e) == "0") return false;
+  const double bytes = double(seq.no_of_chirps) * no_int_states * double(m_no_of_pts) * sizeof(fftw_complex);
+  return bytes <= 64.0 * double(1ull << 30);
+}
+
+// the chirp loop of CRT_Base_IF::run_sequence (CRT_Base_IF.h:769-921) for a
+// phase scan, all chirps advancing together (atus_batch_*); what each chirp
+// prints is collected and printed chirp after chirp, as the loop prints it
+template <class T, int dim, int no_int_states>
+double CRT_Base_IF<T, dim, no_int_states>::run_phase_scan_batched(sequence_item& seq, int Na, int Nk, int seq_counter) {
+  const int nch = seq.no_of_chirps;
+  const int nrm = (int)m_rabi_momentum_list.size();
+  const double dt = m_header.dt;
+  std::vector<double> dw(nch, 0.0);
+  dw[0] = seq.chirp_min;
+  dw[1] = seq.chirp_max;
+  for (int s = 1; s < nch - 1; ++s) dw[s + 1] = (dw[s] + dw[s - 1]) / 2;
+  std::vector<atus_batch_run> runs(nch);
+  for (int s = 0; s < nch; ++s) {
+    chirp_rate[0] = dw[s];
+    phase[0] = s * 2.0 * M_PI / (1.0 * nch);
+    runs[s].laser = laser_params();
+    for (int i = 0; i < 4; ++i) runs[s].gs[i] = this->m_gs[i];
+  }
+  struct BatchCtx {
+    atus_ctx* c;
+    ~BatchCtx() { atus_ctx_destroy(c); }
+  } b{m_set->make_batch(nch)};
+  atus_ctx* single = m_set->device_readonly();
+  for (int s = 0; s < nch; ++s)
+    atus_host::check(atus_copy_components(b.c, 2 * s, single, 0, 2), "atus_copy_components");
+  double al[3] = {0, 0, 0};
+  for (int a = 0; a < dim; ++a) al[a] = this->m_alpha[a];
+  atus_host::check(atus_set_alpha(b.c, al), "atus_set_alpha");
+  atus_host::check(atus_batch_set_runs(b.c, runs.data(), nch), "atus_batch_set_runs");
+
+  std::vector<std::ostringstream> out(nch);
+  std::vector<std::list<std::list<double>>> rabi(nch);
+  std::vector<double> F(this->has_amplitude() ? (size_t)Nk * nch : 0), k0(3 * (size_t)nrm, 0.0);
+  std::vector<double> rv((size_t)nrm * nch), pn((size_t)2 * nch);
+  for (int q = 0; q < nrm; ++q)
+    for (int a = 0; a < dim; ++a) k0[3 * q + a] = m_rabi_momentum_list[q][a];
+  auto batch_rabi = [&]() {
+    atus_host::check(atus_batch_rabi_integrals(b.c, 0, k0.data(), nrm, m_rabi_threshold, rv.data()),
+                     "atus_batch_rabi_integrals");
+    for (int s = 0; s < nch; ++s)
+      rabi[s].push_back(std::list<double>(rv.begin() + (size_t)s * nrm, rv.begin() + (size_t)(s + 1) * nrm));
+  };
+  auto batch_pn = [&]() {
+    atus_host::check(atus_batch_particle_numbers(b.c, pn.data()), "atus_batch_particle_numbers");
+    for (int s = 0; s < nch; ++s)
+      for (int c = 0; c < 2; ++c) out[s] << "N[" << c << "] = " << pn[2 * s + c] << "\n";
+  };
+  // chirp 0's state into the single-run context, for its file outputs
+  auto chirp0_to_single = [&](double t) {
+    atus_host::check(atus_copy_components(m_set->device(), 0, b.c, 0, 2), "atus_copy_components");
+    m_header.t = t;
+  };
+  char filename[1024];
+  double t = m_header.t;
+  for (int i = 1; i <= Na; i++) {
+    if (this->has_amplitude()) {
+      // the t1 of the block's V steps, accumulated as the block runner does
+      double tt = t + 0.5 * dt;
+      for (int m = 1; m <= Nk; ++m) {
+        const double f = this->amplitude_at(tt + 0.5 * dt);
+        for (int s = 0; s < nch; ++s) F[(size_t)(m - 1) * nch + s] = f;
+        tt += m < Nk ? dt : 0.5 * dt;
+      }
+    }
+    atus_host::check(atus_batch_run_block(b.c, ATUS_STEP_BRAGG_AD, dt, Nk, &t, F.empty() ? nullptr : F.data()),
+                     "atus_batch_run_block");
+    for (int s = 0; s < nch; ++s) out[s] << "t = " << std::to_string(t) << "\n";
+    if (seq.output_freq == freq::packed) {
+      chirp0_to_single(t);
+      for (int k = 0; k < no_int_states; k++) {
+        snprintf(filename, sizeof(filename), "Seq_%d_%d.bin", seq_counter, k + 1);
+        this->Append_Phi(filename, k);
+      }
+    }
+    if (seq.compute_pn_freq == freq::each) batch_pn();
+    if (seq.rabi_output_freq == freq::each) batch_rabi();
+  }
+  // chirp 0's end: outputs, restart files and snapshot (CRT_Base_IF.h:826-843)
+  chirp0_to_single(t);
+  if (seq.output_freq == freq::last)
+    for (int k = 0; k < no_int_states; k++) {
+      snprintf(filename, sizeof(filename), "%.3f_%d.bin", t, k + 1);
+      this->Save_Phi(filename, k);
+    }
+  for (int k = 0; k < no_int_states; k++) {
+    snprintf(filename, sizeof(filename), "%.3f_%d.bin", t, k + 1);
+    this->Save_Phi(filename, k);
+  }
+  take_snapshot(1, t);
+  if (seq.compute_pn_freq == freq::last) batch_pn();
+  if (seq.rabi_output_freq == freq::last) {
+    batch_rabi();
+    for (int s = 0; s < nch; ++s) {
+      out[s] << "Rabi output for timestep ";
+      for (auto& row : rabi[s]) {
+        out[s] << t << "\n";
+        for (auto j : row) out[s] << j << "\t";
+        out[s] << "\n";
+      }
+    }
+  }
+  for (int s = 0; s < nch; ++s) {
+    std::cout << out[s].str() << std::flush;
+    if (seq.rabi_output_freq == freq::each) {
+      m_rabi_freq_list = rabi[s];
+      snprintf(filename, sizeof(filename), "Rabi_%d_%d.txt", seq_counter, s);
+      Output_rabi_freq_list(filename, seq.Nk);
+    }
+  }
+  // the Rabi populations at the end of every chirp (CRT_Base_IF.h:884-899)
+  atus_host::check(atus_batch_rabi_integrals(b.c, 0, k0.data(), nrm, m_rabi_threshold, rv.data()),
+                   "atus_batch_rabi_integrals");
+  for (int s = 0; s < nch; ++s) {
+    std::list<double> row(rv.begin() + (size_t)s * nrm, rv.begin() + (size_t)(s + 1) * nrm);
+    row.push_front(s * 2.0 * M_PI / (1.0 * nch));
+    row.push_front(s);
+    m_chirps_list.push_back(row);
+  }
+  m_rabi_freq_list = rabi[nch - 1];
+  m_rabi_freq_list.push_back(std::list<double>(rv.end() - nrm, rv.end()));
+  chirp_rate[0] = dw[nch - 1];
+  phase[0] = (nch - 1) * 2.0 * M_PI / (1.0 * nch);
+  return t;
+}
+
 // CRT_Base_IF::run_sequence (CRT_Base_IF.h:639-946)
 template <class T, int dim, int no_int_states>
 void CRT_Base_IF<T, dim, no_int_states>::run_sequence() {
@@ -286,13 +505,16 @@ void CRT_Base_IF<T, dim, no_int_states>::run_sequence() {
         snprintf(filename, sizeof(filename), "%.3f_%d.bin", this->Get_t(), k + 1);
         this->Save_Phi(filename, k);
       }
+      take_snapshot(0, this->Get_t());
       dw[0] = seq.chirp_min;
       dw[1] = seq.chirp_max;
     } else {
       dw[0] = 0;
     }
     const bool fused = this->device_block_possible(step_fct);
-    for (int s = 0; s < nch; ++s) {
+    const bool batched = fused && phase_scan_batchable(seq, step_fct, Na, Nk);
+    if (batched) backup_end_t = run_phase_scan_batched(seq, Na, Nk, seq_counter);
+    for (int s = 0; s < (batched ? 0 : nch); ++s) {
       m_rabi_freq_list.clear();
       chirp_rate[0] = dw[s];
       if (seq.chirp_mode == 1) phase[0] = s * 2.0 * M_PI / (1.0 * seq.no_of_chirps);
@@ -336,6 +558,7 @@ void CRT_Base_IF<T, dim, no_int_states>::run_sequence() {
           snprintf(filename, sizeof(filename), "%.3f_%d.bin", this->Get_t(), k + 1);
           this->Save_Phi(filename, k);
         }
+        take_snapshot(1, this->Get_t());
       }
       if (seq.compute_pn_freq == freq::last)
         for (int c = 0; c < no_int_states; c++)
@@ -368,12 +591,7 @@ void CRT_Base_IF<T, dim, no_int_states>::run_sequence() {
           m_chirps_list.push_back(tmp);
         }
         if (s < seq.no_of_chirps - 1) {
-          for (int k = 0; k < no_int_states; k++) {
-            snprintf(filename, sizeof(filename), "%.3f_%d.bin", backup_t, k + 1);
-            std::ifstream file1(filename, std::ifstream::binary);
-            file1.seekg(sizeof(generic_header), std::ifstream::beg);
-            file1.read((char*)m_fields[k]->Getp2In(), sizeof(fftw_complex) * m_no_of_pts);
-          }
+          load_restart_files(0, backup_t);
           m_header.t = backup_t;
         }
         if (s > 0 && s < seq.no_of_chirps - 1) {
@@ -388,12 +606,7 @@ void CRT_Base_IF<T, dim, no_int_states>::run_sequence() {
     if (seq.no_of_chirps > 1) {
       snprintf(filename, sizeof(filename), "Chirp_%d.txt", seq_counter);
       Output_chirps_list(filename);
-      for (int k = 0; k < no_int_states; k++) {
-        snprintf(filename, sizeof(filename), "%.3f_%d.bin", backup_end_t, k + 1);
-        std::ifstream file1(filename, std::ifstream::binary);
-        file1.seekg(sizeof(generic_header), std::ifstream::beg);
-        file1.read((char*)m_fields[k]->Getp2In(), sizeof(fftw_complex) * m_no_of_pts);
-      }
+      load_restart_files(1, backup_end_t);
       m_header.t = backup_end_t;
     }
     seq_counter++;

@@ -17,6 +17,11 @@ FieldSet::FieldSet(const generic_header& h, int ncomp) : m_ncomp(ncomp) {
   double dk[3] = {h.dkx, h.dky, h.dkz};
   for (int a = m_dim; a < 3; ++a) dx[a] = dk[a] = 0.0;
   m_npts = n[0] * n[1] * n[2];
+  for (int a = 0; a < 3; ++a) {
+    m_n[a] = n[a];
+    m_dx[a] = dx[a];
+    m_dk[a] = dk[a];
+  }
   check(atus_ctx_create(&m_ctx, m_dim, n, dx, dk, ncomp), "atus_ctx_create");
   m_mirror.resize(ncomp);
   m_state.assign(ncomp, IN_SYNC);
@@ -48,6 +53,20 @@ fftw_complex* FieldSet::host(int c) {
   if (m_state[c] == DEVICE_NEWER) check(atus_download(m_ctx, c, m_mirror[c].data()), "atus_download");
   m_state[c] = HOST_NEWER;
   return reinterpret_cast<fftw_complex*>(m_mirror[c].data());
+}
+
+const fftw_complex* FieldSet::host_view(int c) {
+  if (m_state[c] == DEVICE_NEWER) {
+    check(atus_download(m_ctx, c, m_mirror[c].data()), "atus_download");
+    m_state[c] = IN_SYNC;
+  }
+  return reinterpret_cast<const fftw_complex*>(m_mirror[c].data());
+}
+
+atus_ctx* FieldSet::make_batch(int nbatch) const {
+  atus_ctx* b = nullptr;
+  check(atus_batch_create(&b, m_dim, m_n, m_dx, m_dk, m_ncomp, nbatch), "atus_batch_create");
+  return b;
 }
 
 void FieldSet::device_changed() {

@@ -44,8 +44,14 @@ class FieldSet {
   atus_ctx* device_readonly();
   // host mirror of component c, current; may be modified by the caller
   fftw_complex* host(int c);
+  // host mirror of component c, current, for reading only (no upload
+  // before the next device operation)
+  const fftw_complex* host_view(int c);
   // mark all components as changed on the device
   void device_changed();
+  // a new batch context (atus_batch_create) of `nbatch` runs of this set's
+  // grid, each with this set's number of components; the caller destroys it
+  atus_ctx* make_batch(int nbatch) const;
 
  private:
   enum State { IN_SYNC = 0, HOST_NEWER = 1, DEVICE_NEWER = 2 };
@@ -53,6 +59,8 @@ class FieldSet {
   int m_ncomp = 0;
   int m_dim = 0;
   long long m_npts = 0;
+  long long m_n[3] = {1, 1, 1};
+  double m_dx[3] = {0, 0, 0}, m_dk[3] = {0, 0, 0};
   std::vector<std::vector<double>> m_mirror;
   std::vector<int> m_state;
   void flush();

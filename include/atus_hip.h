@@ -54,6 +54,17 @@ int atus_synchronize(atus_ctx *ctx);
  * 607-618): copies of component `comp` between host and device. */
 int atus_upload(atus_ctx *ctx, int comp, const double *host);
 int atus_download(atus_ctx *ctx, int comp, double *host);
+/* device snapshots of all components in slots 0..3 (allocated on first
+ * save): the chirp scan of CRT_Base_IF::run_sequence saves the start state
+ * to "%.3f_%d.bin" files and reloads them before every chirp
+ * (CRT_Base_IF.h:757-766, 893-903, 913-921); a snapshot restores the same
+ * bits without the file and host round trip */
+int atus_snapshot_save(atus_ctx *ctx, int slot);
+int atus_snapshot_restore(atus_ctx *ctx, int slot);
+/* device copy of src components [src_comp, src_comp + n) into dst
+ * components [dst_comp, dst_comp + n); same device and grid size (e.g. one
+ * run's state into the runs of a batch) */
+int atus_copy_components(atus_ctx *dst, int dst_comp, atus_ctx *src, int src_comp, int n);
 /* CRT_Base::Init_Potential + Setup_Potential (CRT_Base.h:277-298):
  * npts doubles for component comp; host == NULL removes all potentials
  * (m_potenial_initialized == false) */

@@ -197,3 +197,152 @@ def test_bragg_ds_two_species_pulses_then_free_flight(orc, tmp_path):
         S.block(orc.FREEPROP_IF2, 25)
     out = np.concatenate([read_bin(tmp_path / f"0.250_{c}.bin")[1] for c in range(1, 5)])
     assert rel_l2(out, np.concatenate(S.psis)) < tol + 1e-11
+
+
+def _chirp_scan_reference(orc, g, psis, t0, Na, Nk, dt, nch, chirp_min, chirp_max, mode, amp, ifp, k0, thr,
+                          output_each=False):
+    """The chirp loop of CRT_Base_IF::run_sequence (CRT_Base_IF.h:744-921)
+    on the oracle, with a dictionary standing in for the file system: every
+    Save_Phi stores the state under its "%.3f_%d.bin" name and the restarts
+    read it back from there, as the reference reads its files."""
+    files = {}
+
+    def save(S):
+        for k in range(2):
+            files[f"{S.t:.3f}_{k + 1}"] = S.psis[k].copy()
+
+    def load(t):
+        return [files[f"{t:.3f}_{k + 1}"].copy() for k in range(2)]
+
+    dw, dphi = [0.0] * max(nch, 2), [0.0] * max(nch, 2)
+    state, t = [p.copy() for p in psis], t0
+    files.update({f"{t0:.3f}_{k + 1}": psis[k].copy() for k in range(2)})
+    dw[0], dw[1] = chirp_min, chirp_max
+    rows, rabi_per_chirp, end = [], [], None
+    for s in range(nch):
+        phase = s * 2 * np.pi / nch if mode == 1 else 0.0
+        S = orc.RTSolver(g, state, [100., 95., 95., 100.], [0.5], dt=dt, t=t, amp=amp, chirp_rate0=dw[s], phase0=phase,
+                         **ifp)
+        per_block = []
+        for _ in range(Na):
+            S.block(orc.BRAGG_AD, Nk)
+            if output_each and s == 0:
+                save(S)
+            per_block.append(orc.rabi_integrals(g, S.psis[0], k0, thr)[0])
+        rabi_per_chirp.append(per_block)
+        if s == 0:
+            save(S)
+            end = (S.t, [p.copy() for p in S.psis])
+        last = list(orc.rabi_integrals(g, S.psis[0], k0, thr)[0])
+        if mode == 1:
+            rows.append([s, phase] + last)
+        else:
+            dphi[s] = last[0]
+            rows.append([s, dw[s]] + last)
+        if s < nch - 1:
+            state, t = load(t0), t0
+        else:
+            state, t = [p.copy() for p in S.psis], S.t
+        if 0 < s < nch - 1:
+            dw[s + 1] = (dw[s] + dw[s - 1]) / 2
+            if dphi[s] - dphi[s - 1] < 0:
+                dw[s], dphi[s] = dw[s - 1], dphi[s - 1]
+    return rows, rabi_per_chirp, end, load(end[0])
+
+
+@pytest.mark.parametrize("case", ["bisection", "phase_scan", "restart_name_collision"])
+def test_bragg_chirp_scan(orc, tmp_path, case):
+    # chirp scans of the bragg driver: Rabi_<seq>_<chirp>.txt per chirp,
+    # Chirp_<seq>.txt, restarts of every chirp from the sequence's start
+    # (device snapshots) and the end state of chirp 0 handed to the next
+    # sequence
+    n, L = 1024, 40.0
+    g = orc.make_grid((n,), L=L)
+    (x,) = orc.coords_x(g)
+    psi1 = np.exp(-x ** 2 / 32).astype(np.complex128)
+    psi1 /= np.sqrt(orc.particle_number(g, psi1))
+    psis = [psi1, np.zeros_like(psi1)]
+    k = 3 * 2 * np.pi / L
+    if case == "bisection":
+        dt, Nk, dur, nch, cmin, cmax, mode, each = 1e-3, 20, "0.1", 5, -1.0, 1.0, 0, False
+    elif case == "phase_scan":
+        dt, Nk, dur, nch, cmin, cmax, mode, each = 1e-3, 20, "0.1", 4, 0.5, 0.5, 1, False
+    else:
+        # blocks of 4e-4: the first block's output "0.000_k.bin" overwrites
+        # the start files, and every later chirp restarts from it, as in the
+        # reference
+        dt, Nk, dur, nch, cmin, cmax, mode, each = 1e-4, 4, "0.0016", 3, -1.0, 1.0, 0, True
+    Na = int(float(dur) / dt) // Nk
+    for c in range(2):
+        write_bin(tmp_path / f"s{c}.bin", psis[c], [L], dt=dt)
+    (tmp_path / "c.xml").write_text(f"""<SIMULATION><DIM>1</DIM>
+      <FILENAME>s0.bin</FILENAME><FILENAME_2>s1.bin</FILENAME_2>
+      <AMP_T>exp(-0.5*((t-0.05)/0.03)^2)</AMP_T>
+      <CONSTANTS><laser_k>{k!r}</laser_k><laser_domh>0.40</laser_domh><laser_dk>0</laser_dk>
+        <rabi_threshold>0.3</rabi_threshold><chirp>0</chirp></CONSTANTS>
+      <VCONSTANTS><GS_1>100,95</GS_1><GS_2>95,100</GS_2><Alpha_1>0.5</Alpha_1>
+        <Beta>0</Beta><Delta_L>0,0</Delta_L><Amp_1>20,20</Amp_1></VCONSTANTS>
+      <ALGORITHM><DT>{dt!r}</DT></ALGORITHM>
+      <SEQUENCE>
+        <bragg_ad dt="{dt!r}" Nk="{Nk}" rabi_output_freq="each" output_freq="{'each' if each else 'last'}"
+          no_of_chirps="{nch}" chirp_min="{cmin!r}" chirp_max="{cmax!r}" chirp_mode="{mode}">{dur}</bragg_ad>
+        <freeprop dt="{dt!r}" Nk="{Nk}" output_freq="last">{float(dur) / 2!r}</freeprop>
+      </SEQUENCE></SIMULATION>""")
+    r = subprocess.run([tool("bragg"), "c.xml"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    amp = lambda t: float(np.exp(-0.5 * ((t - 0.05) / 0.03) ** 2))
+    ifp = dict(amp0=20.0, laser_k=k, laser_domh=0.40, deltaL=[0.0, 0.0])
+    k0 = [(0.0,), (2 * k,), (-2 * k,)]
+    rows, rabi, (t_end, end_state), restored = _chirp_scan_reference(orc, g, psis, 0.0, Na, Nk, dt, nch, cmin, cmax,
+                                                                     mode, amp, ifp, k0, 0.3, output_each=each)
+    # Chirp_1.txt: step, chirp rate (or phase), the Rabi populations, 8 digits
+    lines = (tmp_path / "Chirp_1.txt").read_text().splitlines()
+    assert lines[0].startswith("# Step \t Chirp \t ")
+    got = [[float(v) for v in ln.split("\t") if v] for ln in lines[1:]]
+    assert len(got) == nch
+    for gr, rr in zip(got, rows):
+        assert gr[0] == rr[0] and gr[1] == pytest.approx(rr[1], rel=1e-7, abs=1e-12)
+        assert np.allclose(gr[2:], rr[2:], rtol=1e-7, atol=1e-10), (gr, rr)
+    for s in range(nch):
+        body = (tmp_path / f"Rabi_1_{s}.txt").read_text().splitlines()[1:]
+        vals = np.array([[float(v) for v in ln.split("\t") if v][1:] for ln in body])
+        assert vals.shape == (Na, 3)
+        assert np.allclose(vals, np.array(rabi[s]), rtol=1e-5, atol=1e-9), s
+    # the next sequence starts from the end of chirp 0, read back from its
+    # restart files
+    S = orc.RTSolver(g, restored, [100., 95., 95., 100.], [0.5], dt=dt, t=t_end)
+    for _ in range(int(float(dur) / 2 / dt) // Nk):
+        S.block(orc.FREEPROP_IF, Nk)
+    out = [read_bin(tmp_path / f"{S.t:.3f}_{c}.bin")[1] for c in (1, 2)]
+    assert rel_l2(np.concatenate(out), np.concatenate(S.psis)) < 1e-11
+    if case == "restart_name_collision":
+        # chirp 1 really restarted from the overwritten file: its first Rabi
+        # row differs from chirp 0's
+        assert not np.allclose(rabi[1][0], rabi[0][0], rtol=1e-3)
+    if case == "phase_scan":
+        # the phase scan ran as one batch; the same scan one chirp after
+        # another prints the same lines and writes the same files
+        seq_dir = tmp_path / "sequential"
+        seq_dir.mkdir()
+        for f in ("c.xml", "s0.bin", "s1.bin"):
+            (seq_dir / f).write_bytes((tmp_path / f).read_bytes())
+        env = dict(os.environ, ATUS_CHIRP_BATCH="0")
+        r2 = subprocess.run([tool("bragg"), "c.xml"], cwd=seq_dir, capture_output=True, text=True, timeout=120, env=env)
+        assert r2.returncode == 0, r2.stdout + r2.stderr
+        l1, l2 = r.stdout.splitlines(), r2.stdout.splitlines()
+        assert len(l1) == len(l2)
+        for a, b in zip(l1, l2):
+            if a.startswith("N["):
+                assert float(a.split("=")[1]) == pytest.approx(float(b.split("=")[1]), rel=1e-5)
+            else:
+                assert a == b
+        names = sorted(p.name for p in tmp_path.iterdir() if p.is_file())
+        assert names == sorted(p.name for p in seq_dir.iterdir() if p.is_file())
+        for name in names:
+            if name.endswith(".txt"):
+                A = [[float(v) for v in ln.split("\t") if v] for ln in (tmp_path / name).read_text().splitlines()[1:]]
+                B = [[float(v) for v in ln.split("\t") if v] for ln in (seq_dir / name).read_text().splitlines()[1:]]
+                assert np.allclose(A, B, rtol=1e-6, atol=1e-12), name
+            elif name.endswith(".bin") and not name.startswith("s"):
+                a, b = read_bin(tmp_path / name)[1], read_bin(seq_dir / name)[1]
+                assert np.linalg.norm(a - b) <= 1e-12 * np.linalg.norm(b), name
