@@ -356,8 +356,22 @@ int vkind_of(int kind) {
     case ATUS_STEP_FREEPROP: return V_NL;
     case ATUS_STEP_FREEPROP_IF: return V_NL_IF;
     case ATUS_STEP_BRAGG_AD: return V_BRAGG;
+    case ATUS_STEP_DOUBLE_BRAGG: return V_DBRAGG;
+    case ATUS_STEP_NUM_BRAGG: return V_NUM_BRAGG;
+    case ATUS_STEP_RAMAN: return V_RAMAN;
     default: return -1;
   }
+}
+
+// the components each coupled step acts on
+int check_vk_ncomp(const atus_ctx* c, int vk) {
+  const int n = c->ncomp;
+  if (vk == V_BRAGG && n != 2) return fail(ATUS_ERR_STATE, "bragg_ad needs exactly two components");
+  if (vk == V_DBRAGG && n != 3) return fail(ATUS_ERR_STATE, "the double Bragg step needs three components");
+  if (vk == V_NUM_BRAGG && n != 2 && n != 3)
+    return fail(ATUS_ERR_STATE, "the numerical Bragg step needs two or three components");
+  if (vk == V_RAMAN && n != 3 && n != 4) return fail(ATUS_ERR_STATE, "the Raman step needs three or four components");
+  return ATUS_OK;
 }
 
 void fill_vargs(atus_ctx* c, VArgs& va, double dt, const atus_laser* L) {
@@ -376,6 +390,8 @@ void fill_vargs(atus_ctx* c, VArgs& va, double dt, const atus_laser* L) {
     va.chirp = L->chirp;
     va.chirp_rate0 = L->chirp_rate0;
     va.phase0 = L->phase0;
+    va.amp1 = L->amp1;
+    for (int a = 0; a < 3; ++a) va.beta[a] = a < c->dim ? L->beta[a] : 0.0;
   }
   va.F = 1.0;
 }
@@ -717,7 +733,7 @@ int single_block(atus_ctx* c, int vk, const VArgs& base, int Nk, double dt, doub
   // the fused kernels cover one species of up to two components; two
   // species (per-component kinetic tables, four-component steps) run the
   // V step and the kinetic step as separate passes
-  const bool fused = c->ncomp <= 2 && !c->alpha2_set && vk != V_NL_IF2 && vk != V_BRAGG_DS;
+  const bool fused = c->ncomp <= 2 && !c->alpha2_set && (vk == V_NL || vk == V_NL_IF || vk == V_BRAGG);
   const int N0 = c->n[0], NL = c->n[c->dim - 1];
   const long long inner0 = c->inner_of(0);
   double tt = *t;
@@ -740,6 +756,7 @@ int single_block(atus_ctx* c, int vk, const VArgs& base, int Nk, double dt, doub
   tt += 0.5 * dt;
   for (int m = 1; m <= Nk; ++m) {
     VArgs va = base;
+    va.t0 = tt;
     va.t1 = tt + 0.5 * dt;
     if (fd.t1dev) {
       va.t1p = fd.t1dev + (m - 1);
@@ -889,7 +906,7 @@ int atus_run_block(atus_ctx* c, const atus_step_params* sp, int Nk, double* t) {
   if (Nk < 1) return fail(ATUS_ERR_ARG, "Nk must be >= 1");
   const int vk = vkind_of(sp->kind);
   if (vk < 0) return fail(ATUS_ERR_ARG, "unknown step kind " + std::to_string(sp->kind));
-  if (vk == V_BRAGG && c->ncomp != 2) return fail(ATUS_ERR_STATE, "bragg_ad needs exactly two components");
+  ATUS_TRY(check_vk_ncomp(c, vk));
   ATUS_TRY(ensure_kin(c, sp->dt));
   VArgs base;
   fill_vargs(c, base, sp->dt, &sp->laser);
@@ -898,6 +915,23 @@ int atus_run_block(atus_ctx* c, const atus_step_params* sp, int Nk, double* t) {
   fd.sp = sp;
   ATUS_TRY(single_block(c, vk, base, Nk, sp->dt, t, fd));
   return finish_block(c);
+}
+
+int atus_v_step(atus_ctx* c, int kind, double dt, double t, double F, const atus_laser* laser) {
+  ATUS_TRY(check_ctx(c));
+  ATUS_TRY(check_single(c, "atus_v_step"));
+  const int vk = vkind_of(kind);
+  if (vk < 0) return fail(ATUS_ERR_ARG, "unknown step kind " + std::to_string(kind));
+  ATUS_TRY(check_vk_ncomp(c, vk));
+  if (!std::isfinite(F)) return fail(ATUS_ERR_NUMERIC, "non-finite amplitude");
+  VArgs va;
+  fill_vargs(c, va, dt, laser);
+  va.t0 = t;
+  va.t1 = t + 0.5 * dt;
+  va.F = F;
+  HIP_TRY(launch_point(vk, c->ncomp, c->psi, c->grid(), va, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return ATUS_OK;
 }
 
 }  // extern "C"
@@ -1444,6 +1478,8 @@ int atus_graph_create(atus_ctx* c, int kind, double dt, const atus_laser* laser,
   if (Nk < 1) return fail(ATUS_ERR_ARG, "Nk must be >= 1");
   const int vk = vkind_of(kind);
   if (vk < 0) return fail(ATUS_ERR_ARG, "unknown step kind " + std::to_string(kind));
+  if (vk != V_NL && vk != V_NL_IF && vk != V_BRAGG)
+    return fail(ATUS_ERR_ARG, "graphs replay freeprop, freeprop (IF) and bragg_ad blocks");
   const bool batch = c->runs != nullptr;
   if (batch) {
     if (!c->runs_set) return fail(ATUS_ERR_STATE, "run parameters not set (atus_batch_set_runs)");

@@ -58,6 +58,7 @@ hipError_t launch_point(int vk, int nc, double2* psi, const Grid& gr, const VArg
   ATUS_PT(V_NL_IF, 1) ATUS_PT(V_NL_IF, 2) ATUS_PT(V_NL_IF, 3) ATUS_PT(V_NL_IF, 4)
   ATUS_PT(V_BRAGG, 2)
   ATUS_PT(V_NL_IF2, 2) ATUS_PT(V_NL_IF2, 4) ATUS_PT(V_BRAGG_DS, 4)
+  ATUS_PT(V_DBRAGG, 3) ATUS_PT(V_NUM_BRAGG, 2) ATUS_PT(V_NUM_BRAGG, 3) ATUS_PT(V_RAMAN, 3) ATUS_PT(V_RAMAN, 4)
 #undef ATUS_PT
   return hipErrorInvalidValue;
 }

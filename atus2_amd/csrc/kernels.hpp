@@ -24,7 +24,19 @@
 namespace atus {
 
 // V_NL_IF2 / V_BRAGG_DS: two species, components [NC/2, NC) are species 2
-enum VKind { V_NONE = 0, V_NL = 1, V_NL_IF = 2, V_BRAGG = 3, V_NL_IF2 = 4, V_BRAGG_DS = 5 };
+// V_DBRAGG / V_NUM_BRAGG / V_RAMAN: three-state couplings (standalone
+// pointwise kernel only)
+enum VKind {
+  V_NONE = 0,
+  V_NL = 1,
+  V_NL_IF = 2,
+  V_BRAGG = 3,
+  V_NL_IF2 = 4,
+  V_BRAGG_DS = 5,
+  V_DBRAGG = 6,
+  V_NUM_BRAGG = 7,
+  V_RAMAN = 8
+};
 
 struct Grid {
   int dim;
@@ -78,6 +90,11 @@ struct VArgs {
   // gates of bragg_ds (1: on, 0: off) for this step
   double amp2, laser_k2, laser_domh2, laser_dk2, chirp_rate2, phase2, beta2;
   double mode1, mode2;
+  // three-state steps: Amp[1], the whole Beta vector (beta.x over all axes)
+  // and Get_t() before the step (the numerical Bragg step reads it)
+  double amp1;
+  double beta[3];
+  double t0;
 };
 
 // the parameters of run r replace the shared ones
@@ -389,6 +406,242 @@ __device__ __forceinline__ void v_point(double2 (&a)[NC], double x0, long long l
     static_assert(NC == 4, "the two-species Bragg step has four components");
     bragg_ds_point(a, x0, p);
   }
+}
+
+// ------------------------------------------------------------ three states
+
+// psi <- exp(-i dt H) psi for a Hermitian N x N matrix H (H is overwritten).
+// Cyclic complex Jacobi rotations diagonalise H = V diag(lambda) V^H; then
+// psi <- V diag(exp(-i dt lambda)) V^H psi.  The reference forms the same
+// exponential from GSL's Hermitian eigensolver (CRT_Base_IF.h:491-512);
+// Jacobi keeps the eigenvalues accurate to eps*|H| also when they are
+// (nearly) degenerate.  A rotation on (p, q) is J = D R with
+// D = diag(1, e^{-i phi}) (phi = arg H_pq) and R the real Jacobi rotation
+// [[c, s], [-s, c]] that zeroes the then real off-diagonal element.
+template <int N>
+__device__ __forceinline__ void herm_expm_apply(double2 (&H)[N][N], double dt, double2 (&psi)[N]) {
+  double2 V[N][N];
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) V[i][j] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+  for (int sweep = 0; sweep < 16; ++sweep) {
+    double off = 0.0, dg = 0.0;
+#pragma unroll
+    for (int p = 0; p < N; ++p) {
+      dg += H[p][p].x * H[p][p].x;
+#pragma unroll
+      for (int q = p + 1; q < N; ++q) off += H[p][q].x * H[p][q].x + H[p][q].y * H[p][q].y;
+    }
+    if (off == 0.0 || off <= 1e-36 * dg) break;
+#pragma unroll
+    for (int p = 0; p < N - 1; ++p) {
+#pragma unroll
+      for (int q = p + 1; q < N; ++q) {
+        const double2 apq = H[p][q];
+        const double r = hypot(apq.x, apq.y);
+        if (r == 0.0) continue;
+        const double d = (H[q][q].x - H[p][p].x) / (2.0 * r);
+        const double t = (d >= 0.0 ? 1.0 : -1.0) / (fabs(d) + sqrt(1.0 + d * d));
+        const double c = 1.0 / sqrt(1.0 + t * t), sn = t * c;
+        const double2 em = make_double2(apq.x / r, -apq.y / r);  // e^{-i phi}
+        const double2 ep = make_double2(em.x, -em.y);            // e^{+i phi}
+        // H J (columns p, q)
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+          const double2 hp = H[k][p], hq = cmul(em, H[k][q]);
+          H[k][p] = make_double2(c * hp.x - sn * hq.x, c * hp.y - sn * hq.y);
+          H[k][q] = make_double2(sn * hp.x + c * hq.x, sn * hp.y + c * hq.y);
+        }
+        // J^H (H J) (rows p, q)
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+          const double2 bp = H[p][k], bq = cmul(ep, H[q][k]);
+          H[p][k] = make_double2(c * bp.x - sn * bq.x, c * bp.y - sn * bq.y);
+          H[q][k] = make_double2(sn * bp.x + c * bq.x, sn * bp.y + c * bq.y);
+        }
+        H[p][q] = make_double2(0.0, 0.0);
+        H[q][p] = make_double2(0.0, 0.0);
+        H[p][p].y = 0.0;
+        H[q][q].y = 0.0;
+        // V J
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+          const double2 vp = V[k][p], vq = cmul(em, V[k][q]);
+          V[k][p] = make_double2(c * vp.x - sn * vq.x, c * vp.y - sn * vq.y);
+          V[k][q] = make_double2(sn * vp.x + c * vq.x, sn * vp.y + c * vq.y);
+        }
+      }
+    }
+  }
+  double2 y[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+      // conj(V[k][i]) * psi[k]
+      acc.x += V[k][i].x * psi[k].x + V[k][i].y * psi[k].y;
+      acc.y += V[k][i].x * psi[k].y - V[k][i].y * psi[k].x;
+    }
+    double sn, cs;
+    sincos(-dt * H[i][i].x, &sn, &cs);
+    y[i] = cmul(acc, make_double2(cs, sn));
+  }
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int i = 0; i < N; ++i) acc = cadd(acc, cmul(V[k][i], y[i]));
+    psi[k] = acc;
+  }
+}
+
+// diagonal of the three-state steps: sum_j g_ij |psi_j|^2 + beta.x - DeltaL_i
+// (CRT_Base_IF.h:455-466, 569-580)
+template <int NC>
+__device__ __forceinline__ void three_state_diag(double2 (&H)[NC][NC], const double2 (&a)[NC], double bx,
+                                                 const VArgs& p) {
+  double n[NC];
+#pragma unroll
+  for (int j = 0; j < NC; ++j) n[j] = a[j].x * a[j].x + a[j].y * a[j].y;
+#pragma unroll
+  for (int i = 0; i < NC; ++i) {
+#pragma unroll
+    for (int j = 0; j < NC; ++j) H[i][j] = make_double2(0.0, 0.0);
+    double ph = 0.0;
+#pragma unroll
+    for (int j = 0; j < NC; ++j) ph += p.gs[j + NC * i] * n[j];
+    ph += bx - p.deltaL[i];
+    H[i][i].x = ph;
+  }
+}
+
+// CRT_Base_IF::Numerical_Bragg (CRT_Base_IF.h:426-527): state 0 coupled to
+// state i+1 by the lasers of wave number +-laser_k and chirp +-chirp_rate,
+// at t = Get_t() before the step
+template <int NC>
+__device__ __forceinline__ void num_bragg_point(double2 (&a)[NC], double x0, double bx, const VArgs& p) {
+  static_assert(NC == 2 || NC == 3, "the numerical Bragg step has two or three states");
+  double2 H[NC][NC];
+  three_state_diag<NC>(H, a, bx, p);
+  const double t = p.t0;
+#pragma unroll
+  for (int i = 0; i < NC - 1; ++i) {
+    const double k = i == 0 ? p.laser_k : -p.laser_k;
+    const double cr = i == 0 ? p.chirp_rate0 : -p.chirp_rate0;
+    double sn, cs;
+    sincos((-p.laser_domh + cr * t) * t + k * x0 - 0.5 * p.phase0, &sn, &cs);
+    const double er = p.amp0 * cs / 2 + p.amp1 * cs / 2;
+    const double ei = p.amp0 * sn / 2 - p.amp1 * sn / 2;
+    H[i + 1][0] = make_double2(er, ei);
+    H[0][i + 1] = make_double2(er, -ei);
+  }
+  herm_expm_apply<NC>(H, p.dt, a);
+}
+
+// CRT_Base_IF::Numerical_Raman (CRT_Base_IF.h:534-632): states 0 and 1
+// coupled to state 2, which carries the detuning laser_domh
+template <int NC>
+__device__ __forceinline__ void raman_point(double2 (&a)[NC], double x0, double bx, const VArgs& p) {
+  static_assert(NC >= 3, "the Raman step couples three states");
+  double2 H[NC][NC];
+  three_state_diag<NC>(H, a, bx, p);
+  H[2][2].x = H[2][2].x + p.laser_domh;
+  double sn, cs;
+  sincos(p.laser_k * x0, &sn, &cs);
+  H[0][2] = make_double2(p.amp0 / 2 * cs, p.amp0 / 2 * sn);
+  H[2][0] = make_double2(p.amp0 / 2 * cs, -(p.amp0 / 2 * sn));
+  H[1][2] = make_double2(p.amp1 / 2 * cs, -(p.amp1 / 2 * sn));
+  H[2][1] = make_double2(p.amp1 / 2 * cs, p.amp1 / 2 * sn);
+  herm_expm_apply<NC>(H, p.dt, a);
+}
+
+// Bragg_double::Do_Double_Bragg_ad (double_bragg.cpp:108-249): the
+// reference's closed-form 3x3 exponential, its operations in its order
+// (O23 and O32 take their phases as written there)
+__device__ __forceinline__ void dbragg_point(double2 (&a)[3], double x0, double bx, const VArgs& p) {
+  const double dt = p.dt, t1 = p.t1;
+  const double F = p.F < 0.0 ? 0.0 : p.F;
+  const double n1 = a[0].x * a[0].x + a[0].y * a[0].y;
+  const double n2 = a[1].x * a[1].x + a[1].y * a[1].y;
+  const double n3 = a[2].x * a[2].x + a[2].y * a[2].y;
+  const double V11 = p.gs[0] * n1 + p.gs[1] * n2 + p.gs[2] * n3 + bx;
+  const double V22 = p.gs[3] * n1 + p.gs[4] * n2 + p.gs[5] * n3 - p.deltaL[1] + bx;
+  double dw = p.laser_domh + p.chirp_rate0 * t1;
+  const double Op = F * p.amp0 * cos(p.laser_k * x0 - dw * t1 - p.phase0 / 2);
+  dw = p.laser_domh - p.chirp_rate0 * t1;
+  const double Om = F * p.amp0 * cos(-p.laser_k * x0 - dw * t1 - p.phase0 / 2);
+  double im1, re1, tmp1, tmp2;
+  if (Om == 0.0 && Op == 0.0) {
+    sincos(-dt * V11, &im1, &re1);
+    tmp1 = a[0].x;
+    a[0].x = tmp1 * re1 - a[0].y * im1;
+    a[0].y = tmp1 * im1 + a[0].y * re1;
+    sincos(-dt * V22, &im1, &re1);
+#pragma unroll
+    for (int c = 1; c < 3; ++c) {
+      tmp1 = a[c].x;
+      a[c].x = tmp1 * re1 - a[c].y * im1;
+      a[c].y = tmp1 * im1 + a[c].y * re1;
+    }
+    return;
+  }
+  sincos(-0.5 * p.laser_dk * x0, &im1, &re1);
+  const double e0 = re1, e1 = im1;
+  tmp1 = sqrt((V11 - V22) * (V11 - V22) + 4.0 * (Op * Op + Om * Om));
+  const double E1 = 0.5 * (V11 + V22 + tmp1);
+  const double E2 = 0.5 * (V11 + V22 - tmp1);
+  sincos(-dt * E1, &im1, &re1);
+  tmp1 = 1.0 / fabs((V22 - E1) * (V22 - E1) + Op * Op + Om * Om);
+  const double g1r = tmp1 * re1, g1i = tmp1 * im1;
+  sincos(-dt * E2, &im1, &re1);
+  tmp1 = 1.0 / fabs((V22 - E2) * (V22 - E2) + Op * Op + Om * Om);
+  const double g2r = tmp1 * re1, g2i = tmp1 * im1;
+  sincos(-dt * V22, &im1, &re1);
+  tmp1 = 1.0 / fabs(Op * Op + Om * Om);
+  const double g3r = tmp1 * re1, g3i = tmp1 * im1;
+  tmp1 = (V22 - E2) * (V22 - E2);
+  tmp2 = (V22 - E1) * (V22 - E1);
+  const double O11r = tmp1 * g2r + tmp2 * g1r, O11i = tmp1 * g2i + tmp2 * g1i;
+  tmp1 = (V22 - E2) * g2r + (V22 - E1) * g1r;
+  tmp2 = (V22 - E2) * g2i + (V22 - E1) * g1i;
+  double O12r = -Op * tmp1, O12i = -Op * tmp2;
+  double O13r = -Om * tmp1, O13i = -Om * tmp2;
+  double O21r = O12r, O21i = O12i;
+  double O31r = O13r, O31i = O13i;
+  tmp1 = O12r;
+  O12r = e0 * tmp1 - e1 * O12i;
+  O12i = e1 * tmp1 + e0 * O12i;
+  tmp1 = O21r;
+  O21r = e0 * tmp1 + e1 * O21i;
+  O21i = e0 * O21i - e1 * tmp1;
+  tmp1 = O31r;
+  O31r = e0 * tmp1 - e1 * O31i;
+  O31i = e1 * tmp1 + e0 * O31i;
+  tmp1 = O13r;
+  O13r = e0 * tmp1 + e1 * O13i;
+  O13i = e0 * O13i - e1 * tmp1;
+  tmp1 = Op * Op;
+  tmp2 = Om * Om;
+  const double O22r = tmp1 * (g2r + g1r) + tmp2 * g3r, O22i = tmp1 * (g2i + g1i) + tmp2 * g3i;
+  const double O33r = tmp2 * (g2r + g1r) + tmp1 * g3r, O33i = tmp2 * (g2i + g1i) + tmp1 * g3i;
+  tmp1 = Om * Op;
+  double O23r = tmp1 * (g1r + g2r - g3r), O23i = tmp1 * (g1i + g2i - g3i);
+  double O32r = O23r, O32i = O23i;
+  tmp1 = O23r;
+  O23r = (e0 * e0 + e1 * e1) * tmp1 + 2 * e0 * e1 * O23i;
+  O23i = (e0 * e0 + e1 * e1) * O23i - 2 * e0 * e1 * tmp1;
+  tmp1 = O32r;
+  O32r = (e0 * e0 - e1 * e1) * tmp1 - 2 * e0 * e1 * O32i;
+  O32i = (e0 * e0 - e1 * e1) * O23i + 2 * e0 * e1 * tmp1;
+  const double2 q1 = a[0], q2 = a[1], q3 = a[2];
+  a[0].x = (O11r * q1.x - O11i * q1.y) + (O12r * q2.x - O12i * q2.y) + (O13r * q3.x - O13i * q3.y);
+  a[0].y = (O11r * q1.y + O11i * q1.x) + (O12r * q2.y + O12i * q2.x) + (O13r * q3.y + O13i * q3.x);
+  a[1].x = (O21r * q1.x - O21i * q1.y) + (O22r * q2.x - O22i * q2.y) + (O23r * q3.x - O23i * q3.y);
+  a[1].y = (O21r * q1.y + O21i * q1.x) + (O22r * q2.y + O22i * q2.x) + (O23r * q3.y + O23i * q3.x);
+  a[2].x = (O31r * q1.x - O31i * q1.y) + (O32r * q2.x - O32i * q2.y) + (O33r * q3.x - O33i * q3.y);
+  a[2].y = (O31r * q1.y + O31i * q1.x) + (O32r * q2.y + O32i * q2.x) + (O33r * q3.y + O33i * q3.x);
 }
 
 // ------------------------------------------------------------ k_pass
@@ -714,6 +967,20 @@ __device__ __forceinline__ double x0_of(long long l, const Grid& gr) {
   return (double)(i0 - gr.n[0] / 2) * gr.dx[0];
 }
 
+// beta.x over the grid's axes, summed as CPoint::operator* does
+// (CPoint.h:145-151)
+__device__ __forceinline__ double beta_dot_x(long long l, const Grid& gr, const double (&beta)[3]) {
+  long long rest = l;
+  long long idx[3] = {0, 0, 0};
+  for (int a = gr.dim - 1; a >= 0; --a) {
+    idx[a] = rest % gr.n[a];
+    rest /= gr.n[a];
+  }
+  double bx = 0.0;
+  for (int a = 0; a < gr.dim; ++a) bx += beta[a] * ((double)(idx[a] - gr.n[a] / 2) * gr.dx[a]);
+  return bx;
+}
+
 template <int VK, int NC>
 __global__ __launch_bounds__(256) void k_point(double2* __restrict__ psi, Grid gr, VArgs va) {
   const long long stride = (long long)gridDim.x * blockDim.x;
@@ -725,7 +992,15 @@ __global__ __launch_bounds__(256) void k_point(double2* __restrict__ psi, Grid g
     double Om = 0.0;
     double2 eta = make_double2(1.0, 0.0);
     if constexpr (VK == V_BRAGG) bragg_field(x0, va, Om, eta);
-    v_point<VK, NC>(a, x0, l, va, Om, eta);
+    if constexpr (VK == V_DBRAGG) {
+      dbragg_point(a, x0, beta_dot_x(l, gr, va.beta), va);
+    } else if constexpr (VK == V_NUM_BRAGG) {
+      num_bragg_point<NC>(a, x0, beta_dot_x(l, gr, va.beta), va);
+    } else if constexpr (VK == V_RAMAN) {
+      raman_point<NC>(a, x0, beta_dot_x(l, gr, va.beta), va);
+    } else {
+      v_point<VK, NC>(a, x0, l, va, Om, eta);
+    }
 #pragma unroll
     for (int c = 0; c < NC; ++c) psi[c * gr.npts + l] = a[c];
   }

@@ -15,6 +15,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libatus_hip.so")
 
 STEP_FREEPROP, STEP_FREEPROP_IF, STEP_BRAGG_AD = 0, 1, 2
+STEP_DOUBLE_BRAGG, STEP_NUM_BRAGG, STEP_RAMAN = 3, 4, 5
 
 EXPORTED = [
     "atus_last_error", "atus_version", "atus_device_info", "atus_ctx_create", "atus_ctx_destroy",
@@ -26,7 +27,7 @@ EXPORTED = [
     "atus_batch_create", "atus_batch_set_runs", "atus_batch_run_block", "atus_batch_particle_numbers",
     "atus_batch_rabi_integrals", "atus_sob_run", "atus_graph_create", "atus_graph_launch", "atus_graph_destroy",
     "atus_set_alpha2", "atus_run_block_ds", "atus_v_step_ds",
-    "atus_snapshot_save", "atus_snapshot_restore", "atus_copy_components",
+    "atus_snapshot_save", "atus_snapshot_restore", "atus_copy_components", "atus_v_step",
     "atus_net_create", "atus_net_destroy", "atus_net_set_precision", "atus_net_param_count", "atus_net_get_params", "atus_net_set_params",
     "atus_net_forward", "atus_net_gradients", "atus_net_set_data", "atus_net_train", "atus_net_warm_start",
 ]
@@ -41,7 +42,8 @@ class AtusError(RuntimeError):
 class Laser(C.Structure):
     _fields_ = [("amp0", C.c_double), ("laser_k", C.c_double), ("laser_domh", C.c_double),
                 ("laser_dk", C.c_double), ("chirp", C.c_double), ("chirp_rate0", C.c_double),
-                ("phase0", C.c_double), ("beta0", C.c_double), ("deltaL", C.c_double * 4)]
+                ("phase0", C.c_double), ("beta0", C.c_double), ("deltaL", C.c_double * 4),
+                ("amp1", C.c_double), ("beta", C.c_double * 3)]
 
 
 AMP_FN = C.CFUNCTYPE(C.c_double, C.c_double, C.c_void_p)
@@ -121,6 +123,7 @@ def load():
         "atus_run_block_ds": (I, [P, C.POINTER(StepParams), C.POINTER(DsParams), I, C.POINTER(D)]),
         "atus_v_step_ds": (I, [P, I, D, D, C.POINTER(Laser), C.POINTER(DsParams)]),
         "atus_snapshot_save": (I, [P, I]),
+        "atus_v_step": (I, [P, I, D, D, D, C.POINTER(Laser)]),
         "atus_snapshot_restore": (I, [P, I]),
         "atus_copy_components": (I, [P, I, P, I, I]),
         "atus_net_create": (I, [pctx, I, I, I, I, I, I, C.c_ulonglong]),
@@ -157,10 +160,13 @@ def _dp(a: np.ndarray):
 def laser(**kw) -> Laser:
     L = Laser()
     dl = kw.pop("deltaL", (0.0,) * 4)
+    beta = kw.pop("beta", (0.0,) * 3)
     for k, v in kw.items():
         setattr(L, k, float(v))
     for i, v in enumerate(dl):
         L.deltaL[i] = float(v)
+    for i, v in enumerate(beta):
+        L.beta[i] = float(v)
     return L
 
 
@@ -215,6 +221,10 @@ class Context:
         a = np.empty(self.shape, np.complex128)
         _check(_lib.atus_download(self._h, comp, _dp(a)))
         return a
+
+    def v_step(self, kind, dt, t, las: Laser, F=1.0):
+        """One V step of any block kind at Get_t() = t, no kinetic step."""
+        _check(_lib.atus_v_step(self._h, int(kind), float(dt), float(t), float(F), C.byref(las)))
 
     def snapshot_save(self, slot=0):
         _check(_lib.atus_snapshot_save(self._h, int(slot)))

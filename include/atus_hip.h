@@ -100,6 +100,8 @@ typedef struct {
   double phase0;      /* phase[0] */
   double beta0;       /* beta[0] */
   double deltaL[4];   /* DeltaL */
+  double amp1;        /* Amp[1]: numerical Bragg and Raman steps */
+  double beta[3];     /* Beta: the three-state steps add beta.x over all axes */
 } atus_laser;
 
 /* Bragg_single::Do_Bragg_ad (bragg.cpp:186-304), "bragg_ad": t = Get_t()
@@ -115,6 +117,11 @@ int atus_set_momentum(atus_ctx *ctx, int comp, const double p[3]);
 #define ATUS_STEP_FREEPROP 0    /* CRT_Base::Do_NL_Step */
 #define ATUS_STEP_FREEPROP_IF 1 /* CRT_Base_IF::Do_NL_Step */
 #define ATUS_STEP_BRAGG_AD 2    /* Bragg_single::Do_Bragg_ad */
+/* three internal states (ncomp 3; the numerical Bragg step also takes 2
+ * and the Raman step 4); these run the V step as its own pass */
+#define ATUS_STEP_DOUBLE_BRAGG 3 /* Bragg_double::Do_Double_Bragg_ad (double_bragg.cpp:108-249) */
+#define ATUS_STEP_NUM_BRAGG 4    /* CRT_Base_IF::Numerical_Bragg (CRT_Base_IF.h:426-527), "bragg" */
+#define ATUS_STEP_RAMAN 5        /* CRT_Base_IF::Numerical_Raman (CRT_Base_IF.h:534-632), "raman" */
 
 /* AMP_T as a callback: returns Amplitude_at_time() for the time t1 that
  * CRT_Base_IF::Amplitude_at_time would evaluate (Get_t()+0.5*Get_dt()) */
@@ -129,6 +136,10 @@ typedef struct {
 } atus_step_params;
 
 int atus_run_block(atus_ctx *ctx, const atus_step_params *sp, int Nk, double *t);
+/* one V step of a single run (no kinetic step), any kind of atus_run_block:
+ * t = Get_t() before the step, F = Amplitude_at_time() (read by BRAGG_AD
+ * and DOUBLE_BRAGG, which clamps it at 0 as double_bragg.cpp:117 does) */
+int atus_v_step(atus_ctx *ctx, int kind, double dt, double t, double F, const atus_laser *laser);
 
 /* ---- two species (CRT_Base_2, CRT_Base_IF_2, the bragg_ds driver):
  * components [ncomp/2, ncomp) are species 2 */

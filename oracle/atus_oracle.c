@@ -680,6 +680,19 @@ static void v_step(orc_rt *s, int kind) {
     orc_bragg_ds(&s->g, s->psi, s, t1, time > s->duration[0] ? 0 : 1, time > s->duration[1] ? 0 : 1);
     return;
   }
+  if (kind == ORC_DOUBLE_BRAGG) {
+    const double t1 = s->t + 0.5 * s->dt;
+    orc_double_bragg(&s->g, s->psi, s, t1, s->amp ? s->amp(t1, s->amp_user) : 1.0);
+    return;
+  }
+  if (kind == ORC_NUM_BRAGG) {
+    orc_num_bragg(&s->g, s->ncomp, s->psi, s, s->t);
+    return;
+  }
+  if (kind == ORC_RAMAN) {
+    orc_raman(&s->g, s->ncomp, s->psi, s);
+    return;
+  }
   if (kind == ORC_FREEPROP) {
     orc_nl_step(&s->g, s->ncomp, s->psi, s->gs, s->dt, s->pot);
   } else if (kind == ORC_FREEPROP_IF) {
@@ -701,6 +714,285 @@ static void v_step(orc_rt *s, int kind) {
     p.deltaL1 = s->deltaL[1];
     for (int q = 0; q < 4; q++) p.gs[q] = s->gs[q];
     orc_bragg_ad(&s->g, s->psi[0], s->psi[1], &p);
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* three-state steps                                                   */
+/* ------------------------------------------------------------------ */
+
+void orc_herm_expm(int n, const double *H, double dt, double *U) {
+  long double ar[4][4], ai[4][4], vr[4][4], vi[4][4];
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++) {
+      ar[i][j] = H[2 * (i * n + j)];
+      ai[i][j] = H[2 * (i * n + j) + 1];
+      vr[i][j] = i == j;
+      vi[i][j] = 0;
+    }
+  for (int it = 0; it < 200; it++) {
+    /* the largest off-diagonal element */
+    int p = 0, q = 1;
+    long double big = -1, dg = 0;
+    for (int i = 0; i < n; i++) {
+      dg += ar[i][i] * ar[i][i];
+      for (int j = i + 1; j < n; j++) {
+        long double m = ar[i][j] * ar[i][j] + ai[i][j] * ai[i][j];
+        if (m > big) {
+          big = m;
+          p = i;
+          q = j;
+        }
+      }
+    }
+    if (n < 2 || big <= 1e-44L * dg || big == 0) break;
+    /* H_pq = r e^{i phi}; with D = diag(1, e^{-i phi}) on (p, q) the pair
+     * is real symmetric, R = [[c, s], [-s, c]] zeroes it; J = D R */
+    long double r = sqrtl(big);
+    long double cr = ar[p][q] / r, ci = -ai[p][q] / r; /* e^{-i phi} */
+    long double th = (ar[q][q] - ar[p][p]) / (2 * r);
+    long double t = (th >= 0 ? 1.0L : -1.0L) / (fabsl(th) + sqrtl(1 + th * th));
+    long double c = 1 / sqrtl(1 + t * t), sn = t * c;
+    for (int k = 0; k < n; k++) { /* columns: H J */
+      long double hpr = ar[k][p], hpi = ai[k][p];
+      long double hqr = cr * ar[k][q] - ci * ai[k][q], hqi = cr * ai[k][q] + ci * ar[k][q];
+      ar[k][p] = c * hpr - sn * hqr;
+      ai[k][p] = c * hpi - sn * hqi;
+      ar[k][q] = sn * hpr + c * hqr;
+      ai[k][q] = sn * hpi + c * hqi;
+    }
+    for (int k = 0; k < n; k++) { /* rows: J^H (H J), conj(e^{-i phi}) = e^{i phi} */
+      long double bpr = ar[p][k], bpi = ai[p][k];
+      long double bqr = cr * ar[q][k] + ci * ai[q][k], bqi = cr * ai[q][k] - ci * ar[q][k];
+      ar[p][k] = c * bpr - sn * bqr;
+      ai[p][k] = c * bpi - sn * bqi;
+      ar[q][k] = sn * bpr + c * bqr;
+      ai[q][k] = sn * bpi + c * bqi;
+    }
+    ar[p][q] = ai[p][q] = ar[q][p] = ai[q][p] = 0;
+    ai[p][p] = ai[q][q] = 0;
+    for (int k = 0; k < n; k++) { /* V J */
+      long double vpr = vr[k][p], vpi = vi[k][p];
+      long double vqr = cr * vr[k][q] - ci * vi[k][q], vqi = cr * vi[k][q] + ci * vr[k][q];
+      vr[k][p] = c * vpr - sn * vqr;
+      vi[k][p] = c * vpi - sn * vqi;
+      vr[k][q] = sn * vpr + c * vqr;
+      vi[k][q] = sn * vpi + c * vqi;
+    }
+  }
+  /* U = V diag(exp(-i dt lambda)) V^H */
+  long double er[4], ei[4];
+  for (int i = 0; i < n; i++) {
+    er[i] = cosl(-(long double)dt * ar[i][i]);
+    ei[i] = sinl(-(long double)dt * ar[i][i]);
+  }
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++) {
+      long double sr = 0, si = 0;
+      for (int k = 0; k < n; k++) {
+        /* V_ik e_k conj(V_jk) */
+        long double xr = vr[i][k] * er[k] - vi[i][k] * ei[k], xi = vr[i][k] * ei[k] + vi[i][k] * er[k];
+        sr += xr * vr[j][k] + xi * vi[j][k];
+        si += xi * vr[j][k] - xr * vi[j][k];
+      }
+      U[2 * (i * n + j)] = (double)sr;
+      U[2 * (i * n + j) + 1] = (double)si;
+    }
+}
+
+/* beta.x as CPoint::operator* sums it (CPoint.h:145-151) */
+static double beta_dot(const orc_grid *g, const double *beta, const double *x) {
+  double r = 0;
+  for (int a = 0; a < g->dim; a++) r += beta[a] * x[a];
+  return r;
+}
+
+static void apply_u(int n, const double *U, double **psi, long long l) {
+  double in[8], out[8];
+  for (int i = 0; i < n; i++) {
+    in[2 * i] = psi[i][2 * l];
+    in[2 * i + 1] = psi[i][2 * l + 1];
+  }
+  for (int i = 0; i < n; i++) {
+    double sr = 0, si = 0;
+    for (int k = 0; k < n; k++) {
+      const double ur = U[2 * (i * n + k)], ui = U[2 * (i * n + k) + 1];
+      sr += ur * in[2 * k] - ui * in[2 * k + 1];
+      si += ur * in[2 * k + 1] + ui * in[2 * k];
+    }
+    out[2 * i] = sr;
+    out[2 * i + 1] = si;
+  }
+  for (int i = 0; i < n; i++) {
+    psi[i][2 * l] = out[2 * i];
+    psi[i][2 * l + 1] = out[2 * i + 1];
+  }
+}
+
+/* diagonal: phi_i = sum_j g_ij |psi_j|^2, then += beta.x - DeltaL_i
+ * (CRT_Base_IF.h:455-466) */
+static void three_state_diag(int n, double *H, double **psi, long long l, const orc_rt *s, double bx) {
+  for (int i = 0; i < 2 * n * n; i++) H[i] = 0;
+  for (int i = 0; i < n; i++) {
+    double phi = 0;
+    for (int j = 0; j < n; j++) {
+      const double re = psi[j][2 * l], im = psi[j][2 * l + 1];
+      phi += s->gs[j + n * i] * (re * re + im * im);
+    }
+    phi += bx - s->deltaL[i];
+    H[2 * (i * n + i)] = phi;
+  }
+}
+
+void orc_num_bragg(const orc_grid *g, int ncomp, double **psi, const orc_rt *s, double t) {
+  long long np = orc_npts(g);
+  const int n = ncomp;
+  /* laser_k[1] = -laser_k[0], chirp_rate[1] = -chirp_rate[0] (:447-448) */
+  const double lk[2] = {s->laser_k, -s->laser_k};
+  const double cr[2] = {s->chirp_rate0, -s->chirp_rate0};
+#pragma omp parallel for schedule(static)
+  for (long long l = 0; l < np; l++) {
+    double x[3], H[32], U[32];
+    orc_get_x(g, l, x);
+    three_state_diag(n, H, psi, l, s, beta_dot(g, s->beta, x));
+    for (int i = 0; i < n - 1; i++) {
+      const double th = (-s->laser_domh + cr[i] * t) * t + lk[i] * x[0] - 0.5 * s->phase0;
+      const double re1 = cos(th), im1 = sin(th);
+      const double e0 = s->amp0 * re1 / 2 + s->amp1 * re1 / 2;
+      const double e1 = s->amp0 * im1 / 2 - s->amp1 * im1 / 2;
+      H[2 * ((i + 1) * n + 0)] = e0;
+      H[2 * ((i + 1) * n + 0) + 1] = e1;
+      H[2 * (0 * n + i + 1)] = e0;
+      H[2 * (0 * n + i + 1) + 1] = -e1;
+    }
+    orc_herm_expm(n, H, s->dt, U);
+    apply_u(n, U, psi, l);
+  }
+}
+
+void orc_raman(const orc_grid *g, int ncomp, double **psi, const orc_rt *s) {
+  long long np = orc_npts(g);
+  const int n = ncomp;
+#pragma omp parallel for schedule(static)
+  for (long long l = 0; l < np; l++) {
+    double x[3], H[32], U[32];
+    orc_get_x(g, l, x);
+    three_state_diag(n, H, psi, l, s, beta_dot(g, s->beta, x));
+    H[2 * (2 * n + 2)] += s->laser_domh;
+    const double re1 = cos(s->laser_k * x[0]), im1 = sin(s->laser_k * x[0]);
+    H[2 * (0 * n + 2)] = s->amp0 / 2 * re1;
+    H[2 * (0 * n + 2) + 1] = s->amp0 / 2 * im1;
+    H[2 * (2 * n + 0)] = s->amp0 / 2 * re1;
+    H[2 * (2 * n + 0) + 1] = -(s->amp0 / 2 * im1);
+    H[2 * (1 * n + 2)] = s->amp1 / 2 * re1;
+    H[2 * (1 * n + 2) + 1] = -(s->amp1 / 2 * im1);
+    H[2 * (2 * n + 1)] = s->amp1 / 2 * re1;
+    H[2 * (2 * n + 1) + 1] = s->amp1 / 2 * im1;
+    orc_herm_expm(n, H, s->dt, U);
+    apply_u(n, U, psi, l);
+  }
+}
+
+/* the reference's closed form restated term by term; O23 and O32 take
+ * their eta factors as written there (double_bragg.cpp:216-222) */
+void orc_double_bragg(const orc_grid *g, double **psi, const orc_rt *s, double t1, double F) {
+  long long np = orc_npts(g);
+  const double dt = s->dt;
+  if (F < 0) F = 0;
+#pragma omp parallel for schedule(static)
+  for (long long l = 0; l < np; l++) {
+    double x[3];
+    orc_get_x(g, l, x);
+    double *P1 = psi[0] + 2 * l, *P2 = psi[1] + 2 * l, *P3 = psi[2] + 2 * l;
+    double tmp1 = P1[0] * P1[0] + P1[1] * P1[1];
+    double tmp2 = P2[0] * P2[0] + P2[1] * P2[1];
+    double tmp3 = P3[0] * P3[0] + P3[1] * P3[1];
+    const double bx = beta_dot(g, s->beta, x);
+    const double V11 = s->gs[0] * tmp1 + s->gs[1] * tmp2 + s->gs[2] * tmp3 + bx;
+    const double V22 = s->gs[3] * tmp1 + s->gs[4] * tmp2 + s->gs[5] * tmp3 - s->deltaL[1] + bx;
+    double dw = s->laser_domh + s->chirp_rate0 * t1;
+    const double Op = F * s->amp0 * cos(s->laser_k * x[0] - dw * t1 - s->phase0 / 2);
+    dw = s->laser_domh - s->chirp_rate0 * t1;
+    const double Om = F * s->amp0 * cos(-s->laser_k * x[0] - dw * t1 - s->phase0 / 2);
+    double re1, im1;
+    if (Om == 0.0 && Op == 0.0) {
+      re1 = cos(-dt * V11);
+      im1 = sin(-dt * V11);
+      tmp1 = P1[0];
+      P1[0] = tmp1 * re1 - P1[1] * im1;
+      P1[1] = tmp1 * im1 + P1[1] * re1;
+      re1 = cos(-dt * V22);
+      im1 = sin(-dt * V22);
+      tmp1 = P2[0];
+      P2[0] = tmp1 * re1 - P2[1] * im1;
+      P2[1] = tmp1 * im1 + P2[1] * re1;
+      tmp1 = P3[0];
+      P3[0] = tmp1 * re1 - P3[1] * im1;
+      P3[1] = tmp1 * im1 + P3[1] * re1;
+      continue;
+    }
+    const double eta0 = cos(-0.5 * s->laser_dk * x[0]), eta1 = sin(-0.5 * s->laser_dk * x[0]);
+    tmp1 = sqrt((V11 - V22) * (V11 - V22) + 4.0 * (Op * Op + Om * Om));
+    const double E1 = 0.5 * (V11 + V22 + tmp1), E2 = 0.5 * (V11 + V22 - tmp1);
+    double G1[2], G2[2], G3[2];
+    tmp1 = 1.0 / fabs((V22 - E1) * (V22 - E1) + Op * Op + Om * Om);
+    G1[0] = tmp1 * cos(-dt * E1);
+    G1[1] = tmp1 * sin(-dt * E1);
+    tmp1 = 1.0 / fabs((V22 - E2) * (V22 - E2) + Op * Op + Om * Om);
+    G2[0] = tmp1 * cos(-dt * E2);
+    G2[1] = tmp1 * sin(-dt * E2);
+    tmp1 = 1.0 / fabs(Op * Op + Om * Om);
+    G3[0] = tmp1 * cos(-dt * V22);
+    G3[1] = tmp1 * sin(-dt * V22);
+    double O11[2], O12[2], O13[2], O21[2], O22[2], O23[2], O31[2], O32[2], O33[2];
+    tmp1 = (V22 - E2) * (V22 - E2);
+    tmp2 = (V22 - E1) * (V22 - E1);
+    for (int c = 0; c < 2; c++) O11[c] = tmp1 * G2[c] + tmp2 * G1[c];
+    const double w[2] = {(V22 - E2) * G2[0] + (V22 - E1) * G1[0], (V22 - E2) * G2[1] + (V22 - E1) * G1[1]};
+    for (int c = 0; c < 2; c++) {
+      O12[c] = -Op * w[c];
+      O13[c] = -Om * w[c];
+      O21[c] = O12[c];
+      O31[c] = O13[c];
+    }
+    /* eta * z for O12 and O31, conj(eta) * z for O21 and O13 */
+    double z0 = O12[0], z1 = O12[1];
+    O12[0] = eta0 * z0 - eta1 * z1;
+    O12[1] = eta1 * z0 + eta0 * z1;
+    z0 = O21[0];
+    z1 = O21[1];
+    O21[0] = eta0 * z0 + eta1 * z1;
+    O21[1] = eta0 * z1 - eta1 * z0;
+    z0 = O31[0];
+    z1 = O31[1];
+    O31[0] = eta0 * z0 - eta1 * z1;
+    O31[1] = eta1 * z0 + eta0 * z1;
+    z0 = O13[0];
+    z1 = O13[1];
+    O13[0] = eta0 * z0 + eta1 * z1;
+    O13[1] = eta0 * z1 - eta1 * z0;
+    const double pp = Op * Op, mm = Om * Om;
+    for (int c = 0; c < 2; c++) {
+      O22[c] = pp * (G2[c] + G1[c]) + mm * G3[c];
+      O33[c] = mm * (G2[c] + G1[c]) + pp * G3[c];
+      O23[c] = (Om * Op) * (G1[c] + G2[c] - G3[c]);
+      O32[c] = O23[c];
+    }
+    const double sq = eta0 * eta0 + eta1 * eta1, df = eta0 * eta0 - eta1 * eta1, cr2 = 2 * eta0 * eta1;
+    z0 = O23[0];
+    O23[0] = sq * z0 + cr2 * O23[1];
+    O23[1] = sq * O23[1] - cr2 * z0;
+    z0 = O32[0];
+    O32[0] = df * z0 - cr2 * O32[1];
+    O32[1] = df * O23[1] + cr2 * z0; /* O23[1] after its update, as written */
+    const double a1[2] = {P1[0], P1[1]}, a2[2] = {P2[0], P2[1]}, a3[2] = {P3[0], P3[1]};
+#define ORC_ROW(O1, O2, O3, D)                                                                              \
+  D[0] = (O1[0] * a1[0] - O1[1] * a1[1]) + (O2[0] * a2[0] - O2[1] * a2[1]) + (O3[0] * a3[0] - O3[1] * a3[1]); \
+  D[1] = (O1[0] * a1[1] + O1[1] * a1[0]) + (O2[0] * a2[1] + O2[1] * a2[0]) + (O3[0] * a3[1] + O3[1] * a3[0]);
+    ORC_ROW(O11, O12, O13, P1)
+    ORC_ROW(O21, O22, O23, P2)
+    ORC_ROW(O31, O32, O33, P3)
+#undef ORC_ROW
   }
 }
 

@@ -90,7 +90,16 @@ double orc_energy(const orc_grid *g, int ncomp, double **psi,
 /* ---- sequence block (CRT_Base::run_sequence / CRT_Base_IF::run_sequence) */
 typedef double (*orc_amp_fn)(double t, void *user);
 
-enum { ORC_FREEPROP = 0, ORC_FREEPROP_IF = 1, ORC_BRAGG_AD = 2, ORC_FREEPROP_IF2 = 3, ORC_BRAGG_DS = 4 };
+enum {
+  ORC_FREEPROP = 0,
+  ORC_FREEPROP_IF = 1,
+  ORC_BRAGG_AD = 2,
+  ORC_FREEPROP_IF2 = 3,
+  ORC_BRAGG_DS = 4,
+  ORC_DOUBLE_BRAGG = 5,
+  ORC_NUM_BRAGG = 6,
+  ORC_RAMAN = 7
+};
 
 typedef struct {
   orc_grid g;
@@ -119,6 +128,9 @@ typedef struct {
   double amp2, laser_k2, laser_domh2, laser_dk2, chirp_rate2, phase2;
   double seq_time;     /* sequence_item::time, start of the pulses */
   double duration[2];  /* sequence_item::duration[0..1] (pulse gating of bragg_ds) */
+  /* three-state steps: Amp[1] and the whole Beta vector */
+  double amp1;
+  double beta[3];
 } orc_rt;
 
 /* CRT_Base_2::Do_FT_Step_*: table 1 on components [0, ncomp/2), table 2 on
@@ -131,6 +143,19 @@ void orc_nl_if2_step(const orc_grid *g, int ncomp, double **psi, const double *g
 /* bragg_ds Bragg_single::Do_Bragg_ad (bragg_ds.cpp:112-300), four components;
  * t1 = Get_t() + dt/2, mode1/mode2 the pulse gates */
 void orc_bragg_ds(const orc_grid *g, double **psi, const orc_rt *s, double t1, int mode1, int mode2);
+
+/* U = exp(-i dt H) of a Hermitian n x n matrix (n <= 4; H and U row-major,
+ * interleaved complex): eigen-decomposition by Jacobi rotations in long
+ * double, each rotation on the largest remaining off-diagonal element */
+void orc_herm_expm(int n, const double *H, double dt, double *U);
+/* Bragg_double::Do_Double_Bragg_ad (double_bragg.cpp:108-249), three
+ * components; t1 = Get_t() + dt/2, F = Amplitude_at_time() before the clamp */
+void orc_double_bragg(const orc_grid *g, double **psi, const orc_rt *s, double t1, double F);
+/* CRT_Base_IF::Numerical_Bragg (CRT_Base_IF.h:426-527), ncomp 2 or 3;
+ * t = Get_t() before the step */
+void orc_num_bragg(const orc_grid *g, int ncomp, double **psi, const orc_rt *s, double t);
+/* CRT_Base_IF::Numerical_Raman (CRT_Base_IF.h:534-632), ncomp >= 3 */
+void orc_raman(const orc_grid *g, int ncomp, double **psi, const orc_rt *s);
 
 /* one outer iteration of the hot loop: half, (V, full) x (Nk-1), V, half
  * (CRT_Base.h:776-786, CRT_Base_IF.h:779-788) */

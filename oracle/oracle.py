@@ -55,7 +55,8 @@ class RT(C.Structure):
                 ("half_tab2", C.c_void_p), ("full_tab2", C.c_void_p),
                 ("beta2", C.c_double), ("amp2", C.c_double), ("laser_k2", C.c_double),
                 ("laser_domh2", C.c_double), ("laser_dk2", C.c_double), ("chirp_rate2", C.c_double),
-                ("phase2", C.c_double), ("seq_time", C.c_double), ("duration", C.c_double * 2)]
+                ("phase2", C.c_double), ("seq_time", C.c_double), ("duration", C.c_double * 2),
+                ("amp1", C.c_double), ("beta", C.c_double * 3)]
 
 
 class SOB(C.Structure):
@@ -71,6 +72,7 @@ class SOB(C.Structure):
 
 
 FREEPROP, FREEPROP_IF, BRAGG_AD, FREEPROP_IF2, BRAGG_DS = 0, 1, 2, 3, 4
+DOUBLE_BRAGG, NUM_BRAGG, RAMAN = 5, 6, 7
 
 _lib = None
 
@@ -95,6 +97,10 @@ def lib():
             "orc_ft_step2": (None, [G, I, P, P, P]),
             "orc_nl_if2_step": (None, [G, I, P, P, D, D, D, P]),
             "orc_bragg_ds": (None, [G, P, C.POINTER(RT), D, I, I]),
+            "orc_herm_expm": (None, [I, P, D, P]),
+            "orc_double_bragg": (None, [G, P, C.POINTER(RT), D, D]),
+            "orc_num_bragg": (None, [G, I, P, C.POINTER(RT), D]),
+            "orc_raman": (None, [G, I, P, C.POINTER(RT)]),
             "orc_particle_number": (D, [G, P]),
             "orc_expval_x": (None, [G, P, P]), "orc_expval_k": (None, [G, P, P]),
             "orc_rabi_integrals": (None, [G, P, P, I, D, P]),
@@ -341,8 +347,11 @@ class RTSolver:
         for i in range(len(dl)):
             s.deltaL[i] = float(dl[i])
         for k in ("beta0", "amp0", "laser_k", "laser_domh", "laser_dk", "chirp",
-                  "chirp_rate0", "phase0"):
+                  "chirp_rate0", "phase0", "amp1"):
             setattr(s, k, float(ifp.pop(k, 0.0)))
+        beta = ifp.pop("beta", (0.0, 0.0, 0.0))
+        for a in range(len(beta)):
+            s.beta[a] = float(beta[a])
         if ifp:
             raise TypeError(f"unknown parameters {sorted(ifp)}")
         self._amp_cb = AMP_FN(lambda t, u: float(amp(t))) if amp else AMP_FN()
@@ -443,4 +452,30 @@ def bragg_ds(g, psis, solver, t1, mode1, mode2):
     RTSolver built with species2=...; returns new arrays."""
     out = [_c128(p).copy() for p in psis]
     lib().orc_bragg_ds(C.byref(g), _ptr_array(out), C.byref(solver.s), float(t1), int(mode1), int(mode2))
+    return out
+
+
+def herm_expm(H, dt):
+    """exp(-i dt H) of a Hermitian n x n matrix (n <= 4), by the oracle's
+    long-double Jacobi eigen-decomposition."""
+    H = np.ascontiguousarray(H, dtype=np.complex128)
+    n = H.shape[0]
+    U = np.zeros((n, n), np.complex128)
+    lib().orc_herm_expm(n, _ptr(H), float(dt), _ptr(U))
+    return U
+
+
+def three_state_step(kind, g, psis, solver, t, F=1.0):
+    """One three-state V step (DOUBLE_BRAGG at t1 = t + dt/2 with amplitude
+    F, NUM_BRAGG at Get_t() = t, RAMAN) with the parameters of an RTSolver;
+    returns new arrays."""
+    out = [_c128(p).copy() for p in psis]
+    if kind == DOUBLE_BRAGG:
+        lib().orc_double_bragg(C.byref(g), _ptr_array(out), C.byref(solver.s), float(t + 0.5 * solver.s.dt), float(F))
+    elif kind == NUM_BRAGG:
+        lib().orc_num_bragg(C.byref(g), len(out), _ptr_array(out), C.byref(solver.s), float(t))
+    elif kind == RAMAN:
+        lib().orc_raman(C.byref(g), len(out), _ptr_array(out), C.byref(solver.s))
+    else:
+        raise ValueError(kind)
     return out

@@ -516,3 +516,201 @@ def test_bragg_ds_block_with_an_empty_second_species_is_single_bragg(orc):
     err = np.linalg.norm(np.concatenate([ds.psis[0] - single.psis[0], ds.psis[1] - single.psis[1]]))
     assert err / np.linalg.norm(np.concatenate(single.psis)) < 1e-12
     assert not np.any(ds.psis[2]) and not np.any(ds.psis[3])
+
+
+# ------------------------------------------------------------ three states (f3)
+
+def _rand_states(shape, n, seed):
+    rng = np.random.default_rng(seed)
+    return [rng.standard_normal(shape) + 1j * rng.standard_normal(shape) for _ in range(n)]
+
+
+@pytest.mark.parametrize("n", [2, 3, 4])
+def test_herm_expm_matches_scipy_also_when_degenerate(orc, n):
+    rng = np.random.default_rng(40 + n)
+    for trial in range(60):
+        A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+        H = (A + A.conj().T) * rng.uniform(0.1, 80.0)
+        if trial % 3 == 0:   # a repeated eigenvalue
+            w, V = np.linalg.eigh(H)
+            w[-1] = w[0]
+            H = (V * w) @ V.conj().T
+            H = (H + H.conj().T) / 2
+        if trial % 7 == 0:   # already diagonal
+            H = np.diag(np.diag(H).real).astype(np.complex128)
+        dt = rng.uniform(1e-3, 1.5)
+        U = orc.herm_expm(H, dt)
+        R = scipy.linalg.expm(-1j * dt * H)
+        scale = max(1.0, np.abs(H).max() * dt)
+        assert np.abs(U - R).max() < 1e-14 * scale, (trial, np.abs(U - R).max())
+        assert np.abs(U @ U.conj().T - np.eye(n)).max() < 1e-14
+
+
+def _three_state_solver(orc, g, psis, gs, **kw):
+    return orc.RTSolver(g, psis, gs, [0.5] * g.dim, dt=kw.pop("dt", 0.05), **kw)
+
+
+def test_num_bragg_two_states_is_the_analytic_bragg_step(orc):
+    # the numerical Bragg step with Amp_1 = (A, A), no chirp and no phase
+    # couples the two states by A cos(k x - domh t): the Hamiltonian of
+    # Do_Bragg_ad with laser_dk = 0, whose closed form is independent code
+    rng = np.random.default_rng(7)
+    shape = (48,)
+    g = orc.make_grid(shape, L=12.0)
+    p1, p2 = _rand_states(shape, 2, 8)
+    gs = (1.0, 0.3, 0.25, 0.8)
+    t, dt = 0.31, 0.07
+    P = orc.bragg_params(dt=dt, t=t, F=1.0, amp0=2.5, laser_k=1.7, laser_domh=0.9, beta0=0.05, deltaL1=-1.2, gs=gs)
+    e1, e2 = orc.bragg_ad(g, p1, p2, P)
+    S = _three_state_solver(orc, g, [p1, p2], list(gs), dt=dt, amp0=2.5, amp1=2.5, laser_k=1.7, laser_domh=0.9,
+                            beta=[0.05], deltaL=[0.0, -1.2])
+    o1, o2 = orc.three_state_step(orc.NUM_BRAGG, g, [p1, p2], S, t + 0.5 * dt)
+    scale = np.abs(np.concatenate([p1, p2])).max()
+    assert np.abs(o1 - e1).max() < 1e-13 * scale and np.abs(o2 - e2).max() < 1e-13 * scale
+
+
+def test_num_bragg_three_states_matches_expm_and_is_unitary(orc):
+    shape = (10, 3)
+    g = orc.make_grid(shape, L=[6.0, 2.0])
+    psis = _rand_states(shape, 3, 9)
+    gs = list(np.random.default_rng(10).uniform(0.1, 1.2, 9))
+    kw = dict(amp0=2.0, amp1=0.7, laser_k=1.3, laser_domh=0.8, chirp_rate0=0.3, phase0=0.5,
+              beta=[0.04, -0.02], deltaL=[0.1, -0.9, 0.6])
+    S = _three_state_solver(orc, g, psis, gs, dt=0.09, **kw)
+    t = 0.42
+    out = orc.three_state_step(orc.NUM_BRAGG, g, psis, S, t)
+    x0, x1 = [np.broadcast_to(c, shape) for c in orc.coords_x(g)]
+    lk, cr = [1.3, -1.3], [0.3, -0.3]
+    for idx in np.ndindex(shape):
+        v = np.array([p[idx] for p in psis])
+        n = np.abs(v) ** 2
+        H = np.zeros((3, 3), complex)
+        for i in range(3):
+            H[i, i] = sum(gs[j + 3 * i] * n[j] for j in range(3)) + 0.04 * x0[idx] - 0.02 * x1[idx] - kw["deltaL"][i]
+        for i in range(2):
+            th = (-0.8 + cr[i] * t) * t + lk[i] * x0[idx] - 0.25
+            H[i + 1, 0] = (2.0 + 0.7) / 2 * np.cos(th) + 1j * (2.0 - 0.7) / 2 * np.sin(th)
+            H[0, i + 1] = np.conj(H[i + 1, 0])
+        ref = scipy.linalg.expm(-1j * 0.09 * H) @ v
+        got = np.array([o[idx] for o in out])
+        assert np.abs(got - ref).max() < 1e-13 * np.abs(v).max()
+        assert abs(np.sum(np.abs(got) ** 2) - np.sum(n)) < 1e-13 * np.sum(n)
+
+
+def test_raman_matches_expm_and_reduces_to_phases(orc):
+    shape = (16,)
+    g = orc.make_grid(shape, L=8.0)
+    psis = _rand_states(shape, 3, 12)
+    gs = list(np.random.default_rng(13).uniform(0.1, 1.0, 9))
+    kw = dict(amp0=3.0, amp1=1.1, laser_k=2.2, laser_domh=4.0, beta=[0.06], deltaL=[0.0, -2.0, 1.5])
+    S = _three_state_solver(orc, g, psis, gs, dt=0.05, **kw)
+    out = orc.three_state_step(orc.RAMAN, g, psis, S, 0.0)
+    (x,) = orc.coords_x(g)
+    for l in range(shape[0]):
+        v = np.array([p[l] for p in psis])
+        n = np.abs(v) ** 2
+        H = np.diag([sum(gs[j + 3 * i] * n[j] for j in range(3)) + 0.06 * x[l] - kw["deltaL"][i]
+                     for i in range(3)]).astype(complex)
+        H[2, 2] += 4.0
+        e = np.exp(1j * 2.2 * x[l])
+        H[0, 2], H[2, 0] = 1.5 * e, 1.5 * np.conj(e)
+        H[1, 2], H[2, 1] = 0.55 * np.conj(e), 0.55 * e
+        ref = scipy.linalg.expm(-1j * 0.05 * H) @ v
+        assert np.abs(np.array([o[l] for o in out]) - ref).max() < 1e-13 * np.abs(v).max()
+    # no light: each state only picks up its phase
+    S0 = _three_state_solver(orc, g, psis, gs, dt=0.05, laser_domh=4.0, beta=[0.06], deltaL=[0.0, -2.0, 1.5])
+    out0 = orc.three_state_step(orc.RAMAN, g, psis, S0, 0.0)
+    n = [np.abs(p) ** 2 for p in psis]
+    for i in range(3):
+        V = sum(gs[j + 3 * i] * n[j] for j in range(3)) + 0.06 * x - kw["deltaL"][i] + (4.0 if i == 2 else 0.0)
+        assert np.abs(out0[i] - psis[i] * np.exp(-1j * 0.05 * V)).max() < 1e-14 * np.abs(psis[i]).max()
+
+
+def _dbragg_columns(orc, g, S, t, F):
+    """The 3x3 matrix of the double Bragg step at every point (g = 0, so the
+    step is linear): column j is the step applied to the unit state j."""
+    shape = tuple(int(v) for v in orc.grid_shape(g))
+    cols = []
+    for j in range(3):
+        e = [np.full(shape, 1.0 + 0j) if c == j else np.zeros(shape, complex) for c in range(3)]
+        cols.append(orc.three_state_step(orc.DOUBLE_BRAGG, g, e, S, t, F))
+    return cols   # cols[j][i] = O_ij
+
+
+def test_double_bragg_closed_form_is_the_exponential_without_dk(orc):
+    # laser_dk = 0: the reference's O is exp(-i dt H) for
+    # H = [[V11, Op, Om], [Op, V22, 0], [Om, 0, V22]]
+    shape = (24,)
+    g = orc.make_grid(shape, L=9.0)
+    kw = dict(amp0=2.4, laser_k=1.1, laser_domh=0.6, chirp_rate0=0.2, phase0=0.3, beta=[0.07],
+              deltaL=[0.0, -1.4, 0.0])
+    S = _three_state_solver(orc, g, [np.zeros(shape, complex)] * 3, [0.0] * 9, dt=0.08, **kw)
+    t, F = 0.55, 0.9
+    cols = _dbragg_columns(orc, g, S, t, F)
+    (x,) = orc.coords_x(g)
+    t1 = t + 0.04
+    for l in range(shape[0]):
+        V11 = 0.07 * x[l]
+        V22 = 1.4 + 0.07 * x[l]          # - DeltaL[1] with DeltaL[1] = -1.4
+        Op = F * 2.4 * np.cos(1.1 * x[l] - (0.6 + 0.2 * t1) * t1 - 0.15)
+        Om = F * 2.4 * np.cos(-1.1 * x[l] - (0.6 - 0.2 * t1) * t1 - 0.15)
+        H = np.array([[V11, Op, Om], [Op, V22, 0], [Om, 0, V22]], complex)
+        U = scipy.linalg.expm(-1j * 0.08 * H)
+        O = np.array([[cols[j][i][l] for j in range(3)] for i in range(3)])
+        assert np.abs(O - U).max() < 1e-12, (l, np.abs(O - U).max())
+
+
+def test_double_bragg_dk_phases_and_clamped_amplitude(orc):
+    # laser_dk != 0 multiplies O12, O31 by eta and O21, O13 by conj(eta),
+    # eta = exp(-i dk x / 2); the diagonal is unchanged.  O23 and O32 take
+    # the reference's own factors (double_bragg.cpp:216-222), restated here
+    shape = (20,)
+    g = orc.make_grid(shape, L=7.0)
+    base = dict(amp0=1.9, laser_k=0.9, laser_domh=0.4, chirp_rate0=0.1, phase0=0.2, beta=[0.03],
+                deltaL=[0.0, -0.8, 0.0])
+    z = [np.zeros(shape, complex)] * 3
+    S0 = _three_state_solver(orc, g, z, [0.0] * 9, dt=0.06, laser_dk=0.0, **base)
+    S1 = _three_state_solver(orc, g, z, [0.0] * 9, dt=0.06, laser_dk=0.7, **base)
+    t, F = 0.2, 1.3
+    c0, c1 = _dbragg_columns(orc, g, S0, t, F), _dbragg_columns(orc, g, S1, t, F)
+    (x,) = orc.coords_x(g)
+    eta = np.exp(-0.5j * 0.7 * x)
+    O0 = lambda i, j: c0[j][i]
+    O1 = lambda i, j: c1[j][i]
+    for i in range(3):
+        assert np.abs(O1(i, i) - O0(i, i)).max() < 1e-14
+    assert np.abs(O1(0, 1) - eta * O0(0, 1)).max() < 1e-14
+    assert np.abs(O1(2, 0) - eta * O0(2, 0)).max() < 1e-14
+    assert np.abs(O1(1, 0) - np.conj(eta) * O0(1, 0)).max() < 1e-14
+    assert np.abs(O1(0, 2) - np.conj(eta) * O0(0, 2)).max() < 1e-14
+    e0, e1 = eta.real, eta.imag
+    z23 = O0(1, 2)
+    o23 = (e0 ** 2 + e1 ** 2) * z23.real + 2 * e0 * e1 * z23.imag + 1j * ((e0 ** 2 + e1 ** 2) * z23.imag
+                                                                         - 2 * e0 * e1 * z23.real)
+    o32 = (e0 ** 2 - e1 ** 2) * z23.real - 2 * e0 * e1 * z23.imag + 1j * ((e0 ** 2 - e1 ** 2) * o23.imag
+                                                                         + 2 * e0 * e1 * z23.real)
+    assert np.abs(O1(1, 2) - o23).max() < 1e-14 and np.abs(O1(2, 1) - o32).max() < 1e-14
+    # F < 0 is clamped to 0: no light, pure phases (V22 on states 2 and 3)
+    cn = _dbragg_columns(orc, g, S1, t, -0.5)
+    assert np.abs(cn[0][0] - np.exp(-1j * 0.06 * 0.03 * x)).max() < 1e-15
+    assert np.abs(cn[2][2] - np.exp(-1j * 0.06 * (0.8 + 0.03 * x))).max() < 1e-15
+    assert not np.any(cn[0][1]) and not np.any(cn[1][2])
+
+
+def test_three_state_blocks_conserve_the_norm(orc):
+    n, L = 64, 16.0
+    g = orc.make_grid((n,), L=L)
+    (x,) = orc.coords_x(g)
+    p = np.exp(-x ** 2 / 4).astype(np.complex128)
+    p /= np.sqrt(orc.particle_number(g, p))
+    psis = [p, 0.3 * p, 0.2j * p]
+    gs = [1.0, 0.9, 0.9, 0.9, 1.1, 0.8, 0.9, 0.8, 1.2]
+    N0 = sum(orc.particle_number(g, q) for q in psis)
+    for kind, kw in ((orc.DOUBLE_BRAGG, dict(amp0=3.0, laser_k=1.2, laser_domh=0.5, deltaL=[0, -0.4, 0])),
+                     (orc.NUM_BRAGG, dict(amp0=3.0, amp1=2.0, laser_k=1.2, laser_domh=0.5, deltaL=[0, -0.4, 0.2])),
+                     (orc.RAMAN, dict(amp0=3.0, amp1=2.0, laser_k=1.2, laser_domh=2.5, deltaL=[0, -0.4, 0.2]))):
+        S = orc.RTSolver(g, psis, gs, [0.5], dt=2e-3, amp=lambda t: 1.0, **kw)
+        for _ in range(3):
+            S.block(kind, 10)
+        N = sum(orc.particle_number(g, q) for q in S.psis)
+        assert abs(N - N0) < 1e-12, (kind, N - N0)
