@@ -92,7 +92,7 @@ class CRT_Base : public CRT_shared {
   // uploads g, alpha and (if initialised) the potentials
   void sync_params();
   bool device_block_possible(StepFunction step) const;
-  void run_device_block(StepFunction step, sequence_item& seq, int Nk);
+  virtual void run_device_block(StepFunction step, sequence_item& seq, int Nk);
   static double amp_callback(double t1, void* self);
   virtual double amplitude_at(double t1) { return 1.0; }
   virtual bool has_amplitude() const { return false; }

@@ -10,10 +10,11 @@
 // (an output of the same "%.3f" name) has replaced a file; then the file is
 // read, which is what the reference's reload sees.
 //
-// The GSL diagonalisers registered as "bragg" and "raman"
-// (CRT_Base_IF.h:427-632, 3-state couplings) are not part of this build:
-// selecting them throws.  phase[0] starts at 0 (the reference leaves it
-// uninitialised unless chirp_mode == 1).
+// "bragg" and "raman" (CRT_Base_IF.h:426-632: the light couplings of two or
+// three states exponentiated through GSL's Hermitian eigensolver) run as
+// device steps whose per-point exponential diagonalises the same matrix.
+// phase[0] starts at 0 (the reference leaves it uninitialised unless
+// chirp_mode == 1).
 #ifndef ATUS_AMD_CRT_BASE_IF_H
 #define ATUS_AMD_CRT_BASE_IF_H
 
@@ -48,6 +49,8 @@ class CRT_Base_IF : public CRT_Base<T, dim, no_int_states> {
   static void Numerical_Bragg_Wrapper(void*, sequence_item&);
   static void Numerical_Raman_Wrapper(void*, sequence_item&);
   void Do_NL_Step();
+  void Numerical_Bragg();
+  void Numerical_Raman();
 
   void UpdateParams();
   void Output_rabi_freq_list(std::string, const long long);
@@ -101,6 +104,8 @@ CRT_Base_IF<T, dim, no_int_states>::CRT_Base_IF(ParameterHandler* params) : CRT_
   this->m_map_stepfcts["raman"] = &Numerical_Raman_Wrapper;
   this->m_device_steps.erase(&CRT_Base<T, dim, no_int_states>::Do_NL_Step_Wrapper);
   this->m_device_steps[&Do_NL_Step_Wrapper] = ATUS_STEP_FREEPROP_IF;
+  if (no_int_states == 2 || no_int_states == 3) this->m_device_steps[&Numerical_Bragg_Wrapper] = ATUS_STEP_NUM_BRAGG;
+  if (no_int_states >= 3) this->m_device_steps[&Numerical_Raman_Wrapper] = ATUS_STEP_RAMAN;
   phase.fill(0.0);
   chirp_rate.fill(0.0);
   Amp.fill(0.0);
@@ -150,6 +155,8 @@ atus_laser CRT_Base_IF<T, dim, no_int_states>::laser_params() const {
   L.phase0 = phase[0];
   L.beta0 = beta[0];
   for (int i = 0; i < no_int_states && i < 4; ++i) L.deltaL[i] = DeltaL[i];
+  L.amp1 = Amp[1];
+  for (int a = 0; a < dim; ++a) L.beta[a] = beta[a];
   return L;
 }
 
@@ -165,12 +172,32 @@ void CRT_Base_IF<T, dim, no_int_states>::Do_NL_Step_Wrapper(void* ptr, sequence_
   static_cast<CRT_Base_IF*>(ptr)->Do_NL_Step();
 }
 template <class T, int dim, int no_int_states>
-void CRT_Base_IF<T, dim, no_int_states>::Numerical_Bragg_Wrapper(void*, sequence_item&) {
-  throw std::string("the numerical Bragg step (GSL eigensolver) is not part of this build");
+void CRT_Base_IF<T, dim, no_int_states>::Numerical_Bragg_Wrapper(void* ptr, sequence_item&) {
+  static_cast<CRT_Base_IF*>(ptr)->Numerical_Bragg();
 }
 template <class T, int dim, int no_int_states>
-void CRT_Base_IF<T, dim, no_int_states>::Numerical_Raman_Wrapper(void*, sequence_item&) {
-  throw std::string("the numerical Raman step (GSL eigensolver) is not part of this build");
+void CRT_Base_IF<T, dim, no_int_states>::Numerical_Raman_Wrapper(void* ptr, sequence_item&) {
+  static_cast<CRT_Base_IF*>(ptr)->Numerical_Raman();
+}
+
+// CRT_Base_IF::Numerical_Bragg (CRT_Base_IF.h:426-527): state 0 coupled to
+// states 1 and 2 by lasers of wave numbers +-laser_k and chirps
+// +-chirp_rate[0] at t = Get_t(); two or three states
+template <class T, int dim, int no_int_states>
+void CRT_Base_IF<T, dim, no_int_states>::Numerical_Bragg() {
+  laser_k[1] = -laser_k[0];
+  chirp_rate[1] = -chirp_rate[0];
+  const atus_laser L = laser_params();
+  atus_host::check(atus_v_step(m_set->device(), ATUS_STEP_NUM_BRAGG, m_header.dt, m_header.t, 1.0, &L),
+                   "atus_v_step");
+}
+
+// CRT_Base_IF::Numerical_Raman (CRT_Base_IF.h:534-632): states 0 and 1
+// coupled to state 2 (three or four states)
+template <class T, int dim, int no_int_states>
+void CRT_Base_IF<T, dim, no_int_states>::Numerical_Raman() {
+  const atus_laser L = laser_params();
+  atus_host::check(atus_v_step(m_set->device(), ATUS_STEP_RAMAN, m_header.dt, m_header.t, 1.0, &L), "atus_v_step");
 }
 
 // CRT_Base_IF::Do_NL_Step (CRT_Base_IF.h:381-419)

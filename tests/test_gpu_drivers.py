@@ -346,3 +346,72 @@ def test_bragg_chirp_scan(orc, tmp_path, case):
             elif name.endswith(".bin") and not name.startswith("s"):
                 a, b = read_bin(tmp_path / name)[1], read_bin(seq_dir / name)[1]
                 assert np.linalg.norm(a - b) <= 1e-12 * np.linalg.norm(b), name
+
+
+def _three_state_inputs(orc, tmp_path, n, L, ncomp, dt):
+    g = orc.make_grid((n,), L=L)
+    (x,) = orc.coords_x(g)
+    p = np.exp(-x ** 2 / 18).astype(np.complex128)
+    p /= np.sqrt(orc.particle_number(g, p))
+    rng = np.random.default_rng(21)
+    psis = [p] + [1e-3 * (rng.standard_normal(n) + 1j * rng.standard_normal(n)) for _ in range(ncomp - 1)]
+    for c in range(ncomp):
+        write_bin(tmp_path / f"q{c + 1}.bin", psis[c], [L], dt=dt)
+    files = "".join(f"<FILENAME{'' if c == 0 else '_' + str(c + 1)}>q{c + 1}.bin</FILENAME{'' if c == 0 else '_' + str(c + 1)}>"
+                    for c in range(ncomp))
+    return g, psis, files
+
+
+@pytest.mark.parametrize("driver", ["double_bragg", "raman", "bragg_numerical"])
+def test_three_state_drivers(orc, tmp_path, driver):
+    # double_bragg ("bragg_ad": closed-form 3x3 step), raman ("raman") and
+    # bragg with the numerical two-state step ("bragg") from XML and .bin
+    # inputs, against the oracle
+    n, L, dt, Nk = 1024, 40.0, 1e-3, 25
+    ncomp = 2 if driver == "bragg_numerical" else 3
+    g, psis, files = _three_state_inputs(orc, tmp_path, n, L, ncomp, dt)
+    gs = [20.0, 19.0, 18.0, 19.0, 21.0, 18.5, 18.0, 18.5, 20.5][:ncomp * ncomp] if ncomp == 3 else [20.0, 19.0, 19.0, 21.0]
+    gsx = "".join(f"<GS_{i + 1}>{','.join(str(v) for v in gs[ncomp * i:ncomp * (i + 1)])}</GS_{i + 1}>"
+                  for i in range(ncomp))
+    k = 4 * 2 * np.pi / L
+    dl = {"double_bragg": [0.0, -0.5, 0.0], "raman": [0.0, -3.0, 2.0], "bragg_numerical": [0.0, -0.5]}[driver]
+    domh = {"double_bragg": 0.8, "raman": 5.0, "bragg_numerical": 0.8}[driver]
+    seq = {"double_bragg": "bragg_ad", "raman": "raman", "bragg_numerical": "bragg"}[driver]
+    amp_t = "<AMP_T>exp(-0.5*((t-0.05)/0.02)^2)</AMP_T>" if driver == "double_bragg" else ""
+    (tmp_path / "t.xml").write_text(f"""<SIMULATION><DIM>1</DIM>{files}{amp_t}
+      <CONSTANTS><laser_k>{k!r}</laser_k><laser_domh>{domh}</laser_domh><laser_dk>0.05</laser_dk>
+        <rabi_threshold>0.3</rabi_threshold><chirp>0</chirp></CONSTANTS>
+      <VCONSTANTS>{gsx}<Alpha_1>0.5</Alpha_1><Beta>0.02</Beta>
+        <Delta_L>{','.join(str(v) for v in dl)}</Delta_L><Amp_1>8,5</Amp_1></VCONSTANTS>
+      <ALGORITHM><DT>{dt!r}</DT></ALGORITHM>
+      <SEQUENCE><{seq} dt="{dt!r}" Nk="{Nk}" rabi_output_freq="each" output_freq="last">0.1</{seq}></SEQUENCE>
+      </SIMULATION>""")
+    exe = "bragg" if driver == "bragg_numerical" else driver
+    r = subprocess.run([tool(exe), "t.xml"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    Na = int(0.1 / dt) // Nk
+    okind = {"double_bragg": orc.DOUBLE_BRAGG, "raman": orc.RAMAN, "bragg_numerical": orc.NUM_BRAGG}[driver]
+    amp = (lambda t: float(np.exp(-0.5 * ((t - 0.05) / 0.02) ** 2))) if driver == "double_bragg" else None
+    lp = dict(amp0=8.0, amp1=5.0, laser_k=k, laser_domh=domh, laser_dk=0.05, beta0=0.02, beta=[0.02], deltaL=dl)
+    S = orc.RTSolver(g, psis, gs, [0.5], dt=dt, amp=amp, **lp)
+    pert = [p * (1 + 2e-16 * np.random.default_rng(99).standard_normal(n)) for p in psis]
+    Sp = orc.RTSolver(g, pert, gs, [0.5], dt=dt, amp=amp, **lp)
+    rabi = []
+    for _ in range(Na):
+        S.block(okind, Nk)
+        Sp.block(okind, Nk)
+        rabi.append(orc.rabi_integrals(g, S.psis[0], [(0.0,), (2 * k,), (-2 * k,)], 0.3)[0])
+    tol = max(1e-11, 20 * rel_l2(np.concatenate(Sp.psis), np.concatenate(S.psis)))
+    out = np.concatenate([read_bin(tmp_path / f"{S.t:.3f}_{c}.bin")[1] for c in range(1, ncomp + 1)])
+    assert rel_l2(out, np.concatenate(S.psis)) < tol
+    body = (tmp_path / "Rabi_1_0.txt").read_text().splitlines()[1:]
+    vals = np.array([[float(v) for v in ln.split("\t") if v][1:] for ln in body])
+    assert np.allclose(vals, np.array(rabi), rtol=1e-5, atol=1e-9)
+    if driver == "double_bragg":
+        # one amplitude line per V step (6 digits): AMP_T at t1 = Get_t() +
+        # dt/2, which is m*dt for the V step m after the half kinetic step
+        lines = [ln for ln in r.stdout.splitlines()
+                 if ln and not ln.startswith(("FYI", "t =", "N[")) and "Amp" not in ln]
+        assert len(lines) == Na * Nk
+        t1 = dt * (1 + np.arange(Na * Nk))
+        assert np.allclose([float(v) for v in lines], np.exp(-0.5 * ((t1 - 0.05) / 0.02) ** 2), rtol=1e-5, atol=1e-7)

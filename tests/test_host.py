@@ -193,7 +193,7 @@ def test_driver_errors_exit_nonzero(tmp_path):
     xml = """<SIMULATION><DIM>1</DIM><FILENAME>does_not_exist.bin</FILENAME>
       <VCONSTANTS><GS_1>1</GS_1><Alpha_1>0.5</Alpha_1></VCONSTANTS></SIMULATION>"""
     (tmp_path / "e.xml").write_text(xml)
-    for drv in ("rt_solver", "bragg", "bragg_ds"):
+    for drv in ("rt_solver", "bragg", "bragg_ds", "double_bragg", "raman"):
         r = subprocess.run([tool(drv), str(tmp_path / "e.xml")], capture_output=True, text=True, cwd=tmp_path)
         assert r.returncode != 0 and "Could not open file does_not_exist.bin" in r.stdout, drv
         r = subprocess.run([tool(drv)], capture_output=True, text=True)
