@@ -1035,6 +1035,70 @@ int atus_expval_k(atus_ctx* c, int comp, double out[3]) {
   return ATUS_OK;
 }
 
+int atus_get_moments(atus_ctx* c, int comp, atus_moments* out) {
+  ATUS_TRY(check_ctx(c));
+  ATUS_TRY(check_comp(c, comp));
+  if (!out) return fail(ATUS_ERR_ARG, "null output");
+  *out = atus_moments{};
+  const double2* p = c->psi + comp * c->npts;
+  const int D = c->dim;
+  double s1[5];
+  ATUS_TRY(reduce(c, RedMom1{p, c->grid()}, c->npts, s1));
+  if (!(s1[0] > 0.0)) return fail(ATUS_ERR_NUMERIC, "the state is zero");
+  // PN, then each moment scaled by ar / PN (ana_expval.cpp:337-341)
+  const double PN = c->ar() * s1[0];
+  out->N = PN;
+  out->phase = s1[4] / s1[0];
+  RedMom2<false> f2{p, c->grid(), {0, 0, 0}};
+  for (int a = 0; a < D; ++a) {
+    out->x[a] = s1[1 + a] * (c->ar() / PN);
+    f2.mu[a] = out->x[a];
+  }
+  double s2[3];
+  ATUS_TRY(reduce(c, f2, c->npts, s2));
+  for (int a = 0; a < D; ++a) out->x2[a] = s2[a] * (c->ar() / PN);
+  // momentum moments of the transformed state (a copy), still over PN
+  ATUS_TRY(ensure_scratch(c));
+  ATUS_TRY(full_ft(c, -1, p, c->scratch, 1));
+  double sk[3];
+  ATUS_TRY(reduce(c, RedK{c->scratch, c->grid()}, c->npts, sk));
+  RedMom2<true> fk{c->scratch, c->grid(), {0, 0, 0}};
+  for (int a = 0; a < D; ++a) {
+    out->k[a] = sk[a] * (c->ar_k() / PN);
+    fk.mu[a] = out->k[a];
+  }
+  double sk2[3];
+  ATUS_TRY(reduce(c, fk, c->npts, sk2));
+  for (int a = 0; a < D; ++a) out->k2[a] = sk2[a] * (c->ar_k() / PN);
+  return ATUS_OK;
+}
+
+int atus_inflate(atus_ctx* dst, atus_ctx* src, int fak, int fix) {
+  ATUS_TRY(check_ctx(dst));
+  ATUS_TRY(check_ctx(src));
+  if (dst->device != src->device) return fail(ATUS_ERR_ARG, "contexts on different devices");
+  if (fak < 1) return fail(ATUS_ERR_ARG, "the inflation factor must be >= 1");
+  if (dst->dim != src->dim || dst->ncomp != src->ncomp)
+    return fail(ATUS_ERR_ARG, "contexts of different dimension or component count");
+  for (int a = 0; a < src->dim; ++a) {
+    const long long want = fix ? src->n[a] : (long long)src->n[a] * fak;
+    if (dst->n[a] != want)
+      return fail(ATUS_ERR_ARG, "destination axis " + std::to_string(a) + " has " + std::to_string(dst->n[a]) +
+                                    " points, the inflation needs " + std::to_string(want));
+  }
+  HIP_TRY(hipStreamSynchronize(src->stream));
+  const dim3 grid(ew_blocks(dst->npts), (unsigned)dst->ncomp), block(256);
+  if (fix)
+    hipLaunchKernelGGL((k_inflate<true>), grid, block, 0, dst->stream, src->psi, dst->psi, src->grid(), dst->grid(), fak,
+                       src->npts, dst->npts);
+  else
+    hipLaunchKernelGGL((k_inflate<false>), grid, block, 0, dst->stream, src->psi, dst->psi, src->grid(), dst->grid(),
+                       fak, src->npts, dst->npts);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(dst->stream));
+  return ATUS_OK;
+}
+
 int atus_rabi_integrals(atus_ctx* c, int comp, const double* k0, int n, double threshold, double* out) {
   ATUS_TRY(check_ctx(c));
   ATUS_TRY(check_comp(c, comp));

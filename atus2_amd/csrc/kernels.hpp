@@ -1144,6 +1144,51 @@ struct RedK {
   }
 };
 
+// ana_expval (ana_expval.cpp:50-107): sum |psi|^2, sum x_a |psi|^2 and
+// sum atan2(Im, Re) |psi|^2
+struct RedMom1 {
+  static constexpr int NV = 5;
+  const double2* psi;
+  Grid gr;
+  __device__ void operator()(long long l, double (&acc)[5]) const {
+    const double2 a = psi[l];
+    const double d = a.x * a.x + a.y * a.y;
+    const long long n12 = (long long)gr.n[1] * gr.n[2];
+    const int i0 = (int)(l / n12);
+    const int i1 = (int)((l - i0 * n12) / gr.n[2]);
+    const int i2 = (int)(l - i0 * n12 - (long long)i1 * gr.n[2]);
+    acc[0] += d;
+    acc[1] += ((double)(i0 - gr.n[0] / 2) * gr.dx[0]) * d;
+    acc[2] += ((double)(i1 - gr.n[1] / 2) * gr.dx[1]) * d;
+    acc[3] += ((double)(i2 - gr.n[2] / 2) * gr.dx[2]) * d;
+    acc[4] += atan2(a.y, a.x) * d;
+  }
+};
+
+// sum (x_a - mu_a)^2 |psi|^2, or of k_a for a transformed state
+// (ana_expval.cpp:109-185, Expval_2nd_moment / _Momentum)
+template <bool KSPACE>
+struct RedMom2 {
+  static constexpr int NV = 3;
+  const double2* psi;
+  Grid gr;
+  double mu[3];
+  __device__ void operator()(long long l, double (&acc)[3]) const {
+    const double2 a = psi[l];
+    const double d = a.x * a.x + a.y * a.y;
+    const long long n12 = (long long)gr.n[1] * gr.n[2];
+    const int i0 = (int)(l / n12);
+    const int i1 = (int)((l - i0 * n12) / gr.n[2]);
+    const int i2 = (int)(l - i0 * n12 - (long long)i1 * gr.n[2]);
+    const int id[3] = {i0, i1, i2};
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+      const double c = KSPACE ? kval(id[ax], gr.n[ax], gr.dk[ax]) : (double)(id[ax] - gr.n[ax] / 2) * gr.dx[ax];
+      acc[ax] += (c - mu[ax]) * (c - mu[ax]) * d;
+    }
+  }
+};
+
 // Rabi momentum bins (CRT_Base_IF::compute_rabi_integrals, CRT_Base_IF.h:
 // 240-256), including the slot counter that only advances on a miss
 struct RedRabi {

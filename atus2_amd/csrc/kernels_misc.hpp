@@ -41,6 +41,45 @@ __global__ __launch_bounds__(256) void k_mul_phase(double2* __restrict__ psi, lo
   }
 }
 
+// inflate_domain (inflate_domain.cpp:41-223) on the device, one thread per
+// destination point of one component (blockIdx.y): FIX == false pads the
+// source grid, fak times smaller per axis, with zeros around the centre
+// (offset (n_dst - n_src)/2); FIX == true keeps the number of points and
+// places every fak-th source point around the centre (offset
+// n*(fak-1)/2/fak); everything else is zero
+template <bool FIX>
+__global__ __launch_bounds__(256) void k_inflate(const double2* __restrict__ src, double2* __restrict__ dst, Grid gs,
+                                                 Grid gd, int fak, long long src_stride, long long dst_stride) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  const double2* s = src + (long long)blockIdx.y * src_stride;
+  double2* d = dst + (long long)blockIdx.y * dst_stride;
+  for (long long l = (long long)blockIdx.x * blockDim.x + threadIdx.x; l < gd.npts; l += stride) {
+    long long rest = l;
+    int id[3] = {0, 0, 0};
+    for (int a = 2; a >= 0; --a) {
+      id[a] = (int)(rest % gd.n[a]);
+      rest /= gd.n[a];
+    }
+    bool inside = true;
+    long long ls = 0;
+    for (int a = 0; a < 3; ++a) {
+      int is;
+      if (!FIX) {
+        is = id[a] - (gd.n[a] - gs.n[a]) / 2;
+      } else {
+        // destination off + i/fak <- source i for i = 0, fak, 2 fak, ...
+        const int off = gs.n[a] * (fak - 1) / 2 / fak;
+        const int q = id[a] - off;
+        is = q * fak;
+        if (gs.n[a] == 1) is = q;  // unused axis
+      }
+      if (is < 0 || is >= gs.n[a]) inside = false;
+      ls = ls * gs.n[a] + (inside ? is : 0);
+    }
+    d[l] = inside ? s[ls] : make_double2(0.0, 0.0);
+  }
+}
+
 // streaming copy (bandwidth probe)
 __global__ __launch_bounds__(256) void k_copy(const double2* __restrict__ src, double2* __restrict__ dst, long long n) {
   const long long stride = (long long)gridDim.x * blockDim.x;

@@ -28,6 +28,7 @@ EXPORTED = [
     "atus_batch_rabi_integrals", "atus_sob_run", "atus_graph_create", "atus_graph_launch", "atus_graph_destroy",
     "atus_set_alpha2", "atus_run_block_ds", "atus_v_step_ds",
     "atus_snapshot_save", "atus_snapshot_restore", "atus_copy_components", "atus_v_step",
+    "atus_get_moments", "atus_inflate",
     "atus_net_create", "atus_net_destroy", "atus_net_set_precision", "atus_net_param_count", "atus_net_get_params", "atus_net_set_params",
     "atus_net_forward", "atus_net_gradients", "atus_net_set_data", "atus_net_train", "atus_net_warm_start",
 ]
@@ -55,6 +56,11 @@ class DsParams(C.Structure):
 
 class BatchRun(C.Structure):
     _fields_ = [("gs", C.c_double * 4), ("laser", Laser)]
+
+
+class Moments(C.Structure):
+    _fields_ = [("N", C.c_double), ("x", C.c_double * 3), ("x2", C.c_double * 3), ("k", C.c_double * 3),
+                ("k2", C.c_double * 3), ("phase", C.c_double)]
 
 
 class StepParams(C.Structure):
@@ -124,6 +130,8 @@ def load():
         "atus_v_step_ds": (I, [P, I, D, D, C.POINTER(Laser), C.POINTER(DsParams)]),
         "atus_snapshot_save": (I, [P, I]),
         "atus_v_step": (I, [P, I, D, D, D, C.POINTER(Laser)]),
+        "atus_get_moments": (I, [P, I, C.POINTER(Moments)]),
+        "atus_inflate": (I, [P, P, I, I]),
         "atus_snapshot_restore": (I, [P, I]),
         "atus_copy_components": (I, [P, I, P, I, I]),
         "atus_net_create": (I, [pctx, I, I, I, I, I, I, C.c_ulonglong]),
@@ -225,6 +233,19 @@ class Context:
     def v_step(self, kind, dt, t, las: Laser, F=1.0):
         """One V step of any block kind at Get_t() = t, no kinetic step."""
         _check(_lib.atus_v_step(self._h, int(kind), float(dt), float(t), float(F), C.byref(las)))
+
+    def moments(self, comp=0):
+        """The moments ana_expval prints: dict of N, x, x2, k, k2 (dim
+        entries each) and phase."""
+        m = Moments()
+        _check(_lib.atus_get_moments(self._h, int(comp), C.byref(m)))
+        d = self.dim
+        return {"N": m.N, "x": list(m.x[:d]), "x2": list(m.x2[:d]), "k": list(m.k[:d]), "k2": list(m.k2[:d]),
+                "phase": m.phase}
+
+    def inflate_from(self, src, fak, fix=False):
+        """This context := src inflated by fak (inflate_domain)."""
+        _check(_lib.atus_inflate(self._h, src._h, int(fak), int(bool(fix))))
 
     def snapshot_save(self, slot=0):
         _check(_lib.atus_snapshot_save(self._h, int(slot)))

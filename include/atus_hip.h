@@ -175,6 +175,24 @@ int atus_expval_k(atus_ctx *ctx, int comp, double out[3]);
 /* compute_rabi_integrals: k0 holds n points of 3 doubles, n <= 8;
  * out[q] = ar_k * sum of |psi^|^2 in slot q with the reference's slot rule */
 int atus_rabi_integrals(atus_ctx *ctx, int comp, const double *k0, int n, double threshold, double *out);
+/* the moments ana_expval prints (ana_expval.cpp:50-185, 281-394) for
+ * component comp: N = ar sum |psi|^2; <x_a> = ar sum x_a |psi|^2 / N and
+ * <(x_a - <x_a>)^2> likewise; <k_a>, <(k_a - <k_a>)^2> from the transformed
+ * state with ar_k, also divided by N; phase = sum atan2(Im, Re) |psi|^2 /
+ * sum |psi|^2 (ana_expval prints it for 1D states).  psi is unchanged. */
+typedef struct {
+  double N;
+  double x[3], x2[3];
+  double k[3], k2[3];
+  double phase;
+} atus_moments;
+int atus_get_moments(atus_ctx *ctx, int comp, atus_moments *out);
+/* inflate_domain (inflate_domain.cpp:41-223) between two contexts of the
+ * same dimension and component count: fix == 0 copies src into the centre
+ * of dst, whose axes are fak times longer (zeros around it); fix == 1 keeps
+ * the number of points and places every fak-th point of src around the
+ * centre of dst.  The header changes (domain, dx, dk) are the caller's. */
+int atus_inflate(atus_ctx *dst, atus_ctx *src, int fak, int fix);
 /* energy functional (no reference counterpart):
  * ar_k sum_i sum_k alpha.k^2 |psi_i^|^2 + ar sum_i sum_x (V_i + 1/2 sum_j g_ij |psi_j|^2) |psi_i|^2 */
 int atus_energy(atus_ctx *ctx, double *out);

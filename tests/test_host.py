@@ -198,3 +198,19 @@ def test_driver_errors_exit_nonzero(tmp_path):
         assert r.returncode != 0 and "Could not open file does_not_exist.bin" in r.stdout, drv
         r = subprocess.run([tool(drv)], capture_output=True, text=True)
         assert r.returncode != 0 and "No parameter xml file specified" in r.stdout, drv
+
+
+def test_bin_tools_argument_errors(tmp_path):
+    # inflate_domain: no inflation option ("Hmmm", inflate_domain.cpp:266-270),
+    # no file name, a missing file; ana_expval: a missing file prints and
+    # exits 0, no argument prints nothing (ana_expval.cpp:236-258)
+    r = subprocess.run([tool("inflate_domain"), "a.bin"], capture_output=True, text=True, cwd=tmp_path)
+    assert r.returncode != 0 and "Hmmm" in r.stdout
+    r = subprocess.run([tool("inflate_domain"), "--fak", "2"], capture_output=True, text=True, cwd=tmp_path)
+    assert r.returncode != 0 and "missing file name" in r.stdout
+    r = subprocess.run([tool("inflate_domain"), "--fak=2", "nope.bin"], capture_output=True, text=True, cwd=tmp_path)
+    assert r.returncode != 0 and "Cannot open file: nope.bin" in r.stdout
+    r = subprocess.run([tool("ana_expval"), "nope.bin"], capture_output=True, text=True, cwd=tmp_path)
+    assert r.returncode == 0 and "Could not open file nope.bin." in r.stdout
+    r = subprocess.run([tool("ana_expval")], capture_output=True, text=True, cwd=tmp_path)
+    assert r.returncode == 0 and r.stdout == ""
