@@ -1,6 +1,8 @@
 """Autotuner for the FFT kernels: picks, per kernel family, the number of
-values each thread holds (8 or 16) by timing profiled blocks of the actual
-workload on the actual context (HIP events on the context's stream).
+values each thread holds (8 or 16), and for 3D grids whether the y/z
+passes run slab by slab (atus_set_slab), by timing profiled blocks of the
+actual workload on the actual context (HIP events on the context's
+stream).
 
 The blocks it runs advance the context's state like any other block, so it
 belongs in the warm-up of a run.  Choices are cached per (grid shape,
@@ -19,23 +21,41 @@ def autotune(ctx: "hip.Context", kind: int, dt: float, t: float, las=None, amp=N
     """Returns (t after the tuning blocks, {family: values_per_thread})."""
     key = (ctx.shape, ctx.ncomp, kind)
     if key in _cache:
-        for f, rm in _cache[key].items():
-            ctx.tune(f, rm)
+        _apply(ctx, _cache[key])
         return t, dict(_cache[key])
     best = {f: (float("inf"), 16) for f in FAMILIES}
+    ctx.set_slab(0)
     for rm in (16, 8):
         for f in FAMILIES:
             ctx.tune(f, rm)
         for _ in range(reps):
             t, prof = ctx.run_block_profiled(kind, dt, Nk, t, las, amp)
             for f in FAMILIES:
-                ms, n = prof[f]
+                ms, n, grid = prof[f]
                 if n:
-                    avg = ms / n
+                    avg = ms / grid     # per whole-grid pass
                     if avg < best[f][0]:
                         best[f] = (avg, rm)
     choice = {f: best[f][1] for f in FAMILIES}
-    for f, rm in choice.items():
-        ctx.tune(f, rm)
+    _apply(ctx, choice)
+    if len(ctx.shape) == 3:
+        # slabs of about 256 MiB against whole-grid passes: block time
+        total = {}
+        for slab in (0, -1):
+            ctx.set_slab(slab)
+            total[slab] = float("inf")
+            for _ in range(reps):
+                t, prof = ctx.run_block_profiled(kind, dt, Nk, t, las, amp)
+                total[slab] = min(total[slab], sum(v[0] for v in prof.values()))
+        choice["slab"] = min(total, key=total.get)
+    _apply(ctx, choice)
     _cache[key] = dict(choice)
     return t, choice
+
+
+def _apply(ctx, choice):
+    for f, v in choice.items():
+        if f == "slab":
+            ctx.set_slab(v)
+        else:
+            ctx.tune(f, v)

@@ -75,7 +75,7 @@ int check_comp(atus_ctx* c, int comp) {
 struct ProfScope {
   atus_ctx* c;
   bool on;
-  ProfScope(atus_ctx* cc, int fam) : c(cc), on(cc->prof) {
+  ProfScope(atus_ctx* cc, int fam, double frac = 1.0) : c(cc), on(cc->prof) {
     if (!on) return;
     hipEvent_t e;
     if (hipEventCreate(&e) != hipSuccess) {
@@ -85,6 +85,7 @@ struct ProfScope {
     (void)hipEventRecord(e, c->stream);
     c->prof_ev.push_back(e);
     c->prof_fam.push_back(fam);
+    c->prof_frac.push_back(frac);
   }
   ~ProfScope() {
     if (!on) return;
@@ -199,6 +200,36 @@ int pass(atus_ctx* c, int a, int dir, const double2* src, double2* dst, int ncom
   ProfScope ps(c, fam);
   HIP_TRY(launch_pass(N, c->rm[fam], inner == 1, dir, src, dst, cs ? cs : c->npts, ncomp, inner, outer, c->tw[N], scale,
                       c->max_lds, c->stream));
+  return ATUS_OK;
+}
+
+// one pass along axis a (1 or 2) of a 3D grid over the x-planes
+// [s0, s0 + S) of all components, in place
+int pass_slab(atus_ctx* c, int a, int dir, int s0, int S) {
+  const int N = c->n[a];
+  ATUS_TRY(ensure_twiddles(c, N));
+  const long long plane = (long long)c->n[1] * c->n[2];
+  const long long inner = c->inner_of(a), outer = c->outer_of(a) / c->n[0] * S;
+  const int fam = inner == 1 ? ATUS_KFAM_PASS_CONTIG : ATUS_KFAM_PASS_STRIDED;
+  double2* p = c->psi + (long long)s0 * plane;
+  ProfScope ps(c, fam, (double)S / c->n[0]);
+  HIP_TRY(launch_pass(N, c->rm[fam], inner == 1, dir, p, p, c->npts, c->ncomp, inner, outer, c->tw[N], 1.0, c->max_lds,
+                      c->stream));
+  return ATUS_OK;
+}
+
+// the fused inverse-z / V / forward-z pass over the x-planes [s0, s0 + S):
+// the kernel's x index counts from the slab, so n0 is shifted to keep
+// x0 = (i0 - n0/2) dx0 (n0 even), and the potential pointer moves along
+int vfuse_slab(atus_ctx* c, int vk, const VArgs& va, int s0, int S) {
+  const int NL = c->n[2];
+  const long long plane = (long long)c->n[1] * c->n[2];
+  VArgs vs = va;
+  if (vs.pot) vs.pot += (long long)s0 * plane;
+  ProfScope ps(c, ATUS_KFAM_VFUSE, (double)S / c->n[0]);
+  HIP_TRY(launch_vfuse(NL, c->rm[ATUS_KFAM_VFUSE], vk, c->ncomp, c->psi + (long long)s0 * plane, c->npts,
+                       (long long)S * c->n[1], c->tw[NL], vs, c->n[1], c->n[0] - 2 * s0, c->dx[0], c->max_lds,
+                       c->stream));
   return ATUS_OK;
 }
 
@@ -729,11 +760,78 @@ struct StepFeed {
 // the launches of one block of a single run on the context's stream (half
 // kinetic, (V, full) x (Nk-1), V, half), without synchronisation; *t
 // advances with the reference's bookkeeping
+// V-step arguments of step m of a block (t1, amplitude, pulse gates)
+int step_vargs(const VArgs& base, int m, double tt, double dt, const StepFeed& fd, VArgs& va) {
+  va = base;
+  va.t0 = tt;
+  va.t1 = tt + 0.5 * dt;
+  if (fd.t1dev) {
+    va.t1p = fd.t1dev + (m - 1);
+    va.Fp = fd.fdev + (m - 1);
+  } else {
+    va.F = (fd.sp && fd.sp->amp) ? fd.sp->amp(va.t1, fd.sp->amp_user) : 1.0;
+    if (!std::isfinite(va.F)) return fail(ATUS_ERR_NUMERIC, "amplitude callback returned a non-finite value");
+  }
+  if (fd.ds) {
+    // bragg_ds.cpp:120-128: a pulse is off once t1 - seq.time exceeds its duration
+    const double time = va.t1 - fd.ds->seq_time;
+    va.mode1 = time > fd.ds->duration[0] ? 0.0 : 1.0;
+    va.mode2 = time > fd.ds->duration[1] ? 0.0 : 1.0;
+  }
+  return ATUS_OK;
+}
+
+// a 3D block with the y and z passes between two kinetic steps chained slab
+// by slab of c->slab x-planes (they only couple points of one x-plane):
+//   per slab [z fwd, y fwd]; kin; per step per slab [y inv, vfuse, y fwd],
+//   kin; per slab [y inv, z inv]
+// Each slab's passes follow each other while the slab's data is still in
+// the device's last-level cache.  Same kernels and arithmetic as the
+// unslabbed order.
+int single_block_slabs(atus_ctx* c, int vk, const VArgs& base, int Nk, double dt, double* t, const StepFeed& fd) {
+  const int S = c->slab_planes(), n0 = c->n[0], N0 = c->n[0];
+  const long long inner0 = c->inner_of(0);
+  auto kin = [&](int f, const VArgs& va) -> int {
+    ProfScope ps(c, ATUS_KFAM_KIN);
+    HIP_TRY(launch_kin(N0, c->rm[ATUS_KFAM_KIN], false, V_NONE, 1, c->psi, c->npts, c->ncomp, inner0, c->tw[N0],
+                       kin_tabs(c, f), va, c->dx[0], c->max_lds, c->stream));
+    return ATUS_OK;
+  };
+  double tt = *t;
+  for (int s0 = 0; s0 < n0; s0 += S) {
+    const int Ss = std::min(S, n0 - s0);
+    ATUS_TRY(pass_slab(c, 2, -1, s0, Ss));
+    ATUS_TRY(pass_slab(c, 1, -1, s0, Ss));
+  }
+  ATUS_TRY(kin(0, base));
+  tt += 0.5 * dt;
+  for (int m = 1; m <= Nk; ++m) {
+    VArgs va;
+    ATUS_TRY(step_vargs(base, m, tt, dt, fd, va));
+    for (int s0 = 0; s0 < n0; s0 += S) {
+      const int Ss = std::min(S, n0 - s0);
+      ATUS_TRY(pass_slab(c, 1, +1, s0, Ss));
+      ATUS_TRY(vfuse_slab(c, vk, va, s0, Ss));
+      ATUS_TRY(pass_slab(c, 1, -1, s0, Ss));
+    }
+    ATUS_TRY(kin(m < Nk ? 1 : 0, va));
+    tt += (m < Nk) ? dt : 0.5 * dt;
+  }
+  for (int s0 = 0; s0 < n0; s0 += S) {
+    const int Ss = std::min(S, n0 - s0);
+    ATUS_TRY(pass_slab(c, 1, +1, s0, Ss));
+    ATUS_TRY(pass_slab(c, 2, +1, s0, Ss));
+  }
+  *t = tt;
+  return ATUS_OK;
+}
+
 int single_block(atus_ctx* c, int vk, const VArgs& base, int Nk, double dt, double* t, const StepFeed& fd) {
   // the fused kernels cover one species of up to two components; two
   // species (per-component kinetic tables, four-component steps) run the
   // V step and the kinetic step as separate passes
   const bool fused = c->ncomp <= 2 && !c->alpha2_set && (vk == V_NL || vk == V_NL_IF || vk == V_BRAGG);
+  if (fused && c->slab_planes() > 0) return single_block_slabs(c, vk, base, Nk, dt, t, fd);
   const int N0 = c->n[0], NL = c->n[c->dim - 1];
   const long long inner0 = c->inner_of(0);
   double tt = *t;
@@ -755,22 +853,8 @@ int single_block(atus_ctx* c, int vk, const VArgs& base, int Nk, double dt, doub
   }
   tt += 0.5 * dt;
   for (int m = 1; m <= Nk; ++m) {
-    VArgs va = base;
-    va.t0 = tt;
-    va.t1 = tt + 0.5 * dt;
-    if (fd.t1dev) {
-      va.t1p = fd.t1dev + (m - 1);
-      va.Fp = fd.fdev + (m - 1);
-    } else {
-      va.F = (fd.sp && fd.sp->amp) ? fd.sp->amp(va.t1, fd.sp->amp_user) : 1.0;
-      if (!std::isfinite(va.F)) return fail(ATUS_ERR_NUMERIC, "amplitude callback returned a non-finite value");
-    }
-    if (fd.ds) {
-      // bragg_ds.cpp:120-128: a pulse is off once t1 - seq.time exceeds its duration
-      const double time = va.t1 - fd.ds->seq_time;
-      va.mode1 = time > fd.ds->duration[0] ? 0.0 : 1.0;
-      va.mode2 = time > fd.ds->duration[1] ? 0.0 : 1.0;
-    }
+    VArgs va;
+    ATUS_TRY(step_vargs(base, m, tt, dt, fd, va));
     const int f = (m < Nk) ? 1 : 0;
     if (!fused) {
       {
@@ -1317,16 +1401,19 @@ int atus_sob_run(atus_ctx* c, double eps, long long maxit, long long* iters, dou
   return ATUS_OK;
 }
 
-int atus_run_block_profiled(atus_ctx* c, const atus_step_params* sp, int Nk, double* t, double* ms, int* count) {
+int atus_run_block_profiled(atus_ctx* c, const atus_step_params* sp, int Nk, double* t, double* ms, int* count,
+                            double* grid) {
   ATUS_TRY(check_ctx(c));
   if (!ms || !count) return fail(ATUS_ERR_ARG, "null output");
   for (int f = 0; f < ATUS_NKFAM; ++f) {
     ms[f] = 0;
     count[f] = 0;
+    if (grid) grid[f] = 0;
   }
   c->prof = true;
   c->prof_ev.clear();
   c->prof_fam.clear();
+  c->prof_frac.clear();
   int rc = atus_run_block(c, sp, Nk, t);
   c->prof = false;
   if (rc == ATUS_OK) {
@@ -1341,10 +1428,12 @@ int atus_run_block_profiled(atus_ctx* c, const atus_step_params* sp, int Nk, dou
     const int f = c->prof_fam[i / 2];
     ms[f] += e;
     count[f] += 1;
+    if (grid) grid[f] += c->prof_frac[i / 2];
   }
   for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
   c->prof_ev.clear();
   c->prof_fam.clear();
+  c->prof_frac.clear();
   return rc;
 }
 
@@ -1369,6 +1458,13 @@ int atus_copy_bandwidth(atus_ctx* c, int reps, double* gbps) {
   (void)hipFree(dst);
   if (rc != ATUS_OK) return rc;
   *gbps = 2.0 * sizeof(double2) * (double)n * reps / (ms * 1e-3) / 1e9;
+  return ATUS_OK;
+}
+
+int atus_set_slab(atus_ctx* c, int planes) {
+  ATUS_TRY(check_ctx(c));
+  if (planes < -1) return fail(ATUS_ERR_ARG, "slab planes must be -1 (automatic), 0 (off) or positive");
+  c->slab = planes;
   return ATUS_OK;
 }
 

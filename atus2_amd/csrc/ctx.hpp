@@ -98,11 +98,26 @@ struct atus_ctx {
   // defaults: what the autotuner picks at 256^3 (8 for the strided passes
   // and for the register-heavy V pass, which spills at 16)
   int rm[ATUS_NKFAM] = {16, 8, 16, 8, 16};
+  // 3D blocks: the y and z passes between two kinetic steps run slab by
+  // slab of this many x-planes (0: over the whole grid each; -1: chosen
+  // from the size of the device's last-level cache)
+  int slab = -1;
+  int slab_planes() const {
+    if (dim != 3 || fs) return 0;
+    int s = slab;
+    if (s < 0) {
+      // the slab's data fills about the 256 MiB Infinity Cache of MI355X
+      const double plane = 16.0 * ncomp * n[1] * n[2];
+      s = (int)std::max(1.0, std::floor(268435456.0 / plane));
+    }
+    return s < n[0] ? s : 0;
+  }
 
   // per-launch event pairs while profiling
   bool prof = false;
   std::vector<hipEvent_t> prof_ev;
   std::vector<int> prof_fam;
+  std::vector<double> prof_frac;  // share of the whole grid each launch covers
 
   Grid grid() const {
     Grid g{};

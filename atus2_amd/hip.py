@@ -28,7 +28,7 @@ EXPORTED = [
     "atus_batch_rabi_integrals", "atus_sob_run", "atus_graph_create", "atus_graph_launch", "atus_graph_destroy",
     "atus_set_alpha2", "atus_run_block_ds", "atus_v_step_ds",
     "atus_snapshot_save", "atus_snapshot_restore", "atus_copy_components", "atus_v_step",
-    "atus_get_moments", "atus_inflate",
+    "atus_get_moments", "atus_inflate", "atus_set_slab",
     "atus_net_create", "atus_net_destroy", "atus_net_set_precision", "atus_net_param_count", "atus_net_get_params", "atus_net_set_params",
     "atus_net_forward", "atus_net_gradients", "atus_net_set_data", "atus_net_train", "atus_net_warm_start",
 ]
@@ -113,7 +113,7 @@ def load():
         "atus_set_timing": (I, [P, I]),
         "atus_last_block_ms": (I, [P, C.POINTER(D)]),
         "atus_run_block_profiled": (I, [P, C.POINTER(StepParams), I, C.POINTER(D), C.POINTER(D),
-                                        C.POINTER(I)]),
+                                        C.POINTER(I), C.POINTER(D)]),
         "atus_copy_bandwidth": (I, [P, I, C.POINTER(D)]),
         "atus_tune": (I, [P, I, I]),
         "atus_batch_create": (I, [pctx, I, C.POINTER(C.c_longlong), C.POINTER(D), C.POINTER(D), I, I]),
@@ -132,6 +132,7 @@ def load():
         "atus_v_step": (I, [P, I, D, D, D, C.POINTER(Laser)]),
         "atus_get_moments": (I, [P, I, C.POINTER(Moments)]),
         "atus_inflate": (I, [P, P, I, I]),
+        "atus_set_slab": (I, [P, I]),
         "atus_snapshot_restore": (I, [P, I]),
         "atus_copy_components": (I, [P, I, P, I, I]),
         "atus_net_create": (I, [pctx, I, I, I, I, I, I, C.c_ulonglong]),
@@ -310,7 +311,8 @@ class Context:
         return tt.value
 
     def run_block_profiled(self, kind, dt, Nk, t, las: Laser | None = None, amp=None):
-        """Runs one block like run_block; returns (t, {family: (ms_total, launches)})."""
+        """Runs one block like run_block; returns (t, {family: (ms_total,
+        launches, whole-grid equivalents)})."""
         sp = StepParams()
         sp.kind = kind
         sp.dt = dt
@@ -323,8 +325,9 @@ class Context:
         tt = C.c_double(t)
         ms = (C.c_double * len(KFAM))()
         cnt = (C.c_int * len(KFAM))()
-        _check(_lib.atus_run_block_profiled(self._h, C.byref(sp), int(Nk), C.byref(tt), ms, cnt))
-        return tt.value, {KFAM[i]: (ms[i], cnt[i]) for i in range(len(KFAM))}
+        grid = (C.c_double * len(KFAM))()
+        _check(_lib.atus_run_block_profiled(self._h, C.byref(sp), int(Nk), C.byref(tt), ms, cnt, grid))
+        return tt.value, {KFAM[i]: (ms[i], cnt[i], grid[i]) for i in range(len(KFAM))}
 
     # reductions
     def particle_number(self, comp=0):
@@ -428,6 +431,11 @@ class Context:
         """family: index or name in KFAM; values_per_thread: 8 or 16."""
         f = KFAM.index(family) if isinstance(family, str) else int(family)
         _check(_lib.atus_tune(self._h, f, int(values_per_thread)))
+
+    def set_slab(self, planes):
+        """3D blocks: y/z passes per slab of `planes` x-planes (-1 automatic,
+        0 whole grid)."""
+        _check(_lib.atus_set_slab(self._h, int(planes)))
 
     def copy_bandwidth(self, reps=20):
         """GB/s of a plain streaming copy of all components (HIP events)."""

@@ -84,12 +84,16 @@ def gpu_run(args, wl):
     t, prof = ctx.run_block_profiled(hip.STEP_BRAGG_AD, wl.dt, wl.Nk, t, las, wl.amp)
     fams = {k: v for k, v in prof.items() if v[1] > 0}
     dom = max(fams, key=lambda k: fams[k][0])
-    avg_ms = fams[dom][0] / fams[dom][1]
+    ms_tot, nl, grid = fams[dom]
+    avg_ms = ms_tot / nl
     # every pass family reads and writes each point of both components once
-    bytes_per_launch = 2 * npts * 32.0
+    # per whole-grid pass; a launch over a slab of x-planes moves its share
+    grid_bytes = 2 * npts * 32.0
+    bytes_per_launch = grid_bytes * grid / nl
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-    kernels = {k: {"avg_us": 1e3 * v[0] / v[1], "launches": v[1],
-                   "GBps": bytes_per_launch / (v[0] / v[1] * 1e-3) / 1e9} for k, v in fams.items()}
+    kernels = {k: {"avg_us": 1e3 * v[0] / v[1], "launches": v[1], "grid_share_per_launch": v[2] / v[1],
+                   "us_per_grid": 1e3 * v[0] / v[2], "GBps": grid_bytes * v[2] / (v[0] * 1e-3) / 1e9}
+               for k, v in fams.items()}
     copy_gbps = ctx.copy_bandwidth(20)
     log(f"[bench] copy probe {copy_gbps:.0f} GB/s; dominant kernel {dom} {achieved:.0f} GB/s")
     ctx.close()
@@ -99,6 +103,7 @@ def gpu_run(args, wl):
         "rabi": [float(v) for v in rabi],
         "dominant": dom,
         "dom_avg_ms": avg_ms,
+        "dom_share": grid / nl,
         "achieved": achieved,
         "bytes_per_launch": bytes_per_launch,
         "kernels": kernels,
@@ -179,6 +184,8 @@ def main():
     npts = float(math.prod(wl.shape))
     value = npts * args.steps / r["elapsed"] / 1e9
     traffic = measured_traffic(wl.n, r["dominant"], r["tuning"].get(r["dominant"], DEFAULT_RM[r["dominant"]]))
+    if traffic is not None:
+        traffic *= r["dom_share"]   # the table holds whole-grid launches
     gs = ground_state_run(args.gs_cases) if args.gs_cases > 0 else None
     cpu = None
     if not args.no_cpu_baseline:

@@ -224,14 +224,17 @@ int atus_last_block_ms(atus_ctx *ctx, double *ms);
 /* ---- per-kernel device times: runs one block exactly as atus_run_block,
  * with a HIP event pair on the context's stream around every launch.
  * ms[f] is the summed duration and count[f] the number of launches of
- * kernel family f. */
+ * kernel family f; grid[f] (optional, may be NULL) sums the share of the
+ * whole grid each launch covered (1 for a whole-grid pass, S/nx for a pass
+ * over a slab of S x-planes). */
 #define ATUS_KFAM_PASS_CONTIG 0  /* DFT pass along the contiguous axis */
 #define ATUS_KFAM_PASS_STRIDED 1 /* DFT pass along a strided axis */
 #define ATUS_KFAM_KIN 2          /* fused kinetic step along axis 0 */
 #define ATUS_KFAM_VFUSE 3        /* fused inverse / V step / forward along the last axis */
 #define ATUS_KFAM_POINT 4        /* standalone pointwise V step */
 #define ATUS_NKFAM 5
-int atus_run_block_profiled(atus_ctx *ctx, const atus_step_params *sp, int Nk, double *t, double *ms, int *count);
+int atus_run_block_profiled(atus_ctx *ctx, const atus_step_params *sp, int Nk, double *t, double *ms, int *count,
+                            double *grid);
 
 /* tuning: values per thread (8 or 16) of the FFT kernels of one kernel
  * family (ATUS_KFAM_*; the Sobolev filter follows ATUS_KFAM_KIN).  16 halves
@@ -239,6 +242,12 @@ int atus_run_block_profiled(atus_ctx *ctx, const atus_step_params *sp, int Nk, d
  * the contiguous passes and the kinetic step, 8 for the strided passes and
  * the fused V pass. */
 int atus_tune(atus_ctx *ctx, int family, int values_per_thread);
+/* scheduling of 3D blocks (no reference counterpart): the y and z passes
+ * between two kinetic steps only couple points of one x-plane, so they run
+ * slab by slab of `planes` x-planes, each slab's passes back to back while
+ * its data is still in the last-level cache.  -1 (default): slabs of about
+ * 256 MiB of state; 0: whole-grid passes.  The results do not change. */
+int atus_set_slab(atus_ctx *ctx, int planes);
 
 /* HBM copy probe: copies all components (ncomp * npts complex128) into a
  * scratch buffer `reps` times with a plain streaming kernel and returns the
