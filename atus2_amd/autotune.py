@@ -10,6 +10,8 @@ components, step kind) for the life of the process.
 """
 from __future__ import annotations
 
+import time
+
 from . import hip
 
 FAMILIES = ["pass_contig", "pass_strided", "kin", "vfuse"]
@@ -39,14 +41,19 @@ def autotune(ctx: "hip.Context", kind: int, dt: float, t: float, las=None, amp=N
     choice = {f: best[f][1] for f in FAMILIES}
     _apply(ctx, choice)
     if len(ctx.shape) == 3:
-        # slabs of about 256 MiB against whole-grid passes: block time
+        # slab-chained passes on two streams against whole-grid passes: wall
+        # time of ordinary blocks (profiled blocks run on one stream)
         total = {}
         for slab in (0, -1):
             ctx.set_slab(slab)
+            t = ctx.run_block(kind, dt, Nk, t, las, amp)
+            ctx.synchronize()
             total[slab] = float("inf")
             for _ in range(reps):
-                t, prof = ctx.run_block_profiled(kind, dt, Nk, t, las, amp)
-                total[slab] = min(total[slab], sum(v[0] for v in prof.values()))
+                tic = time.perf_counter()
+                t = ctx.run_block(kind, dt, Nk, t, las, amp)
+                ctx.synchronize()
+                total[slab] = min(total[slab], time.perf_counter() - tic)
         choice["slab"] = min(total, key=total.get)
     _apply(ctx, choice)
     _cache[key] = dict(choice)

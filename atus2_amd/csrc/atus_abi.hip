@@ -75,14 +75,18 @@ int check_comp(atus_ctx* c, int comp) {
 struct ProfScope {
   atus_ctx* c;
   bool on;
-  ProfScope(atus_ctx* cc, int fam, double frac = 1.0) : c(cc), on(cc->prof) {
+  hipStream_t st;
+  // events on the stream the launch goes to (the context's stream unless
+  // given)
+  ProfScope(atus_ctx* cc, int fam, double frac = 1.0, hipStream_t s = nullptr)
+      : c(cc), on(cc->prof), st(s ? s : cc->stream) {
     if (!on) return;
     hipEvent_t e;
     if (hipEventCreate(&e) != hipSuccess) {
       on = false;
       return;
     }
-    (void)hipEventRecord(e, c->stream);
+    (void)hipEventRecord(e, st);
     c->prof_ev.push_back(e);
     c->prof_fam.push_back(fam);
     c->prof_frac.push_back(frac);
@@ -91,7 +95,7 @@ struct ProfScope {
     if (!on) return;
     hipEvent_t e;
     if (hipEventCreate(&e) != hipSuccess) return;
-    (void)hipEventRecord(e, c->stream);
+    (void)hipEventRecord(e, st);
     c->prof_ev.push_back(e);
   }
 };
@@ -205,31 +209,31 @@ int pass(atus_ctx* c, int a, int dir, const double2* src, double2* dst, int ncom
 
 // one pass along axis a (1 or 2) of a 3D grid over the x-planes
 // [s0, s0 + S) of all components, in place
-int pass_slab(atus_ctx* c, int a, int dir, int s0, int S) {
+int pass_slab(atus_ctx* c, int a, int dir, int s0, int S, hipStream_t st) {
   const int N = c->n[a];
   ATUS_TRY(ensure_twiddles(c, N));
   const long long plane = (long long)c->n[1] * c->n[2];
   const long long inner = c->inner_of(a), outer = c->outer_of(a) / c->n[0] * S;
   const int fam = inner == 1 ? ATUS_KFAM_PASS_CONTIG : ATUS_KFAM_PASS_STRIDED;
   double2* p = c->psi + (long long)s0 * plane;
-  ProfScope ps(c, fam, (double)S / c->n[0]);
+  ProfScope ps(c, fam, (double)S / c->n[0], st);
   HIP_TRY(launch_pass(N, c->rm[fam], inner == 1, dir, p, p, c->npts, c->ncomp, inner, outer, c->tw[N], 1.0, c->max_lds,
-                      c->stream));
+                      st));
   return ATUS_OK;
 }
 
 // the fused inverse-z / V / forward-z pass over the x-planes [s0, s0 + S):
 // the kernel's x index counts from the slab, so n0 is shifted to keep
 // x0 = (i0 - n0/2) dx0 (n0 even), and the potential pointer moves along
-int vfuse_slab(atus_ctx* c, int vk, const VArgs& va, int s0, int S) {
+int vfuse_slab(atus_ctx* c, int vk, const VArgs& va, int s0, int S, hipStream_t st) {
   const int NL = c->n[2];
   const long long plane = (long long)c->n[1] * c->n[2];
   VArgs vs = va;
   if (vs.pot) vs.pot += (long long)s0 * plane;
-  ProfScope ps(c, ATUS_KFAM_VFUSE, (double)S / c->n[0]);
+  ProfScope ps(c, ATUS_KFAM_VFUSE, (double)S / c->n[0], st);
   HIP_TRY(launch_vfuse(NL, c->rm[ATUS_KFAM_VFUSE], vk, c->ncomp, c->psi + (long long)s0 * plane, c->npts,
                        (long long)S * c->n[1], c->tw[NL], vs, c->n[1], c->n[0] - 2 * s0, c->dx[0], c->max_lds,
-                       c->stream));
+                       st));
   return ATUS_OK;
 }
 
@@ -561,6 +565,10 @@ int atus_ctx_destroy(atus_ctx* c) {
   (void)hipFree(c->sred_out);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream2) (void)hipStreamSynchronize(c->stream2);
+  if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
+  if (c->join_ev) (void)hipEventDestroy(c->join_ev);
+  if (c->stream2) (void)hipStreamDestroy(c->stream2);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return ATUS_OK;
@@ -797,31 +805,59 @@ int single_block_slabs(atus_ctx* c, int vk, const VArgs& base, int Nk, double dt
                        kin_tabs(c, f), va, c->dx[0], c->max_lds, c->stream));
     return ATUS_OK;
   };
+  // odd slabs on the second stream (profiled blocks record their events on
+  // each launch's stream)
+  const bool two = c->slab_streams > 1;
+  if (two && !c->stream2) {
+    if (hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming) != hipSuccess)
+      return fail(ATUS_ERR_HIP, "second stream of the slab chains");
+  }
+  auto st_of = [&](int s0) { return (two && (s0 / S) % 2 == 1) ? c->stream2 : c->stream; };
+  auto fork = [&]() -> int {
+    if (!two) return ATUS_OK;
+    HIP_TRY(hipEventRecord(c->fork_ev, c->stream));
+    HIP_TRY(hipStreamWaitEvent(c->stream2, c->fork_ev, 0));
+    return ATUS_OK;
+  };
+  auto join = [&]() -> int {
+    if (!two) return ATUS_OK;
+    HIP_TRY(hipEventRecord(c->join_ev, c->stream2));
+    HIP_TRY(hipStreamWaitEvent(c->stream, c->join_ev, 0));
+    return ATUS_OK;
+  };
   double tt = *t;
+  ATUS_TRY(fork());
   for (int s0 = 0; s0 < n0; s0 += S) {
     const int Ss = std::min(S, n0 - s0);
-    ATUS_TRY(pass_slab(c, 2, -1, s0, Ss));
-    ATUS_TRY(pass_slab(c, 1, -1, s0, Ss));
+    ATUS_TRY(pass_slab(c, 2, -1, s0, Ss, st_of(s0)));
+    ATUS_TRY(pass_slab(c, 1, -1, s0, Ss, st_of(s0)));
   }
+  ATUS_TRY(join());
   ATUS_TRY(kin(0, base));
   tt += 0.5 * dt;
   for (int m = 1; m <= Nk; ++m) {
     VArgs va;
     ATUS_TRY(step_vargs(base, m, tt, dt, fd, va));
+    ATUS_TRY(fork());
     for (int s0 = 0; s0 < n0; s0 += S) {
       const int Ss = std::min(S, n0 - s0);
-      ATUS_TRY(pass_slab(c, 1, +1, s0, Ss));
-      ATUS_TRY(vfuse_slab(c, vk, va, s0, Ss));
-      ATUS_TRY(pass_slab(c, 1, -1, s0, Ss));
+      ATUS_TRY(pass_slab(c, 1, +1, s0, Ss, st_of(s0)));
+      ATUS_TRY(vfuse_slab(c, vk, va, s0, Ss, st_of(s0)));
+      ATUS_TRY(pass_slab(c, 1, -1, s0, Ss, st_of(s0)));
     }
+    ATUS_TRY(join());
     ATUS_TRY(kin(m < Nk ? 1 : 0, va));
     tt += (m < Nk) ? dt : 0.5 * dt;
   }
+  ATUS_TRY(fork());
   for (int s0 = 0; s0 < n0; s0 += S) {
     const int Ss = std::min(S, n0 - s0);
-    ATUS_TRY(pass_slab(c, 1, +1, s0, Ss));
-    ATUS_TRY(pass_slab(c, 2, +1, s0, Ss));
+    ATUS_TRY(pass_slab(c, 1, +1, s0, Ss, st_of(s0)));
+    ATUS_TRY(pass_slab(c, 2, +1, s0, Ss, st_of(s0)));
   }
+  ATUS_TRY(join());
   *t = tt;
   return ATUS_OK;
 }

@@ -52,6 +52,11 @@ struct atus_ctx {
   double2* tbuf = nullptr;  // ncomp * npts, the transposes of full transforms
   int device = 0;
   hipStream_t stream = nullptr;
+  // second stream of the slab chains (created on first use, default
+  // priority) and the fork / join events between the two
+  hipStream_t stream2 = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+  int slab_streams = 2;
   size_t max_lds = 65536;
 
   double2* psi = nullptr;   // ncomp * npts
@@ -99,16 +104,18 @@ struct atus_ctx {
   // and for the register-heavy V pass, which spills at 16)
   int rm[ATUS_NKFAM] = {16, 8, 16, 8, 16};
   // 3D blocks: the y and z passes between two kinetic steps run slab by
-  // slab of this many x-planes (0: over the whole grid each; -1: chosen
-  // from the size of the device's last-level cache)
+  // slab of this many x-planes, alternate slabs on two streams (0: over the
+  // whole grid each; -1: chosen from the size of the device's last-level
+  // cache)
   int slab = -1;
   int slab_planes() const {
     if (dim != 3 || fs) return 0;
     int s = slab;
     if (s < 0) {
-      // the slab's data fills about the 256 MiB Infinity Cache of MI355X
+      // two slabs in flight hold about 160 MiB of the 256 MiB Infinity
+      // Cache of MI355X (measured best at 256^3: 32 to 40 planes)
       const double plane = 16.0 * ncomp * n[1] * n[2];
-      s = (int)std::max(1.0, std::floor(268435456.0 / plane));
+      s = (int)std::max(1.0, std::floor(83886080.0 / plane));
     }
     return s < n[0] ? s : 0;
   }

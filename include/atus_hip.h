@@ -245,8 +245,10 @@ int atus_tune(atus_ctx *ctx, int family, int values_per_thread);
 /* scheduling of 3D blocks (no reference counterpart): the y and z passes
  * between two kinetic steps only couple points of one x-plane, so they run
  * slab by slab of `planes` x-planes, each slab's passes back to back while
- * its data is still in the last-level cache.  -1 (default): slabs of about
- * 256 MiB of state; 0: whole-grid passes.  The results do not change. */
+ * its data is still in the last-level cache, alternate slabs on a second
+ * stream of the context (joined before each kinetic step).  -1 (default):
+ * slabs of about 80 MiB of state; 0: whole-grid passes on one stream.
+ * The results do not change. */
 int atus_set_slab(atus_ctx *ctx, int planes);
 
 /* HBM copy probe: copies all components (ncomp * npts complex128) into a

@@ -112,3 +112,38 @@ def test_graph_replays_equal_eager_batch_blocks(hip, orc, shape):
     finally:
         for B in bs:
             B.close()
+
+
+def test_graph_of_a_slab_chained_3d_block(hip, orc):
+    # a block whose y/z passes run per slab on two streams (fork / join by
+    # events) captured as a graph: replays equal eager whole-grid blocks bit
+    # for bit
+    shape = (32, 16, 64)
+    g = orc.make_grid(shape, L=[40.0, 30.0, 36.0])
+    psis = _state(orc, g, 2, 9)
+    las = hip.laser(amp0=5.0, laser_k=3 * 2 * np.pi / 40.0, laser_domh=8.0, deltaL=(0.0, 0.3))
+    dt, Nk = 1e-3, 12
+    ctxs = []
+    for slab in (0, 5):
+        ctx = hip.Context.from_grid(g, 2)
+        for c in range(2):
+            ctx.upload(c, psis[c])
+        ctx.set_alpha([0.5] * 3)
+        ctx.set_g([100.0, 95.0, 95.0, 100.0])
+        ctx.set_slab(slab)
+        ctxs.append(ctx)
+    eager, rep = ctxs
+    try:
+        with hip.Graph(rep, hip.STEP_BRAGG_AD, dt, Nk, las) as G:
+            te = tr = 0.0
+            for blk in range(3):
+                te = eager.run_block(hip.STEP_BRAGG_AD, dt, Nk, te, las, _amp)
+                F = np.array([_amp(t1) for t1 in hip.BatchContext.step_times(tr, dt, Nk)])
+                tr = G.launch(tr, F)
+                assert tr == te
+                for c in range(2):
+                    assert np.array_equal(rep.download(c), eager.download(c)), (blk, c)
+                assert rep.particle_number(1) == eager.particle_number(1)
+    finally:
+        for c in ctxs:
+            c.close()
