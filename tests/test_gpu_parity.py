@@ -325,7 +325,7 @@ def test_autotune_choices_keep_parity(hip, orc):
         ctx.set_g(gs)
         ctx.set_alpha([0.5] * 3)
         t, choice = autotune(ctx, hip.STEP_BRAGG_AD, 1e-3, 0.0, las, amp, Nk=5, reps=1)
-        assert set(choice.values()) <= {8, 16}
+        assert {v for f, v in choice.items() if f != "slab"} <= {8, 16} and choice["slab"] in (0, -1)
         start = [ctx.download(c) for c in range(2)]
         S = orc.RTSolver(g, start, gs, [0.5] * 3, dt=1e-3, t=t, amp=amp, deltaL=[0.0, 0.0], amp0=2.0,
                          laser_k=2 * np.pi / 24.0 * 4, laser_domh=1.0)
