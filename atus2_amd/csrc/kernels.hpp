@@ -690,15 +690,8 @@ struct KinTabs {
   int split;
 };
 
-#ifndef ATUS_KIN_WAVES
-#define ATUS_KIN_WAVES 1
-#endif
-#ifndef ATUS_VF_WAVES
-#define ATUS_VF_WAVES 1
-#endif
-
 template <int N, int RM, bool CONTIG, int VK, int NC, bool BATCH = false>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ATUS_KIN_WAVES))) void k_kin(double2* __restrict__ psi, long long comp_stride, AxisGeo g,
+__global__ __launch_bounds__(256) void k_kin(double2* __restrict__ psi, long long comp_stride, AxisGeo g,
                                              const double2* __restrict__ tw, KinTabs kt, VArgs va, double x0_dx) {
   extern __shared__ double2 lds_all[];
   constexpr int R = Plan<N, RM>::R, P = Plan<N, RM>::P;
@@ -778,7 +771,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ATUS_KIN_WA
 // The pencil (contiguous) has fixed coordinates on the other axes, so
 // x0 and the Bragg field are per-pencil constants.
 template <int N, int RM, int VK, int NC, bool BATCH = false>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ATUS_VF_WAVES))) void k_vfuse(double2* __restrict__ psi, long long comp_stride, AxisGeo g,
+__global__ __launch_bounds__(256) void k_vfuse(double2* __restrict__ psi, long long comp_stride, AxisGeo g,
                                                const double2* __restrict__ tw, VArgs va, long long pencils_per_x0,
                                                int n0, double dx0) {
   extern __shared__ double2 lds_all[];
@@ -813,9 +806,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ATUS_VF_WAV
     if (m.valid) v_point<VK, NC>(a, x0, m.base + m.t + P * i, va, Om, eta);
 #pragma unroll
     for (int c = 0; c < NC; ++c) v[c][i] = a[c];
-#ifdef ATUS_VF_SERIAL
-    __builtin_amdgcn_sched_barrier(0);
-#endif
   }
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
