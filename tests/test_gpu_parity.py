@@ -367,3 +367,30 @@ def test_slabbed_3d_blocks_are_bit_identical(hip, orc, slab, kind):
             outs.append([ctx.download(c) for c in range(2)])
     for c in range(2):
         assert np.array_equal(outs[0][c], outs[1][c]), (slab, c)
+
+
+def test_default_slab_schedule_at_256_cubed_is_bit_identical(hip):
+    # the automatic schedule (slabs of about 80 MiB on two streams) engages
+    # at the benchmark's size; two 50-step Bragg blocks equal whole-grid
+    # passes bit for bit
+    n, L = 256, 64.0
+    x = (np.arange(n) - n // 2) * (L / n)
+    g1 = np.exp(-x ** 2 / 50.0)
+    psi = (g1[:, None, None] * g1[None, :, None] * np.exp(-x ** 2 / 72.0)[None, None, :]).astype(np.complex128)
+    psi /= np.sqrt(np.sum(np.abs(psi) ** 2) * (L / n) ** 3)
+    las = hip.laser(amp0=5.0, laser_k=2.0, laser_domh=8.0, deltaL=(0.0, 0.0))
+    amp = lambda t: float(np.exp(-0.5 * ((t - 0.01) / 0.005) ** 2))
+    outs = []
+    for slab in (-1, 0):
+        with hip.Context((n, n, n), [L / n] * 3, [2 * np.pi / L] * 3, 2) as ctx:
+            ctx.upload(0, psi)
+            ctx.upload(1, np.zeros_like(psi))
+            ctx.set_alpha([0.5] * 3)
+            ctx.set_g([100.0, 95.0, 95.0, 100.0])
+            ctx.set_slab(slab)
+            t = 0.0
+            for _ in range(2):
+                t = ctx.run_block(hip.STEP_BRAGG_AD, 2e-4, 50, t, las, amp)
+            outs.append([ctx.download(c) for c in range(2)])
+    for c in range(2):
+        assert np.array_equal(outs[0][c], outs[1][c]), c
