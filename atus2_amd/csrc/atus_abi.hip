@@ -565,10 +565,13 @@ int atus_ctx_destroy(atus_ctx* c) {
   (void)hipFree(c->sred_out);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
-  if (c->stream2) (void)hipStreamSynchronize(c->stream2);
+  for (hipStream_t x : c->xstream)
+    if (x) (void)hipStreamSynchronize(x);
   if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
-  if (c->join_ev) (void)hipEventDestroy(c->join_ev);
-  if (c->stream2) (void)hipStreamDestroy(c->stream2);
+  for (hipEvent_t e : c->join_ev)
+    if (e) (void)hipEventDestroy(e);
+  for (hipStream_t x : c->xstream)
+    if (x) (void)hipStreamDestroy(x);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return ATUS_OK;
@@ -807,24 +810,31 @@ int single_block_slabs(atus_ctx* c, int vk, const VArgs& base, int Nk, double dt
   };
   // odd slabs on the second stream (profiled blocks record their events on
   // each launch's stream)
-  const bool two = c->slab_streams > 1;
-  if (two && !c->stream2) {
-    if (hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming) != hipSuccess)
-      return fail(ATUS_ERR_HIP, "second stream of the slab chains");
+  const int K = std::max(1, std::min(c->slab_streams, atus_ctx::MAX_SLAB_STREAMS));
+  if (K > 1 && !c->fork_ev &&
+      hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming) != hipSuccess)
+    return fail(ATUS_ERR_HIP, "fork event of the slab chains");
+  for (int k = 0; k + 1 < K; ++k) {
+    if (c->xstream[k]) continue;
+    if (hipStreamCreateWithFlags(&c->xstream[k], hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->join_ev[k], hipEventDisableTiming) != hipSuccess)
+      return fail(ATUS_ERR_HIP, "extra stream of the slab chains");
   }
-  auto st_of = [&](int s0) { return (two && (s0 / S) % 2 == 1) ? c->stream2 : c->stream; };
+  auto st_of = [&](int s0) {
+    const int k = (s0 / S) % K;
+    return k == 0 ? c->stream : c->xstream[k - 1];
+  };
   auto fork = [&]() -> int {
-    if (!two) return ATUS_OK;
+    if (K == 1) return ATUS_OK;
     HIP_TRY(hipEventRecord(c->fork_ev, c->stream));
-    HIP_TRY(hipStreamWaitEvent(c->stream2, c->fork_ev, 0));
+    for (int k = 0; k + 1 < K; ++k) HIP_TRY(hipStreamWaitEvent(c->xstream[k], c->fork_ev, 0));
     return ATUS_OK;
   };
   auto join = [&]() -> int {
-    if (!two) return ATUS_OK;
-    HIP_TRY(hipEventRecord(c->join_ev, c->stream2));
-    HIP_TRY(hipStreamWaitEvent(c->stream, c->join_ev, 0));
+    for (int k = 0; k + 1 < K; ++k) {
+      HIP_TRY(hipEventRecord(c->join_ev[k], c->xstream[k]));
+      HIP_TRY(hipStreamWaitEvent(c->stream, c->join_ev[k], 0));
+    }
     return ATUS_OK;
   };
   double tt = *t;

@@ -52,10 +52,13 @@ struct atus_ctx {
   double2* tbuf = nullptr;  // ncomp * npts, the transposes of full transforms
   int device = 0;
   hipStream_t stream = nullptr;
-  // second stream of the slab chains (created on first use, default
-  // priority) and the fork / join events between the two
-  hipStream_t stream2 = nullptr;
-  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+  // extra streams of the slab chains (created on first use, default
+  // priority), the fork event on `stream` and one join event per extra
+  // stream; slab i runs on stream i % slab_streams (0: `stream`)
+  static constexpr int MAX_SLAB_STREAMS = 4;
+  hipStream_t xstream[MAX_SLAB_STREAMS - 1] = {nullptr, nullptr, nullptr};
+  hipEvent_t fork_ev = nullptr;
+  hipEvent_t join_ev[MAX_SLAB_STREAMS - 1] = {nullptr, nullptr, nullptr};
   int slab_streams = 2;
   size_t max_lds = 65536;
 
