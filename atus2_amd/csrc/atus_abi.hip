@@ -563,6 +563,9 @@ int atus_ctx_destroy(atus_ctx* c) {
   (void)hipFree(c->kin2);
   (void)hipFree(c->sred_part);
   (void)hipFree(c->sred_out);
+  (void)hipFree(c->blk_dev);
+  if (c->blk_host) (void)hipHostFree(c->blk_host);
+  if (c->blk_ev) (void)hipEventDestroy(c->blk_ev);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   for (hipStream_t x : c->xstream)
@@ -872,12 +875,79 @@ int single_block_slabs(atus_ctx* c, int vk, const VArgs& base, int Nk, double dt
   return ATUS_OK;
 }
 
+// room for n doubles in the block tables, the host side free to rewrite
+int block_tables(atus_ctx* c, size_t n) {
+  if (c->blk_ev) HIP_TRY(hipEventSynchronize(c->blk_ev));
+  if (!c->blk_ev && hipEventCreateWithFlags(&c->blk_ev, hipEventDisableTiming) != hipSuccess)
+    return fail(ATUS_ERR_HIP, "event of the block tables");
+  if (c->blk_cap >= n) return ATUS_OK;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  (void)hipFree(c->blk_dev);
+  if (c->blk_host) (void)hipHostFree(c->blk_host);
+  c->blk_dev = nullptr;
+  c->blk_host = nullptr;
+  c->blk_cap = 0;
+  if (hipMalloc(&c->blk_dev, sizeof(double) * n) != hipSuccess ||
+      hipHostMalloc(&c->blk_host, sizeof(double) * n, hipHostMallocDefault) != hipSuccess)
+    return fail(ATUS_ERR_NOMEM, "block tables");
+  c->blk_cap = n;
+  return ATUS_OK;
+}
+
+// whether a block of this context runs as one k_block1d launch
+bool resident_ok(const atus_ctx* c) {
+  return c->resident && c->dim == 1 && !c->fs && c->n[0] <= 4096;
+}
+
+// a 1D block in one launch: the t1 (and, for a single run, the amplitude)
+// of every step are evaluated with the eager block's bookkeeping and
+// uploaded first, unless a replayed graph supplies them; fbatch: the
+// batch's amplitude table [Nk][runs], or null for a single run
+int resident_block(atus_ctx* c, int vk, const VArgs& base, int Nk, double dt, double* t, const StepFeed& fd,
+                   bool batch) {
+  const int N0 = c->n[0];
+  double tt = *t + 0.5 * dt;
+  const double* t1tab = fd.t1dev;
+  const double* Ftab = batch ? fd.fbatch : fd.fdev;
+  if (!t1tab) {
+    ATUS_TRY(block_tables(c, 2 * (size_t)Nk));
+    double tc = tt;
+    for (int m = 1; m <= Nk; ++m) {
+      if (batch) {
+        c->blk_host[m - 1] = tc + 0.5 * dt;
+      } else {
+        VArgs va;
+        ATUS_TRY(step_vargs(base, m, tc, dt, fd, va));
+        c->blk_host[m - 1] = va.t1;
+        c->blk_host[Nk + m - 1] = va.F;
+      }
+      tc += (m < Nk) ? dt : 0.5 * dt;
+    }
+    HIP_TRY(hipMemcpyAsync(c->blk_dev, c->blk_host, sizeof(double) * (batch ? Nk : 2 * Nk), hipMemcpyHostToDevice,
+                           c->stream));
+    HIP_TRY(hipEventRecord(c->blk_ev, c->stream));
+    t1tab = c->blk_dev;
+    if (!batch) Ftab = c->blk_dev + Nk;
+  }
+  if (!t1tab || !Ftab || (batch && !base.runs)) return fail(ATUS_ERR_STATE, "resident block without its step tables");
+  {
+    ProfScope ps(c, ATUS_KFAM_KIN, Nk + 1.0);
+    HIP_TRY(launch_block1d(N0, c->rm[ATUS_KFAM_KIN], vk, batch ? 2 : c->ncomp, batch, c->psi, c->npts,
+                           batch ? c->nbatch : 1, c->tw[N0], kin_tabs(c, 0).e0, kin_tabs(c, 1).e0, base, t1tab, Ftab,
+                           c->dx[0], Nk, c->max_lds, c->stream));
+  }
+  for (int m = 1; m <= Nk; ++m) tt += (m < Nk) ? dt : 0.5 * dt;
+  *t = tt;
+  return ATUS_OK;
+}
+
 int single_block(atus_ctx* c, int vk, const VArgs& base, int Nk, double dt, double* t, const StepFeed& fd) {
   // the fused kernels cover one species of up to two components; two
   // species (per-component kinetic tables, four-component steps) run the
   // V step and the kinetic step as separate passes
   const bool fused = c->ncomp <= 2 && !c->alpha2_set && (vk == V_NL || vk == V_NL_IF || vk == V_BRAGG);
   if (fused && c->slab_planes() > 0) return single_block_slabs(c, vk, base, Nk, dt, t, fd);
+  if (fused && !fd.ds && resident_ok(c)) return resident_block(c, vk, base, Nk, dt, t, fd, false);
   const int N0 = c->n[0], NL = c->n[c->dim - 1];
   const long long inner0 = c->inner_of(0);
   double tt = *t;
@@ -952,6 +1022,7 @@ void batch_vargs(atus_ctx* c, VArgs& base, double dt) {
 // the launches of one block of every run of a batch, without synchronisation
 int batch_block(atus_ctx* c, int vk, const VArgs& base, int Nk, double dt, double* t, const StepFeed& fd) {
   const int B = c->nbatch;
+  if (resident_ok(c)) return resident_block(c, vk, base, Nk, dt, t, fd, true);
   const int N0 = c->n[0], NL = c->n[c->dim - 1];
   const long long inner0 = c->inner_of(0);
   double tt = *t;
@@ -1511,6 +1582,13 @@ int atus_set_slab(atus_ctx* c, int planes) {
   ATUS_TRY(check_ctx(c));
   if (planes < -1) return fail(ATUS_ERR_ARG, "slab planes must be -1 (automatic), 0 (off) or positive");
   c->slab = planes;
+  return ATUS_OK;
+}
+
+int atus_set_resident(atus_ctx* c, int on) {
+  ATUS_TRY(check_ctx(c));
+  if (on != 0 && on != 1) return fail(ATUS_ERR_ARG, "resident blocks: 0 (off) or 1 (on)");
+  c->resident = on;
   return ATUS_OK;
 }
 

@@ -122,6 +122,16 @@ struct atus_ctx {
     }
     return s < n[0] ? s : 0;
   }
+  // 1D blocks of up to 4096 points in one launch, each run's pencil
+  // resident in registers for all steps (k_block1d); 0: a launch per step
+  int resident = 1;
+  // per-step t1 and amplitudes of a resident block: pinned host staging,
+  // the device table, and the event after its upload (the host side is
+  // rewritten only once that upload has completed)
+  double* blk_host = nullptr;
+  double* blk_dev = nullptr;
+  size_t blk_cap = 0;
+  hipEvent_t blk_ev = nullptr;
 
   // per-launch event pairs while profiling
   bool prof = false;

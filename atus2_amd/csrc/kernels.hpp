@@ -766,6 +766,80 @@ __global__ __launch_bounds__(256) void k_kin(double2* __restrict__ psi, long lon
   }
 }
 
+// ------------------------------------------------------------ k_block1d
+// A whole block of 1D runs in one launch (N <= 4096: each run's pencil
+// stays in the registers of its threads for all Nk steps): the half
+// kinetic step, then Nk times the V step followed by the full kinetic step
+// (the last one half), as the eager block launches them, with k_kin's
+// arithmetic (the same plan, V step and phase multiply), so the results
+// are bit-identical.  eh / ef: phase tables of the half and the full step.
+// Step m reads its t1 from t1tab[m-1] and its amplitude from Ftab[m-1]
+// (single run) or Ftab[(m-1)*runs + run] (BATCH: pencil r is run r).
+template <int N, int RM, int VK, int NC, bool BATCH>
+__global__ __launch_bounds__(256) void k_block1d(double2* __restrict__ psi, long long comp_stride, AxisGeo g,
+                                                 const double2* __restrict__ tw, const double2* __restrict__ eh,
+                                                 const double2* __restrict__ ef, VArgs va,
+                                                 const double* __restrict__ t1tab, const double* __restrict__ Ftab,
+                                                 double x0_dx, int Nk) {
+  extern __shared__ double2 lds_all[];
+  constexpr int R = Plan<N, RM>::R, P = Plan<N, RM>::P;
+  constexpr bool WAVE = P <= 64;
+  PencilMap<N, RM, true> m(g);
+  double2* lds = lds_all + m.p * lds_stride<N>();
+  double2* tab = lds_all + (size_t)g.T * lds_stride<N>();
+  const double2* twl = stage_table<N>(tab, tw);
+  const double2* ehl = stage_table<N>(tab + N, eh);
+  const double2* efl = stage_table<N>(tab + 2 * N, ef);
+  long long c0 = 0;
+  if constexpr (BATCH) {
+    // bind_run takes the amplitude from the row of Fm: the first step's
+    // (every step sets its own below)
+    va.Fm = Ftab;
+    if (m.valid) bind_run(va, m.pid);
+    c0 = m.pid * va.run_shift;
+  }
+  double2 v[NC][R];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) load_pencil<N, RM>(v[c], psi + c0 + c * comp_stride, m.base, 1, m.t, m.valid);
+  if constexpr (tables_in_lds<N>()) __syncthreads();
+  auto kinetic = [&](const double2* e) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      fft_pencil<N, RM, -1, WAVE>(v[c], m.t, lds, twl);
+#pragma unroll
+      for (int i = 0; i < R; ++i) v[c][i] = cmul(v[c][i], e[m.t + P * i]);
+      fft_pencil<N, RM, +1, WAVE>(v[c], m.t, lds, twl);
+    }
+  };
+  kinetic(ehl);
+  for (int s = 1; s <= Nk; ++s) {
+    va.t1p = t1tab + (s - 1);
+    if constexpr (BATCH) {
+      va.Fp = nullptr;
+      if (m.valid) va.F = Ftab[(size_t)(s - 1) * g.outer + m.pid];
+    } else {
+      va.Fp = Ftab + (s - 1);
+    }
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      const int j = m.t + P * i;
+      const double x0 = (double)(j - N / 2) * x0_dx;
+      double2 a[NC];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) a[c] = v[c][i];
+      double Om = 0.0;
+      double2 eta = make_double2(1.0, 0.0);
+      if constexpr (VK == V_BRAGG) bragg_field(x0, va, Om, eta);
+      if (m.valid) v_point<VK, NC>(a, x0, m.base + j, va, Om, eta);
+#pragma unroll
+      for (int c = 0; c < NC; ++c) v[c][i] = a[c];
+    }
+    kinetic(s < Nk ? efl : ehl);
+  }
+#pragma unroll
+  for (int c = 0; c < NC; ++c) store_pencil<N, RM>(v[c], psi + c0 + c * comp_stride, m.base, 1, m.t, m.valid);
+}
+
 // ------------------------------------------------------------ k_vfuse
 // inverse DFT along the last axis, V step, forward DFT along the last axis.
 // The pencil (contiguous) has fixed coordinates on the other axes, so

@@ -61,6 +61,15 @@ hipError_t launch_vstep_batch(int N, int rm, bool one_d, int vk, double2* psi, l
                               const double2* tw, const KinTabs& kt, const VArgs& va, long long pencils_per_x0, int n0,
                               double dx0, size_t max_lds, hipStream_t s);
 
+// a whole block of a 1D grid (N <= 4096) in one launch: one run (nc
+// components, runs = 1) or a batch of `runs` two-component runs; eh / ef
+// the half and full kinetic phase tables, t1tab [Nk] and Ftab ([Nk] or
+// [Nk][runs]) the per-step t1 and amplitudes in device memory
+hipError_t launch_block1d(int N, int rm, int vk, int nc, bool batch, double2* psi, long long comp_stride,
+                          long long runs, const double2* tw, const double2* eh, const double2* ef, const VArgs& va,
+                          const double* t1tab, const double* Ftab, double dx0, int Nk, size_t max_lds,
+                          hipStream_t s);
+
 // column pass of the four-step 1D transform (k_fs_col): n1 in 64..1024,
 // mode FS_ENTER / FS_EXIT / FS_STEP; ycount components (NC == 1, no
 // batch), runs (batch) or 1

@@ -28,7 +28,7 @@ EXPORTED = [
     "atus_batch_rabi_integrals", "atus_sob_run", "atus_graph_create", "atus_graph_launch", "atus_graph_destroy",
     "atus_set_alpha2", "atus_run_block_ds", "atus_v_step_ds",
     "atus_snapshot_save", "atus_snapshot_restore", "atus_copy_components", "atus_v_step",
-    "atus_get_moments", "atus_inflate", "atus_set_slab",
+    "atus_get_moments", "atus_inflate", "atus_set_slab", "atus_set_resident",
     "atus_net_create", "atus_net_destroy", "atus_net_set_precision", "atus_net_param_count", "atus_net_get_params", "atus_net_set_params",
     "atus_net_forward", "atus_net_gradients", "atus_net_set_data", "atus_net_train", "atus_net_warm_start",
 ]
@@ -133,6 +133,7 @@ def load():
         "atus_get_moments": (I, [P, I, C.POINTER(Moments)]),
         "atus_inflate": (I, [P, P, I, I]),
         "atus_set_slab": (I, [P, I]),
+        "atus_set_resident": (I, [P, I]),
         "atus_snapshot_restore": (I, [P, I]),
         "atus_copy_components": (I, [P, I, P, I, I]),
         "atus_net_create": (I, [pctx, I, I, I, I, I, I, C.c_ulonglong]),
@@ -436,6 +437,11 @@ class Context:
         """3D blocks: y/z passes per slab of `planes` x-planes (-1 automatic,
         0 whole grid)."""
         _check(_lib.atus_set_slab(self._h, int(planes)))
+
+    def set_resident(self, on):
+        """1D blocks in one launch with the pencils resident (True, the
+        default) or a launch per step (False)."""
+        _check(_lib.atus_set_resident(self._h, 1 if on else 0))
 
     def copy_bandwidth(self, reps=20):
         """GB/s of a plain streaming copy of all components (HIP events)."""

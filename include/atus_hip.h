@@ -250,6 +250,14 @@ int atus_tune(atus_ctx *ctx, int family, int values_per_thread);
  * slabs of about 80 MiB of state; 0: whole-grid passes on one stream.
  * The results do not change. */
 int atus_set_slab(atus_ctx *ctx, int planes);
+/* scheduling of 1D blocks (no reference counterpart): grids of up to 4096
+ * points (single runs of one or two components, and batches) run a whole
+ * block of atus_run_block / atus_batch_run_block / a graph in one launch,
+ * each run's pencil resident in registers from the first half kinetic step
+ * to the last; the per-step t1 and amplitudes are evaluated on the host
+ * with the per-step bookkeeping and uploaded before the launch.  on = 1
+ * (default) or 0 (a launch per step).  The results do not change. */
+int atus_set_resident(atus_ctx *ctx, int on);
 
 /* HBM copy probe: copies all components (ncomp * npts complex128) into a
  * scratch buffer `reps` times with a plain streaming kernel and returns the
