@@ -94,3 +94,81 @@ class BraggWorkload:
         s = n / 256.0
         return cls(n=n, L=64.0 * s, widths=tuple(w * s for w in cls.widths),
                    gs=tuple(g * s ** 3 for g in cls.gs))
+
+
+@dataclass
+class HarmonicLineWorkload:
+    """1D one-component rt_solver run: N = 4096 on [-40,40), V = x^2/2,
+    g = 10, alpha = 1/2, psi0 = pi^-1/4 exp(-x^2/2) plus seeded 1e-3 complex
+    noise, normalised to 1; freeprop at dt = 1e-3 in blocks of Nk = 100,
+    norm and energy after every block (rt_solver.cpp:74-131,
+    CRT_Base.h:397-441, 685-841)."""
+    n: int = 4096
+    L: float = 80.0
+    g: float = 10.0
+    dt: float = 1e-3
+    Nk: int = 100
+    seed: int = 1
+    shape: tuple = field(init=False)
+
+    def __post_init__(self):
+        self.shape = (self.n,)
+
+    @property
+    def spacings(self):
+        return grid_spacings(self.shape, self.L)
+
+    def potential(self):
+        x = axis_coords(self.n, self.L)
+        return 0.5 * x ** 2
+
+    def initial_state(self):
+        x = axis_coords(self.n, self.L)
+        rng = np.random.default_rng(self.seed)
+        psi = np.pi ** -0.25 * np.exp(-0.5 * x ** 2) + 1e-3 * (rng.standard_normal(self.n)
+                                                             + 1j * rng.standard_normal(self.n))
+        psi = psi.astype(np.complex128)
+        dx, _ = self.spacings
+        psi /= math.sqrt(particle_number(psi, dx))
+        return psi
+
+
+@dataclass
+class DoubleSlitWorkload:
+    """2D one-component rt_solver run: 2048^2 on [-64,64)^2, a wall
+    V = 1e3 rect(x,-0.5,0.5) (1 - rect(y,-5,-3) - rect(y,3,5)) (rect is 1 on
+    [xl,xr), misc.cpp:40-53), psi0 = exp(-((x+20)^2 + y^2)/(2*4^2))
+    normalised, then the momentum (3, 0) (CRT_Base.h:449-473); g = 1,
+    alpha = 1/2, freeprop at dt = 5e-4 in blocks of Nk = 100."""
+    n: int = 2048
+    L: float = 128.0
+    g: float = 1.0
+    dt: float = 5e-4
+    Nk: int = 100
+    momentum: tuple = (3.0, 0.0)
+    shape: tuple = field(init=False)
+
+    def __post_init__(self):
+        self.shape = (self.n, self.n)
+
+    @property
+    def spacings(self):
+        return grid_spacings(self.shape, self.L)
+
+    def _xy(self):
+        x = axis_coords(self.n, self.L)
+        return np.broadcast_arrays(x[:, None], x[None, :])
+
+    def potential(self):
+        X, Y = self._xy()
+
+        def rect(v, lo, hi):
+            return ((v >= lo) & (v < hi)).astype(np.float64)
+        return 1e3 * rect(X, -0.5, 0.5) * (1.0 - rect(Y, -5.0, -3.0) - rect(Y, 3.0, 5.0))
+
+    def initial_state(self):
+        X, Y = self._xy()
+        psi = np.exp(-((X + 20.0) ** 2 + Y ** 2) / (2 * 4.0 ** 2)).astype(np.complex128)
+        dx, _ = self.spacings
+        psi /= math.sqrt(particle_number(psi, dx))
+        return psi

@@ -112,6 +112,72 @@ def gpu_run(args, wl):
     }
 
 
+def rt_solver_run(wl, steps, energy):
+    """One-component rt_solver freeprop (V + g|psi|^2, kinetic step) on the
+    GPU: `steps` steps in blocks of Nk after one untimed block, the norm
+    (and the energy) after every block."""
+    from atus2_amd import hip
+    dx, dk = wl.spacings
+    npts = float(math.prod(wl.shape))
+    ctx = hip.Context(wl.shape, dx, dk, 1)
+    try:
+        ctx.upload(0, wl.initial_state())
+        ctx.set_g([wl.g])
+        ctx.set_alpha([0.5] * len(wl.shape))
+        ctx.set_potential(0, wl.potential())
+        if getattr(wl, "momentum", None):
+            ctx.set_momentum(0, wl.momentum)
+        N0 = ctx.particle_number(0)
+        t = ctx.run_block(hip.STEP_FREEPROP, wl.dt, wl.Nk, 0.0)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        left = steps
+        while left > 0:
+            nk = min(wl.Nk, left)
+            t = ctx.run_block(hip.STEP_FREEPROP, wl.dt, nk, t)
+            ctx.particle_number(0)
+            if energy:
+                ctx.energy()
+            left -= nk
+        ctx.synchronize()
+        elapsed = time.perf_counter() - t0
+        drift = abs(ctx.particle_number(0) - N0) / (steps + wl.Nk)
+        t, prof = ctx.run_block_profiled(hip.STEP_FREEPROP, wl.dt, wl.Nk, t)
+    finally:
+        ctx.close()
+    kernels = {k: {"avg_us": 1e3 * v[0] / v[1], "launches": v[1]} for k, v in prof.items() if v[1] > 0}
+    return npts * steps / elapsed / 1e9, 1e3 * elapsed / steps, drift, kernels
+
+
+def other_configs_run():
+    """The reference's 1D (4096, V = x^2/2, g = 10) and 2D (2048^2 double
+    slit) rt_solver workloads, with the bytes per point and step of the
+    reference's unfused split step (one nonlinear sweep and two sweeps per
+    axis, plus the potential) and of this design (one pass per axis, the
+    nonlinear step and the kinetic phase fused into them, plus the
+    potential)."""
+    from atus2_amd.workloads import DoubleSlitWorkload, HarmonicLineWorkload
+    out = []
+    for wl, steps, energy, name in (
+            (HarmonicLineWorkload(), 1000, True,
+             "1D rt_solver: 4096 on [-40,40), V = x^2/2, g = 10, dt = 1e-3, norm and energy per 100 steps"),
+            (DoubleSlitWorkload(), 500, False,
+             "2D rt_solver double slit: 2048^2 on [-64,64)^2, g = 1, dt = 5e-4, norm per 100 steps")):
+        d = len(wl.shape)
+        ref_bytes = (1 + 2 * d) * 32 + 8
+        own_bytes = d * 32 + 8
+        value, ms, drift, kernels = rt_solver_run(wl, steps, energy)
+        npts = float(math.prod(wl.shape))
+        gbps = own_bytes * npts / (ms * 1e-3) / 1e9
+        log(f"[bench] {name}: {value:.3f} Gpt-steps/s, {ms * 1e3:.1f} us per step")
+        out.append({"workload": name, "grid": list(wl.shape), "steps": steps, "value": value,
+                    "unit": "Gpt-steps/s", "ms_per_step": ms, "norm_drift_per_step": drift,
+                    "bytes_per_point_step": {"reference_model": ref_bytes, "this_design": own_bytes},
+                    "achieved_GBps": gbps, "frac": gbps / HBM_PEAK_GBS,
+                    "state_MiB": npts * 24 / 2 ** 20, "kernels": kernels})
+    return out
+
+
 def measured_traffic(n, family, values_per_thread):
     """HBM bytes per launch measured by counters for this family, or None
     when the table does not cover the configuration."""
@@ -173,6 +239,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tuning", default="", help="fixed values per thread, e.g. pass_strided=8,vfuse=8")
     ap.add_argument("--gs-cases", type=int, default=4, help="ground states timed cold and warm (0: none)")
+    ap.add_argument("--no-other-configs", action="store_true",
+                    help="skip the 1D 4096 and 2D 2048^2 rt_solver workloads")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -195,6 +263,7 @@ def main():
     ys = measured_traffic(wl.n, "pass_strided", r["tuning"].get("pass_strided", DEFAULT_RM["pass_strided"]))
     step_traffic = None if None in fam_traffic or ys is None else sum(fam_traffic) + 2 * ys
     gs = ground_state_run(args.gs_cases) if args.gs_cases > 0 else None
+    others = None if args.no_other_configs else other_configs_run()
     cpu = None
     if not args.no_cpu_baseline:
         log(f"[bench] CPU baseline: oracle, {args.cpu_steps} steps")
@@ -224,12 +293,18 @@ def main():
                      "kernel": "3D step: k_vfuse + k_pass (y) x 2 + k_kin, y/z passes slab-chained on two streams",
                      "avg_us": 1e6 * r["elapsed"] / args.steps, "algorithmic_bytes_per_launch": step_bytes,
                      "measured_copy_GBps": r["copy_gbps"], "copy_frac": step_gbps / r["copy_gbps"],
+                     # the reference's unfused split step moves C (1 + 2d) 32 B
+                     # per point (448 B at d = 3, C = 2); the same step rate
+                     # expressed in those bytes
+                     "reference_model": {"bytes_per_point_step": 448,
+                                         "effective_GBps": 448 * npts / (r["elapsed"] / args.steps) / 1e9},
                      "dominant_kernel": {"kernel": r["dominant"], "avg_us": 1e3 * r["dom_avg_ms"],
                                          "algorithmic_bytes_per_launch": r["bytes_per_launch"],
                                          "achieved": r["achieved"], "frac": r["achieved"] / HBM_PEAK_GBS,
                                          "traffic": traffic}},
         "cpu_baseline": cpu,
         "ground_state": gs,
+        "other_configs": others,
         "kernels": r["kernels"],
         "tuning": r["tuning"],
         "checks": {"norm_drift_per_step": r["norm_drift_per_step"], "rabi_last_block": r["rabi"]},
