@@ -3,16 +3,18 @@
 
 namespace atus {
 
-template <int N, int RM, bool CONTIG, int VK, int NC>
+// LB: threads per workgroup at most (the kernel's launch bounds)
+template <int N, int RM, bool CONTIG, int VK, int NC, int LB = 256>
 static hipError_t kin_go(double2* psi, long long cs, int ycount, long long inner, long long outer,
                          const double2* tw, const KinTabs& kt, const VArgs& va, double dx0, size_t max_lds,
                          hipStream_t s) {
-  PassCfg c = make_pass_cfg(N, RM, inner, outer, ycount, max_lds, 2);
-  if (c.block.x == 0 || c.block.x > 256 || c.grid.x == 0) return hipErrorInvalidConfiguration;
+  PassCfg c = make_pass_cfg(N, RM, inner, outer, ycount, max_lds, 2, LB);
+  if (c.block.x == 0 || c.block.x > LB || c.grid.x == 0) return hipErrorInvalidConfiguration;
   static bool ok = false;
-  hipError_t e = allow_lds(k_kin<N, RM, CONTIG, VK, NC>, c.lds, ok);
+  hipError_t e = allow_lds(k_kin<N, RM, CONTIG, VK, NC, false, LB>, c.lds, ok);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((k_kin<N, RM, CONTIG, VK, NC>), c.grid, c.block, c.lds, s, psi, cs, c.g, tw, kt, va, dx0);
+  hipLaunchKernelGGL((k_kin<N, RM, CONTIG, VK, NC, false, LB>), c.grid, c.block, c.lds, s, psi, cs, c.g, tw, kt, va,
+                     dx0);
   return hipGetLastError();
 }
 
@@ -23,6 +25,10 @@ static hipError_t kin_nr(bool contig, int vk, int nc, double2* psi, long long cs
   if (!contig) {
     // 2D/3D: axis 0 is strided, one component per blockIdx.y, no V prologue
     if (vk != V_NONE) return hipErrorInvalidValue;
+    // columns of 2048 and more: workgroups of up to 512 threads hold twice
+    // the columns, so each row is read in segments twice as long
+    if constexpr (N >= 2048)
+      return kin_go<N, RM, false, V_NONE, 1, 512>(psi, cs, ncomp, inner, 1, tw, kt, va, dx0, max_lds, s);
     return kin_go<N, RM, false, V_NONE, 1>(psi, cs, ncomp, inner, 1, tw, kt, va, dx0, max_lds, s);
   }
   // 1D: the pencil is the whole grid

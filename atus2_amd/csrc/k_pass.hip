@@ -29,9 +29,9 @@ PassCfg make_pass_cfg(int N, int rm, long long inner, long long outer, int ycoun
     c.g.ntiles = outer * c.g.col_tiles;
     c.g.T = (int)T;
   }
-  // more than 256 threads per pencil cannot be launched (launch bounds);
-  // such a configuration is returned with an empty grid and rejected
-  if (T * P > 256) T = 0;
+  // more threads than the kernel's launch bounds cannot be launched; such
+  // a configuration is returned with an empty grid and rejected
+  if (T * P > max_threads) T = 0;
   c.block = dim3((unsigned)(T * P));
   c.grid = dim3((unsigned)c.g.ntiles, (unsigned)ycount);
   c.lds = (size_t)T * pencil_bytes + table_bytes;

@@ -690,8 +690,8 @@ struct KinTabs {
   int split;
 };
 
-template <int N, int RM, bool CONTIG, int VK, int NC, bool BATCH = false>
-__global__ __launch_bounds__(256) void k_kin(double2* __restrict__ psi, long long comp_stride, AxisGeo g,
+template <int N, int RM, bool CONTIG, int VK, int NC, bool BATCH = false, int LB = 256>
+__global__ __launch_bounds__(LB) void k_kin(double2* __restrict__ psi, long long comp_stride, AxisGeo g,
                                              const double2* __restrict__ tw, KinTabs kt, VArgs va, double x0_dx) {
   extern __shared__ double2 lds_all[];
   constexpr int R = Plan<N, RM>::R, P = Plan<N, RM>::P;
