@@ -256,7 +256,9 @@ int atus_set_slab(atus_ctx *ctx, int planes);
  * each run's pencil resident in registers from the first half kinetic step
  * to the last; the per-step t1 and amplitudes are evaluated on the host
  * with the per-step bookkeeping and uploaded before the launch.  on = 1
- * (default) or 0 (a launch per step).  The results do not change. */
+ * (default) or 0 (a launch per step).  The two agree to rounding: the
+ * arithmetic is the same, but multiply-adds are fused differently inside
+ * the kernel's step loop. */
 int atus_set_resident(atus_ctx *ctx, int on);
 
 /* HBM copy probe: copies all components (ncomp * npts complex128) into a
