@@ -7,15 +7,18 @@ template <int N, int RM, int VK, int NC, bool BATCH>
 static hipError_t block_go(double2* psi, long long cs, long long runs, const double2* tw, const double2* eh,
                            const double2* ef, const VArgs& va, const double* t1tab, const double* Ftab, double dx0,
                            int Nk, size_t max_lds, hipStream_t s) {
+  // pencils of more than 256 threads (4096 points at 8 values per thread)
+  // take workgroups of 512
+  constexpr int LB = Plan<N, RM>::P > 256 ? 512 : 256;
   // a single run: one pencil per workgroup; a batch: as many pencils as
   // the workgroup holds
-  const int max_threads = BATCH ? 256 : threads_per_pencil(N, RM);
+  const int max_threads = BATCH ? LB : threads_per_pencil(N, RM);
   PassCfg c = make_pass_cfg(N, RM, 1, runs, 1, max_lds, 3, max_threads);
-  if (c.block.x == 0 || c.block.x > 256 || c.grid.x == 0 || c.lds > max_lds) return hipErrorInvalidConfiguration;
+  if (c.block.x == 0 || c.block.x > LB || c.grid.x == 0) return hipErrorInvalidConfiguration;
   static bool ok = false;
-  hipError_t e = allow_lds(k_block1d<N, RM, VK, NC, BATCH>, c.lds, ok);
+  hipError_t e = allow_lds(k_block1d<N, RM, VK, NC, BATCH, LB>, c.lds, ok);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((k_block1d<N, RM, VK, NC, BATCH>), c.grid, c.block, c.lds, s, psi, cs, c.g, tw, eh, ef, va,
+  hipLaunchKernelGGL((k_block1d<N, RM, VK, NC, BATCH, LB>), c.grid, c.block, c.lds, s, psi, cs, c.g, tw, eh, ef, va,
                      t1tab, Ftab, dx0, Nk);
   return hipGetLastError();
 }
@@ -47,14 +50,14 @@ static hipError_t block_nr(int vk, int nc, bool batch, double2* psi, long long c
   return hipErrorInvalidValue;
 }
 
-// the same choice of values per thread as launch_kin, so that the plan and
-// with it the rounding are those of the eager block
+// values per thread as tuned for the kinetic family (8 or 16); 8 also at
+// 4096 points, where the per-step kinetic pass falls back to 16
 template <int N>
 static hipError_t block_n(int rm, int vk, int nc, bool batch, double2* psi, long long cs, long long runs,
                           const double2* tw, const double2* eh, const double2* ef, const VArgs& va,
                           const double* t1tab, const double* Ftab, double dx0, int Nk, size_t max_lds,
                           hipStream_t s) {
-  if constexpr (N > 8 && N / 8 <= 256) {
+  if constexpr (N > 8 && N / 8 <= 512) {
     if (rm == 8) return block_nr<N, 8>(vk, nc, batch, psi, cs, runs, tw, eh, ef, va, t1tab, Ftab, dx0, Nk, max_lds, s);
   }
   return block_nr<N, 16>(vk, nc, batch, psi, cs, runs, tw, eh, ef, va, t1tab, Ftab, dx0, Nk, max_lds, s);

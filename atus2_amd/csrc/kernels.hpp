@@ -775,8 +775,8 @@ __global__ __launch_bounds__(LB) void k_kin(double2* __restrict__ psi, long long
 // are bit-identical.  eh / ef: phase tables of the half and the full step.
 // Step m reads its t1 from t1tab[m-1] and its amplitude from Ftab[m-1]
 // (single run) or Ftab[(m-1)*runs + run] (BATCH: pencil r is run r).
-template <int N, int RM, int VK, int NC, bool BATCH>
-__global__ __launch_bounds__(256) void k_block1d(double2* __restrict__ psi, long long comp_stride, AxisGeo g,
+template <int N, int RM, int VK, int NC, bool BATCH, int LB = 256>
+__global__ __launch_bounds__(LB) void k_block1d(double2* __restrict__ psi, long long comp_stride, AxisGeo g,
                                                  const double2* __restrict__ tw, const double2* __restrict__ eh,
                                                  const double2* __restrict__ ef, VArgs va,
                                                  const double* __restrict__ t1tab, const double* __restrict__ Ftab,
