@@ -52,7 +52,9 @@ def test_ft_matches_oracle(hip, orc, shape, rm):
     assert rel_l2(back, psi) < 1e-13
 
 
-@pytest.mark.parametrize("shape", [(64,), (4096,), (8192,), (65536,), (32, 64), (8, 16, 32), (64, 64, 64)])
+# (2048, 4), (4096, 2): strided kinetic passes in 512-thread workgroups
+@pytest.mark.parametrize("shape", [(64,), (4096,), (8192,), (65536,), (32, 64), (8, 16, 32), (64, 64, 64),
+                                   (1024, 8), (2048, 4), (4096, 2)])
 @pytest.mark.parametrize("frac", [0.5, 1.0])
 @pytest.mark.parametrize("rm", [16, 8])
 def test_kinetic_matches_oracle(hip, orc, shape, frac, rm):
