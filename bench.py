@@ -215,8 +215,21 @@ def ground_state_run(ncases):
             "sample": "held-out (w, g) of seed 55; warm time includes the network and the interpolation"}
 
 
-def cpu_baseline(wl, steps):
-    """The oracle on the same 256^3 Bragg workload, `steps` steps."""
+def cpu_model():
+    """The host CPU's model name (Linux /proc/cpuinfo), or 'unknown'."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(wl, steps, reps=3):
+    """The oracle on the same 256^3 Bragg workload: `reps` blocks of `steps`
+    steps, the median block time."""
     from oracle import oracle as O
     g = O.make_grid(wl.shape, L=wl.L)
     psi1, psi2 = wl.initial_state()
@@ -224,13 +237,16 @@ def cpu_baseline(wl, steps):
                    beta0=0.0, deltaL=[0.0, 0.0], amp0=wl.amp0, laser_k=wl.laser_k,
                    laser_domh=wl.laser_domh)
     del psi1, psi2
-    t0 = time.perf_counter()
-    S.block(O.BRAGG_AD, steps)
-    el = time.perf_counter() - t0
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        S.block(O.BRAGG_AD, steps)
+        times.append(time.perf_counter() - t0)
+    el = sorted(times)[len(times) // 2]
     npts = float(math.prod(wl.shape))
     return {"value": npts * steps / el / 1e9, "unit": "Gpt-steps/s", "cores": O.threads(), "kind": "port",
-            "sample": f"{steps} Strang steps (one block) of the 256^3 two-component Bragg workload, "
-                      f"{el:.1f} s, radix-2 OpenMP FFT oracle"}
+            "sample": f"median of {reps} blocks of {steps} Strang steps of the 256^3 two-component Bragg workload "
+                      f"({', '.join(f'{x:.2f}' for x in times)} s), radix-2 OpenMP FFT oracle on {cpu_model()}"}
 
 
 def main():
