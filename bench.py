@@ -112,10 +112,10 @@ def gpu_run(args, wl):
     }
 
 
-def rt_solver_run(wl, steps, energy):
+def rt_solver_run(wl, steps, energy, tune=True):
     """One-component rt_solver freeprop (V + g|psi|^2, kinetic step) on the
-    GPU: `steps` steps in blocks of Nk after one untimed block, the norm
-    (and the energy) after every block."""
+    GPU: `steps` steps in blocks of Nk after the autotuner's blocks and one
+    untimed block, the norm (and the energy) after every block."""
     from atus2_amd import hip
     dx, dk = wl.spacings
     npts = float(math.prod(wl.shape))
@@ -128,7 +128,13 @@ def rt_solver_run(wl, steps, energy):
         if getattr(wl, "momentum", None):
             ctx.set_momentum(0, wl.momentum)
         N0 = ctx.particle_number(0)
-        t = ctx.run_block(hip.STEP_FREEPROP, wl.dt, wl.Nk, 0.0)
+        t, choice = 0.0, {}
+        nsteps = steps + wl.Nk
+        if tune:
+            from atus2_amd.autotune import autotune
+            t, choice = autotune(ctx, hip.STEP_FREEPROP, wl.dt, t, Nk=10, reps=2)
+            nsteps += 2 * 2 * 10 + (2 * (1 + 2) * 10 if len(wl.shape) == 3 else 0)
+        t = ctx.run_block(hip.STEP_FREEPROP, wl.dt, wl.Nk, t)
         ctx.synchronize()
         t0 = time.perf_counter()
         left = steps
@@ -141,15 +147,15 @@ def rt_solver_run(wl, steps, energy):
             left -= nk
         ctx.synchronize()
         elapsed = time.perf_counter() - t0
-        drift = abs(ctx.particle_number(0) - N0) / (steps + wl.Nk)
+        drift = abs(ctx.particle_number(0) - N0) / nsteps
         t, prof = ctx.run_block_profiled(hip.STEP_FREEPROP, wl.dt, wl.Nk, t)
     finally:
         ctx.close()
     kernels = {k: {"avg_us": 1e3 * v[0] / v[1], "launches": v[1]} for k, v in prof.items() if v[1] > 0}
-    return npts * steps / elapsed / 1e9, 1e3 * elapsed / steps, drift, kernels
+    return npts * steps / elapsed / 1e9, 1e3 * elapsed / steps, drift, kernels, choice
 
 
-def other_configs_run():
+def other_configs_run(tune=True):
     """The reference's 1D (4096, V = x^2/2, g = 10) and 2D (2048^2 double
     slit) rt_solver workloads, with the bytes per point and step of the
     reference's unfused split step (one nonlinear sweep and two sweeps per
@@ -171,7 +177,7 @@ def other_configs_run():
         d = len(wl.shape)
         ref_bytes = (1 + 2 * d) * 32 + 8
         own_bytes = 32.0 / wl.Nk + 8 if d == 1 else d * 32 + 8
-        value, ms, drift, kernels = rt_solver_run(wl, steps, energy)
+        value, ms, drift, kernels, choice = rt_solver_run(wl, steps, energy, tune)
         npts = float(math.prod(wl.shape))
         gbps = own_bytes * npts / (ms * 1e-3) / 1e9
         log(f"[bench] {name}: {value:.3f} Gpt-steps/s, {ms * 1e3:.1f} us per step")
@@ -179,7 +185,7 @@ def other_configs_run():
                     "unit": "Gpt-steps/s", "ms_per_step": ms, "norm_drift_per_step": drift,
                     "bytes_per_point_step": {"reference_model": ref_bytes, "this_design": own_bytes},
                     "achieved_GBps": gbps, "frac": gbps / HBM_PEAK_GBS, "bound": bound,
-                    "state_MiB": npts * 24 / 2 ** 20, "kernels": kernels})
+                    "state_MiB": npts * 24 / 2 ** 20, "kernels": kernels, "tuning": choice})
     return out
 
 
@@ -284,7 +290,7 @@ def main():
     ys = measured_traffic(wl.n, "pass_strided", r["tuning"].get("pass_strided", DEFAULT_RM["pass_strided"]))
     step_traffic = None if None in fam_traffic or ys is None else sum(fam_traffic) + 2 * ys
     gs = ground_state_run(args.gs_cases) if args.gs_cases > 0 else None
-    others = None if args.no_other_configs else other_configs_run()
+    others = None if args.no_other_configs else other_configs_run(not (args.no_autotune or args.tuning))
     cpu = None
     if not args.no_cpu_baseline:
         log(f"[bench] CPU baseline: oracle, {args.cpu_steps} steps")
