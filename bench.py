@@ -270,8 +270,9 @@ def main():
                     help="skip the 1D 4096 and 2D 2048^2 rt_solver workloads")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(sharding is described in DESIGN.md); rerun with --gpus 1", file=sys.stderr)
+        print("bench.py runs on one GPU only: the propagator has no multi-GPU path "
+              "(it would shard by x-slabs with one all-to-all per transform); rerun with --gpus 1",
+              file=sys.stderr)
         sys.exit(2)
     from atus2_amd.workloads import BraggWorkload
     wl = BraggWorkload(n=args.n)

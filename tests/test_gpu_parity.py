@@ -1,5 +1,5 @@
 """GPU parity: the HIP path (through the C-ABI) against the CPU oracle on the
-same seeded inputs.  Tolerances: the north-star gates are a relative L2 error
+same seeded inputs.  Tolerances: the parity gates are a relative L2 error
 <= 1e-10 in psi after a full run and <= 1e-12 norm drift per step; single
 operations are held to 1e-13 relative (a few ulps of accumulated FFT
 rounding)."""
@@ -196,7 +196,7 @@ def test_run_block_matches_oracle(hip, orc, shape, rm, kind):
                 # the same sensitivity as psi
                 assert abs(Ns[c] - Ns_ref[c]) < max(1e-13, tol), (c, Ns[c], Ns_ref[c])
             assert abs(sum(Ns) - sum(Ns_ref)) < max(1e-13, tol)
-            # the north-star gate on the device path itself: <= 1e-12 per step
+            # the norm gate on the device path itself: <= 1e-12 per step
             assert abs(sum(Ns) - N_start) <= 1e-12 * Nk * (b + 1), (b, sum(Ns) - N_start)
 
 
