@@ -771,8 +771,10 @@ __global__ __launch_bounds__(LB) void k_kin(double2* __restrict__ psi, long long
 // stays in the registers of its threads for all Nk steps): the half
 // kinetic step, then Nk times the V step followed by the full kinetic step
 // (the last one half), as the eager block launches them, with k_kin's
-// arithmetic (the same plan, V step and phase multiply), so the results
-// are bit-identical.  eh / ef: phase tables of the half and the full step.
+// operations (transform, V step, phase multiply).  The results agree with
+// a launch per step to rounding: the plan may differ (RM), and multiply-adds
+// are fused differently inside the step loop.  eh / ef: phase tables of the
+// half and the full step.
 // Step m reads its t1 from t1tab[m-1] and its amplitude from Ftab[m-1]
 // (single run) or Ftab[(m-1)*runs + run] (BATCH: pencil r is run r).
 template <int N, int RM, int VK, int NC, bool BATCH, int LB = 256>
