@@ -901,8 +901,9 @@ bool resident_ok(const atus_ctx* c) {
 
 // a 1D block in one launch: the t1 (and, for a single run, the amplitude)
 // of every step are evaluated with the eager block's bookkeeping and
-// uploaded first, unless a replayed graph supplies them; fbatch: the
-// batch's amplitude table [Nk][runs], or null for a single run
+// uploaded first, unless a replayed graph supplies them; batch: the
+// amplitudes come from fd.fbatch ([Nk][runs]), else from the callback or
+// fd.fdev
 int resident_block(atus_ctx* c, int vk, const VArgs& base, int Nk, double dt, double* t, const StepFeed& fd,
                    bool batch) {
   const int N0 = c->n[0];
