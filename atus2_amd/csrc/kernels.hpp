@@ -196,13 +196,20 @@ struct PencilMap {
   __device__ __forceinline__ long long elem(int j) const { return base + (long long)j * stride; }
 };
 
+// element t + P*i of the pencil: one 64-bit multiply for the first, then
+// pointer increments of P*stride (64-bit index products per element cost
+// several VALU instructions each)
 template <int N, int RM>
 __device__ __forceinline__ void load_pencil(double2 (&v)[(Plan<N, RM>::R)], const double2* src,
                                             long long base, long long stride, int t, bool valid) {
   constexpr int R = Plan<N, RM>::R, P = Plan<N, RM>::P;
+  const double2* p = src + base + (long long)t * stride;
+  const long long step = (long long)P * stride;
 #pragma unroll
-  for (int i = 0; i < R; ++i)
-    v[i] = valid ? src[base + (long long)(t + P * i) * stride] : make_double2(0.0, 0.0);
+  for (int i = 0; i < R; ++i) {
+    v[i] = valid ? p[0] : make_double2(0.0, 0.0);
+    p += step;
+  }
 }
 
 template <int N, int RM>
@@ -210,8 +217,13 @@ __device__ __forceinline__ void store_pencil(const double2 (&v)[(Plan<N, RM>::R)
                                              long long base, long long stride, int t, bool valid) {
   constexpr int R = Plan<N, RM>::R, P = Plan<N, RM>::P;
   if (!valid) return;
+  double2* p = dst + base + (long long)t * stride;
+  const long long step = (long long)P * stride;
 #pragma unroll
-  for (int i = 0; i < R; ++i) dst[base + (long long)(t + P * i) * stride] = v[i];
+  for (int i = 0; i < R; ++i) {
+    p[0] = v[i];
+    p += step;
+  }
 }
 
 // ------------------------------------------------------------ pointwise V
