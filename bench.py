@@ -189,6 +189,22 @@ def other_configs_run(tune=True):
     return out
 
 
+def scan_run(steps):
+    """The 512-run batched Bragg scan (seed 4: 384 1D and 128 2D runs) for
+    `steps` steps per run, batched and one run after another."""
+    import numpy as np
+    from atus2_amd import scan
+    runs = scan.scan_runs()
+    pts = sum(math.prod(r.shape) for r in runs) * steps
+    rb, tb = scan.run_batched(runs, steps=steps)
+    rs, ts = scan.run_single(runs, steps=steps)
+    diff = max(float(np.max(np.abs(rb[i] - rs[i]))) for i in range(len(runs)))
+    log(f"[bench] scan of {len(runs)} runs x {steps} steps: batched {tb:.2f} s, one after another {ts:.2f} s")
+    return {"runs": len(runs), "steps_per_run": steps, "batched_s": tb, "single_s": ts, "speedup": ts / tb,
+            "batched_Gpt_steps_per_s": pts / tb / 1e9, "max_rabi_difference": diff,
+            "sample": f"{steps} of the configuration's 3000 steps per run (python -m atus2_amd.scan runs all)"}
+
+
 def measured_traffic(n, family, values_per_thread):
     """HBM bytes per launch measured by counters for this family, or None
     when the table does not cover the configuration."""
@@ -268,6 +284,8 @@ def main():
     ap.add_argument("--gs-cases", type=int, default=4, help="ground states timed cold and warm (0: none)")
     ap.add_argument("--no-other-configs", action="store_true",
                     help="skip the 1D 4096 and 2D 2048^2 rt_solver workloads")
+    ap.add_argument("--scan-steps", type=int, default=300,
+                    help="steps per run of the 512-run scan, batched and one after another (0: none)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: the propagator has no multi-GPU path "
@@ -292,6 +310,7 @@ def main():
     step_traffic = None if None in fam_traffic or ys is None else sum(fam_traffic) + 2 * ys
     gs = ground_state_run(args.gs_cases) if args.gs_cases > 0 else None
     others = None if args.no_other_configs else other_configs_run(not (args.no_autotune or args.tuning))
+    scan = scan_run(args.scan_steps) if args.scan_steps > 0 else None
     cpu = None
     if not args.no_cpu_baseline:
         log(f"[bench] CPU baseline: oracle, {args.cpu_steps} steps")
@@ -333,6 +352,7 @@ def main():
         "cpu_baseline": cpu,
         "ground_state": gs,
         "other_configs": others,
+        "scan": scan,
         "kernels": r["kernels"],
         "tuning": r["tuning"],
         "checks": {"norm_drift_per_step": r["norm_drift_per_step"], "rabi_last_block": r["rabi"]},
