@@ -100,39 +100,6 @@ static double *make_twiddles(long long n) {
   return w;
 }
 
-/* in-place radix-2 DIT on a contiguous complex vector of power-of-two n */
-static void fft_pow2(double *x, long long n, int sign, const double *w) {
-  /* bit reversal */
-  for (long long i = 1, j = 0; i < n; i++) {
-    long long bit = n >> 1;
-    for (; j & bit; bit >>= 1) j ^= bit;
-    j ^= bit;
-    if (i < j) {
-      double tr = x[2 * i], ti = x[2 * i + 1];
-      x[2 * i] = x[2 * j];
-      x[2 * i + 1] = x[2 * j + 1];
-      x[2 * j] = tr;
-      x[2 * j + 1] = ti;
-    }
-  }
-  for (long long len = 2; len <= n; len <<= 1) {
-    long long half = len >> 1, step = n / len;
-    for (long long i = 0; i < n; i += len) {
-      for (long long j = 0; j < half; j++) {
-        double wr = w[2 * j * step];
-        double wi = (sign < 0) ? w[2 * j * step + 1] : -w[2 * j * step + 1];
-        double *a = x + 2 * (i + j), *b = x + 2 * (i + j + half);
-        double br = b[0] * wr - b[1] * wi;
-        double bi = b[0] * wi + b[1] * wr;
-        b[0] = a[0] - br;
-        b[1] = a[1] - bi;
-        a[0] += br;
-        a[1] += bi;
-      }
-    }
-  }
-}
-
 /* direct DFT for lengths that are not powers of two (small n only) */
 static void dft_direct(double *x, long long n, int sign, double *tmp) {
   for (long long k = 0; k < n; k++) {
@@ -151,74 +118,244 @@ static void dft_direct(double *x, long long n, int sign, double *tmp) {
   memcpy(x, tmp, sizeof(double) * 2 * n);
 }
 
-static void fft_line(double *x, long long n, int sign, const double *w,
-                     double *tmp) {
-  if (n == 1) return;
-  if (is_pow2(n))
-    fft_pow2(x, n, sign, w);
-  else
-    dft_direct(x, n, sign, tmp);
-}
-
 #define ORC_BLK 16
 
-/* transform along axis `a` of the [n0][n1][n2] array; pencils of the
- * strided axes are gathered ORC_BLK at a time so every read touches whole
- * cache lines */
-static void fft_axis(const orc_grid *g, double *data, int a, int sign) {
+/* the radix-2 DIT (bit reversal, then butterflies of growing length with
+ * w[j * n/len]) applied to up to ORC_BLK pencils at once, held split as
+ * re[i * ORC_BLK + c], im[i * ORC_BLK + c] (element i of pencil c); the
+ * loops over c vectorise */
+static void fft_pow2_blk(double *re, double *im, long long n, int sign,
+                         const double *w, int cw) {
+  for (long long i = 1, j = 0; i < n; i++) {
+    long long bit = n >> 1;
+    for (; j & bit; bit >>= 1) j ^= bit;
+    j ^= bit;
+    if (i < j)
+      for (int c = 0; c < cw; c++) {
+        double tr = re[i * ORC_BLK + c], ti = im[i * ORC_BLK + c];
+        re[i * ORC_BLK + c] = re[j * ORC_BLK + c];
+        im[i * ORC_BLK + c] = im[j * ORC_BLK + c];
+        re[j * ORC_BLK + c] = tr;
+        im[j * ORC_BLK + c] = ti;
+      }
+  }
+  for (long long len = 2; len <= n; len <<= 1) {
+    long long half = len >> 1, step = n / len;
+    for (long long i = 0; i < n; i += len) {
+      for (long long j = 0; j < half; j++) {
+        const double wr = w[2 * j * step];
+        const double wi = (sign < 0) ? w[2 * j * step + 1] : -w[2 * j * step + 1];
+        double *ar = re + (i + j) * ORC_BLK, *ai = im + (i + j) * ORC_BLK;
+        double *br = re + (i + j + half) * ORC_BLK, *bi = im + (i + j + half) * ORC_BLK;
+        for (int c = 0; c < ORC_BLK; c++) {
+          const double tr = br[c] * wr - bi[c] * wi;
+          const double ti = br[c] * wi + bi[c] * wr;
+          br[c] = ar[c] - tr;
+          bi[c] = ai[c] - ti;
+          ar[c] += tr;
+          ai[c] += ti;
+        }
+      }
+    }
+  }
+}
+
+/* what the last pass does to each element at its store (the cft scale,
+ * then optionally a pointwise complex or real multiplier) */
+typedef struct {
+  int on;
+  double fak;
+  const double *ctab; /* complex multiplier, interleaved, or NULL */
+  const double *rmul; /* real multiplier, or NULL */
+} orc_post;
+
+/* pencils p0 .. p0+cw-1 of a power-of-two axis of length n whose elements
+ * lie `inner` apart (pencil p starts at (p / inner) * n * inner + p % inner
+ * of `data`); `post` applies at the store, indexed by the element's offset
+ * from `whole` (the start of the full array) */
+static void axis_block(double *data, const double *whole, long long n,
+                       long long inner, long long p0, int cw, int sign,
+                       const double *w, double *re, double *im,
+                       const orc_post *post) {
+  long long start[ORC_BLK];
+  for (int c = 0; c < cw; c++) {
+    const long long p = p0 + c;
+    start[c] = (p / inner) * n * inner + p % inner;
+  }
+  for (long long i = 0; i < n; i++) {
+    for (int c = 0; c < cw; c++) {
+      const double *s = data + 2 * (start[c] + i * inner);
+      re[i * ORC_BLK + c] = s[0];
+      im[i * ORC_BLK + c] = s[1];
+    }
+    for (int c = cw; c < ORC_BLK; c++) {
+      re[i * ORC_BLK + c] = 0.0;
+      im[i * ORC_BLK + c] = 0.0;
+    }
+  }
+  fft_pow2_blk(re, im, n, sign, w, cw);
+  for (long long i = 0; i < n; i++)
+    for (int c = 0; c < cw; c++) {
+      double *d = data + 2 * (start[c] + i * inner);
+      double vr = re[i * ORC_BLK + c], vi = im[i * ORC_BLK + c];
+      if (post && post->on) {
+        /* scale_all, then the step's multiply, in that order */
+        vr *= post->fak;
+        vi *= post->fak;
+        const long long l = (d - whole) / 2;
+        if (post->ctab) {
+          const double tr = post->ctab[2 * l], ti = post->ctab[2 * l + 1];
+          const double xr = vr;
+          vr = xr * tr - vi * ti;
+          vi = vi * tr + xr * ti;
+        } else if (post->rmul) {
+          vr *= post->rmul[l];
+          vi *= post->rmul[l];
+        }
+      }
+      d[0] = vr;
+      d[1] = vi;
+    }
+}
+
+/* direct transform along axis `a` for a length that is not a power of two
+ * (small test grids only), pencil by pencil */
+static void dft_axis(const orc_grid *g, double *data, int a, int sign) {
   long long n = g->n[a];
-  if (n == 1) return;
   long long inner = 1, outer = 1;
   for (int b = a + 1; b < 3; b++) inner *= g->n[b];
   for (int b = 0; b < a; b++) outer *= g->n[b];
-  double *w = make_twiddles(n);
-  if (inner == 1) {
 #pragma omp parallel
-    {
-      double *tmp = (double *)malloc(sizeof(double) * 2 * n);
+  {
+    double *line = (double *)malloc(sizeof(double) * 2 * n);
+    double *tmp = (double *)malloc(sizeof(double) * 2 * n);
 #pragma omp for schedule(static)
-      for (long long o = 0; o < outer; o++)
-        fft_line(data + 2 * o * n, n, sign, w, tmp);
-      free(tmp);
-    }
-  } else {
-    long long nblk = (inner + ORC_BLK - 1) / ORC_BLK;
-#pragma omp parallel
-    {
-      double *buf = (double *)malloc(sizeof(double) * 2 * n * ORC_BLK);
-      double *tmp = (double *)malloc(sizeof(double) * 2 * n);
-#pragma omp for schedule(static)
-      for (long long ob = 0; ob < outer * nblk; ob++) {
-        long long o = ob / nblk, c0 = (ob % nblk) * ORC_BLK;
-        long long cw = inner - c0 < ORC_BLK ? inner - c0 : ORC_BLK;
-        double *base = data + 2 * (o * n * inner + c0);
-        for (long long i = 0; i < n; i++)
-          for (long long c = 0; c < cw; c++) {
-            buf[2 * (c * n + i)] = base[2 * (i * inner + c)];
-            buf[2 * (c * n + i) + 1] = base[2 * (i * inner + c) + 1];
-          }
-        for (long long c = 0; c < cw; c++)
-          fft_line(buf + 2 * c * n, n, sign, w, tmp);
-        for (long long i = 0; i < n; i++)
-          for (long long c = 0; c < cw; c++) {
-            base[2 * (i * inner + c)] = buf[2 * (c * n + i)];
-            base[2 * (i * inner + c) + 1] = buf[2 * (c * n + i) + 1];
-          }
+    for (long long p = 0; p < outer * inner; p++) {
+      double *base = data + 2 * ((p / inner) * n * inner + p % inner);
+      for (long long i = 0; i < n; i++) {
+        line[2 * i] = base[2 * i * inner];
+        line[2 * i + 1] = base[2 * i * inner + 1];
       }
-      free(buf);
-      free(tmp);
+      dft_direct(line, n, sign, tmp);
+      for (long long i = 0; i < n; i++) {
+        base[2 * i * inner] = line[2 * i];
+        base[2 * i * inner + 1] = line[2 * i + 1];
+      }
     }
+    free(line);
+    free(tmp);
   }
-  free(w);
 }
 
-void orc_fft(const orc_grid *g, double *data, int sign) {
-  for (int a = 0; a < 3; a++) fft_axis(g, data, a, sign);
+/* every pencil of axis `a` over the whole array (ORC_BLK adjacent columns
+ * at a time for a strided axis, so every read touches whole cache lines) */
+static void pow2_axis(const orc_grid *g, double *data, int a, int sign,
+                      const double *w, const orc_post *post) {
+  long long n = g->n[a];
+  long long inner = 1, outer = 1;
+  for (int b = a + 1; b < 3; b++) inner *= g->n[b];
+  for (int b = 0; b < a; b++) outer *= g->n[b];
+  const long long npen = outer * inner;
+  const long long nblk = (npen + ORC_BLK - 1) / ORC_BLK;
+#pragma omp parallel
+  {
+    double *re = (double *)malloc(sizeof(double) * n * ORC_BLK);
+    double *im = (double *)malloc(sizeof(double) * n * ORC_BLK);
+#pragma omp for schedule(static)
+    for (long long b = 0; b < nblk; b++) {
+      const long long p0 = b * ORC_BLK;
+      const int cw = (int)(npen - p0 < ORC_BLK ? npen - p0 : ORC_BLK);
+      axis_block(data, data, n, inner, p0, cw, sign, w, re, im, post);
+    }
+    free(re);
+    free(im);
+  }
 }
 
 static void scale_all(double *data, long long npts, double fak) {
 #pragma omp parallel for schedule(static)
   for (long long i = 0; i < 2 * npts; i++) data[i] *= fak;
+}
+
+static void apply_post(double *data, long long np, const orc_post *post) {
+  scale_all(data, np, post->fak);
+  if (post->ctab) {
+#pragma omp parallel for schedule(static)
+    for (long long l = 0; l < np; l++) {
+      double re = data[2 * l], im = data[2 * l + 1];
+      data[2 * l] = re * post->ctab[2 * l] - im * post->ctab[2 * l + 1];
+      data[2 * l + 1] = im * post->ctab[2 * l] + re * post->ctab[2 * l + 1];
+    }
+  } else if (post->rmul) {
+#pragma omp parallel for schedule(static)
+    for (long long l = 0; l < np; l++) {
+      data[2 * l] *= post->rmul[l];
+      data[2 * l + 1] *= post->rmul[l];
+    }
+  }
+}
+
+/* the unscaled transform along axes 0, 1, 2 in that order, then `post`.
+ * With power-of-two axes the last axis applies `post` at its store, and
+ * on 3D grids axes 1 and 2 run plane by plane while the plane is in cache;
+ * each element sees the same operations in the same order either way */
+static void fft_post(const orc_grid *g, double *data, int sign, const orc_post *post) {
+  int last = -1, all_pow2 = 1;
+  for (int a = 0; a < 3; a++) {
+    if (g->n[a] == 1) continue;
+    last = a;
+    if (!is_pow2(g->n[a])) all_pow2 = 0;
+  }
+  if (last < 0 || !all_pow2) {
+    for (int a = 0; a < 3; a++)
+      if (g->n[a] > 1) {
+        if (is_pow2(g->n[a])) {
+          double *w = make_twiddles(g->n[a]);
+          pow2_axis(g, data, a, sign, w, NULL);
+          free(w);
+        } else {
+          dft_axis(g, data, a, sign);
+        }
+      }
+    if (post && post->on) apply_post(data, orc_npts(g), post);
+    return;
+  }
+  double *w[3] = {NULL, NULL, NULL};
+  for (int a = 0; a < 3; a++)
+    if (g->n[a] > 1) w[a] = make_twiddles(g->n[a]);
+  const long long n0 = g->n[0], n1 = g->n[1], n2 = g->n[2];
+  if (n0 > 1 && n1 > 1 && n2 > 1) {
+    pow2_axis(g, data, 0, sign, w[0], NULL);
+    const long long plane = n1 * n2;
+#pragma omp parallel
+    {
+      const long long nm = n1 > n2 ? n1 : n2;
+      double *re = (double *)malloc(sizeof(double) * nm * ORC_BLK);
+      double *im = (double *)malloc(sizeof(double) * nm * ORC_BLK);
+#pragma omp for schedule(static)
+      for (long long i0 = 0; i0 < n0; i0++) {
+        double *pl = data + 2 * i0 * plane;
+        for (long long p0 = 0; p0 < n2; p0 += ORC_BLK) {
+          const int cw = (int)(n2 - p0 < ORC_BLK ? n2 - p0 : ORC_BLK);
+          axis_block(pl, data, n1, n2, p0, cw, sign, w[1], re, im, NULL);
+        }
+        for (long long p0 = 0; p0 < n1; p0 += ORC_BLK) {
+          const int cw = (int)(n1 - p0 < ORC_BLK ? n1 - p0 : ORC_BLK);
+          axis_block(pl, data, n2, 1, p0, cw, sign, w[2], re, im, post);
+        }
+      }
+      free(re);
+      free(im);
+    }
+  } else {
+    for (int a = 0; a < 3; a++)
+      if (g->n[a] > 1) pow2_axis(g, data, a, sign, w[a], a == last ? post : NULL);
+  }
+  for (int a = 0; a < 3; a++) free(w[a]);
+}
+
+void orc_fft(const orc_grid *g, double *data, int sign) {
+  fft_post(g, data, sign, NULL);
 }
 
 /* the scale() factor of cft_Nd (cft_1d.cpp:193, cft_2d.cpp:451,
@@ -231,11 +368,17 @@ static double ft_factor(const orc_grid *g, const double *s) {
   }
 }
 
-/* cft_Nd::ft (cft_3d.cpp:58-74, m_bfix == false path) */
-void orc_ft(const orc_grid *g, double *data, int isign) {
+/* cft_Nd::ft (cft_3d.cpp:58-74, m_bfix == false path): FFTW's unscaled
+ * transform, then scale(); ctab / rmul: the pointwise multiply a caller
+ * applies right after the transform (NULL: none) */
+static void ft_mul(const orc_grid *g, double *data, int isign, const double *ctab, const double *rmul) {
   if (isign != 1 && isign != -1) return;
-  orc_fft(g, data, isign);
-  scale_all(data, orc_npts(g), ft_factor(g, isign == -1 ? g->dx : g->dk));
+  orc_post post = {1, ft_factor(g, isign == -1 ? g->dx : g->dk), ctab, rmul};
+  fft_post(g, data, isign, &post);
+}
+
+void orc_ft(const orc_grid *g, double *data, int isign) {
+  ft_mul(g, data, isign, NULL, NULL);
 }
 
 /* ------------------------------------------------------------------ */
@@ -264,18 +407,12 @@ void orc_kinetic_table(const orc_grid *g, const double alpha[3], double dt,
 /* CRT_Base::Do_FT_Step_full / _half (CRT_Base.h:337-392) */
 void orc_ft_step(const orc_grid *g, int ncomp, double **psi,
                  const double *tab) {
-  long long np = orc_npts(g);
-  for (int c = 0; c < ncomp; c++) orc_ft(g, psi[c], -1);
+  /* ft(-1), psi *= tab, ft(+1) per component (the components are
+   * independent, so each runs through all three before the next) */
   for (int c = 0; c < ncomp; c++) {
-    double *p = psi[c];
-#pragma omp parallel for schedule(static)
-    for (long long l = 0; l < np; l++) {
-      double re = p[2 * l], im = p[2 * l + 1];
-      p[2 * l] = re * tab[2 * l] - im * tab[2 * l + 1];
-      p[2 * l + 1] = im * tab[2 * l] + re * tab[2 * l + 1];
-    }
+    ft_mul(g, psi[c], -1, tab, NULL);
+    orc_ft(g, psi[c], 1);
   }
-  for (int c = 0; c < ncomp; c++) orc_ft(g, psi[c], 1);
 }
 
 /* CRT_Base::Do_NL_Step (CRT_Base.h:397-441): all phases from the
@@ -552,19 +689,10 @@ void orc_rt_set_dt(orc_rt *s, double dt) {
 }
 
 void orc_ft_step2(const orc_grid *g, int ncomp, double **psi, const double *tab1, const double *tab2) {
-  long long np = orc_npts(g);
-  for (int c = 0; c < ncomp; c++) orc_ft(g, psi[c], -1);
   for (int c = 0; c < ncomp; c++) {
-    double *p = psi[c];
-    const double *tab = c < ncomp / 2 ? tab1 : tab2;
-#pragma omp parallel for schedule(static)
-    for (long long l = 0; l < np; l++) {
-      double re = p[2 * l], im = p[2 * l + 1];
-      p[2 * l] = re * tab[2 * l] - im * tab[2 * l + 1];
-      p[2 * l + 1] = im * tab[2 * l] + re * tab[2 * l + 1];
-    }
+    ft_mul(g, psi[c], -1, c < ncomp / 2 ? tab1 : tab2, NULL);
+    orc_ft(g, psi[c], 1);
   }
-  for (int c = 0; c < ncomp; c++) orc_ft(g, psi[c], 1);
 }
 
 void orc_nl_if2_step(const orc_grid *g, int ncomp, double **psi, const double *gs, double dt, double beta,
@@ -1061,12 +1189,7 @@ static void sob_filter(const orc_sob *s, const double *in, const double *mult,
                        double *out) {
   long long np = orc_npts(&s->g);
   if (out != in) memcpy(out, in, sizeof(double) * 2 * np);
-  orc_ft(&s->g, out, -1);
-#pragma omp parallel for schedule(static)
-  for (long long l = 0; l < np; l++) {
-    out[2 * l] *= mult[l];
-    out[2 * l + 1] *= mult[l];
-  }
+  ft_mul(&s->g, out, -1, NULL, mult);
   orc_ft(&s->g, out, 1);
 }
 
