@@ -47,8 +47,6 @@ unsigned ew_blocks(long long n) {
 
 }  // namespace
 
-
-
 namespace {
 
 int check_ctx(atus_ctx* c) {
@@ -355,15 +353,38 @@ int reduce(atus_ctx* c, const Fn& fn, long long n, double* out) {
   return ATUS_OK;
 }
 
+// workgroups per segment of a segmented reduction: about 2048 over all
+// segments, at least one per segment
+unsigned seg_blocks(long long n, int nseg) {
+  unsigned nb = red_blocks(n);
+  const unsigned want = (unsigned)std::max(1, 2048 / nseg);
+  return nb > want ? want : nb;
+}
+
+// the launches of reduce_seg with caller-given device buffers: `part`
+// holds nseg * nb * NV partials, `res` nseg * NV results; out (host)
+// receives res
+template <class Fn>
+int reduce_seg_into(atus_ctx* c, const Fn& fn, long long n, long long seg_stride, int nseg, unsigned nb, double* part,
+                    double* res, double* out) {
+  constexpr int NV = Fn::NV;
+  hipLaunchKernelGGL((k_reduce_seg<Fn>), dim3(nb, (unsigned)nseg), dim3(RED_THREADS), 0, c->stream, fn, n, seg_stride,
+                     part);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL((k_reduce_seg_final<NV>), dim3((unsigned)nseg), dim3(RED_THREADS), 0, c->stream, part, (int)nb,
+                     res);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(out, res, sizeof(double) * (size_t)nseg * NV, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return ATUS_OK;
+}
+
 // nseg reductions of n elements each, segment y starting y * seg_stride
 // elements after the functor's data: out[y * NV + v]
 template <class Fn>
 int reduce_seg(atus_ctx* c, const Fn& fn, long long n, long long seg_stride, int nseg, double* out) {
   constexpr int NV = Fn::NV;
-  // about 2048 workgroups over all segments, at least one per segment
-  unsigned nb = red_blocks(n);
-  const unsigned want = (unsigned)std::max(1, 2048 / nseg);
-  if (nb > want) nb = want;
+  const unsigned nb = seg_blocks(n, nseg);
   const size_t need = (size_t)nseg * nb * NV;
   if (c->sred_cap < need) {
     (void)hipFree(c->sred_part);
@@ -375,15 +396,7 @@ int reduce_seg(atus_ctx* c, const Fn& fn, long long n, long long seg_stride, int
       return fail(ATUS_ERR_NOMEM, "segmented reduction buffers");
     c->sred_cap = need;
   }
-  hipLaunchKernelGGL((k_reduce_seg<Fn>), dim3(nb, (unsigned)nseg), dim3(RED_THREADS), 0, c->stream, fn, n, seg_stride,
-                     c->sred_part);
-  HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL((k_reduce_seg_final<NV>), dim3((unsigned)nseg), dim3(RED_THREADS), 0, c->stream, c->sred_part,
-                     (int)nb, c->sred_out);
-  HIP_TRY(hipGetLastError());
-  HIP_TRY(hipMemcpyAsync(out, c->sred_out, sizeof(double) * (size_t)nseg * NV, hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(hipStreamSynchronize(c->stream));
-  return ATUS_OK;
+  return reduce_seg_into(c, fn, n, seg_stride, nseg, nb, c->sred_part, c->sred_out, out);
 }
 
 int vkind_of(int kind) {
@@ -542,6 +555,9 @@ int atus_ctx_create(atus_ctx** out, int dim, const long long n[3], const double 
 
 int atus_ctx_destroy(atus_ctx* c) {
   if (!c) return ATUS_OK;
+  if (c->live_graphs > 0)
+    return fail(ATUS_ERR_STATE, "context still has " + std::to_string(c->live_graphs) +
+                                    " captured graph(s); destroy them first (atus_graph_destroy)");
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   (void)hipFree(c->psi);
@@ -556,6 +572,9 @@ int atus_ctx_destroy(atus_ctx* c) {
   (void)hipFree(c->sob);
   (void)hipFree(c->lap);
   (void)hipFree(c->grad);
+  (void)hipFree(c->sob_st);
+  (void)hipFree(c->sob_part);
+  (void)hipFree(c->sob_log);
   (void)hipFree(c->runs);
   (void)hipFree(c->ftab);
   (void)hipFree(c->kinz);
@@ -633,6 +652,9 @@ int atus_copy_components(atus_ctx* dst, int dst_comp, atus_ctx* src, int src_com
   if (dst->npts != src->npts) return fail(ATUS_ERR_ARG, "contexts of different grid sizes");
   if (n < 1 || src_comp < 0 || dst_comp < 0 || src_comp + n > src->ncomp || dst_comp + n > dst->ncomp)
     return fail(ATUS_ERR_ARG, "component range out of bounds");
+  if (dst == src && dst_comp != src_comp && dst_comp < src_comp + n && src_comp < dst_comp + n)
+    return fail(ATUS_ERR_ARG, "overlapping component ranges within one context");
+  if (dst == src && dst_comp == src_comp) return ATUS_OK;
   // the source's pending work first; the copy runs on the destination's stream
   HIP_TRY(hipStreamSynchronize(src->stream));
   HIP_TRY(hipMemcpyAsync(dst->psi + (size_t)dst_comp * dst->npts, src->psi + (size_t)src_comp * src->npts,
@@ -644,6 +666,9 @@ int atus_copy_components(atus_ctx* dst, int dst_comp, atus_ctx* src, int src_com
 int atus_set_potential(atus_ctx* c, int comp, const double* host) {
   ATUS_TRY(check_ctx(c));
   ATUS_TRY(check_single(c, "atus_set_potential"));
+  if ((!host || !c->pot) && c->live_graphs > 0)
+    return fail(ATUS_ERR_STATE, "the potential buffer cannot be freed or allocated while captured graphs of this "
+                                "context exist (their V steps read it); destroy them first");
   if (!host) {
     HIP_TRY(hipStreamSynchronize(c->stream));
     (void)hipFree(c->pot);
@@ -683,6 +708,8 @@ int atus_set_alpha2(atus_ctx* c, const double alpha2[3]) {
   if (!alpha2) return fail(ATUS_ERR_ARG, "null alpha2");
   if (c->ncomp < 2 || c->ncomp % 2) return fail(ATUS_ERR_STATE, "two species need an even number of components");
   if (c->fs) return fail(ATUS_ERR_UNSUPPORTED, "two species on 1D grids longer than 4096 are not covered");
+  if (!c->alpha2_set && c->live_graphs > 0)
+    return fail(ATUS_ERR_STATE, "a second species cannot be added while captured graphs of this context exist");
   for (int a = 0; a < 3; ++a) c->alpha2[a] = a < c->dim ? alpha2[a] : 0.0;
   c->alpha2_set = true;
   c->kin_valid = false;
@@ -1385,6 +1412,75 @@ int sob_filter(atus_ctx* c, int comp, const double2* src, double2* out_sob, doub
   return ATUS_OK;
 }
 
+// the launches of one iteration up to the update: psi_sob, the Laplacian,
+// the Sobolev gradient and the projection products of every wave function
+// (Compute_Sobolev_Psi, Compute_Laplace, Compute_Sobolev_Gradient and the
+// dot products of Project_Sobolev_Gradient, CSOB_Base.h:251-367).  On 2D
+// and 3D grids the L2 gradient is formed inside the last-axis passes
+// (k_sob_gradfuse): the Laplacian's last inverse pass and the gradient
+// filter's first forward pass.
+int sob_body(atus_ctx* c) {
+  const int nw = c->ncomp, dim = c->dim;
+  const long long np = c->npts;
+  SobState* st = (SobState*)c->sob_st;
+  if (dim == 1) {
+    for (int i = 0; i < nw; ++i) ATUS_TRY(sob_filter(c, i, c->psi, c->sob, c->lap));
+    for (int i = 0; i < nw; ++i) {
+      hipLaunchKernelGGL(k_sob_grad_in, dim3(ew_blocks(np)), dim3(256), 0, c->stream, c->psi, c->lap, c->pot, c->grad,
+                         np, nw, i, c->gs[0 + i * nw], nw > 1 ? c->gs[1 + i * nw] : 0.0);
+      HIP_TRY(hipGetLastError());
+      ATUS_TRY(sob_filter(c, i, c->grad, c->grad, nullptr));
+    }
+  } else {
+    const int last = dim - 1, NL = c->n[last], N0 = c->n[0];
+    SobArgs sa{};
+    sa.dim = dim;
+    for (int a = 0; a < 3; ++a) {
+      sa.dk[a] = c->dk[a];
+      sa.n[a] = c->n[a];
+    }
+    sa.scale = ft_factor(dim, c->dx) * ft_factor(dim, c->dk);
+    ATUS_TRY(ensure_twiddles(c, N0));
+    for (int i = 0; i < nw; ++i) {
+      // psi_sob complete; the Laplacian short of its last-axis inverse pass
+      const long long off = i * np;
+      sa.alpha = c->sob_alpha[i];
+      const double2* s = c->psi + off;
+      double2* w = c->sob + off;
+      for (int a = last; a >= 1; --a) {
+        ATUS_TRY(pass(c, a, -1, s, w, 1, 1.0));
+        s = w;
+      }
+      HIP_TRY(launch_sobf(N0, c->rm[ATUS_KFAM_KIN], false, s, w, c->lap + off, np, 1, c->inner_of(0), c->tw[N0], sa,
+                          c->max_lds, c->stream));
+      for (int a = 1; a <= last; ++a) ATUS_TRY(pass(c, a, +1, w, w, 1, 1.0));
+      for (int a = 1; a < last; ++a) ATUS_TRY(pass(c, a, +1, c->lap + off, c->lap + off, 1, 1.0));
+    }
+    for (int i = 0; i < nw; ++i) {
+      const long long off = i * np;
+      sa.alpha = c->sob_alpha[i];
+      double2* w = c->grad + off;
+      HIP_TRY(launch_sob_gradfuse(NL, c->rm[ATUS_KFAM_VFUSE], nw, c->lap, c->psi, c->pot, c->grad, np, i,
+                                  c->gs[0 + i * nw], nw > 1 ? c->gs[1 + i * nw] : 0.0, c->tw[NL], c->max_lds,
+                                  c->stream));
+      for (int a = last - 1; a >= 1; --a) ATUS_TRY(pass(c, a, -1, w, w, 1, 1.0));
+      HIP_TRY(launch_sobf(N0, c->rm[ATUS_KFAM_KIN], false, w, w, nullptr, np, 1, c->inner_of(0), c->tw[N0], sa,
+                          c->max_lds, c->stream));
+      for (int a = 1; a <= last; ++a) ATUS_TRY(pass(c, a, +1, w, w, 1, 1.0));
+    }
+  }
+  for (int i = 0; i < nw; ++i) {
+    RedProj fp{c->psi + i * np, c->grad + i * np, c->sob + i * np};
+    const unsigned nb = red_blocks(np);
+    hipLaunchKernelGGL((k_reduce<RedProj>), dim3(nb), dim3(RED_THREADS), 0, c->stream, fp, np, c->red_part);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL((k_reduce_final<2>), dim3(1), dim3(RED_THREADS), 0, c->stream, c->red_part, (int)nb,
+                       &st->proj[i][0]);
+    HIP_TRY(hipGetLastError());
+  }
+  return ATUS_OK;
+}
+
 struct RedMu {
   static constexpr int NV = 1;
   const double2* psi;
@@ -1419,6 +1515,10 @@ int atus_sob_setup(atus_ctx* c, const double* alpha_iso, double stepsize) {
   if (!c->sob && hipMalloc(&c->sob, bytes) != hipSuccess) return fail(ATUS_ERR_NOMEM, "Sobolev buffers");
   if (!c->lap && hipMalloc(&c->lap, bytes) != hipSuccess) return fail(ATUS_ERR_NOMEM, "Sobolev buffers");
   if (!c->grad && hipMalloc(&c->grad, bytes) != hipSuccess) return fail(ATUS_ERR_NOMEM, "Sobolev buffers");
+  if ((!c->sob_st && hipMalloc(&c->sob_st, sizeof(SobState)) != hipSuccess) ||
+      (!c->sob_part && hipMalloc(&c->sob_part, sizeof(double) * 2 * RED_MAX_BLOCKS * 2) != hipSuccess) ||
+      (!c->sob_log && hipMalloc(&c->sob_log, sizeof(double) * SOB_MAX_CHUNK * SOB_LOG_VALS) != hipSuccess))
+    return fail(ATUS_ERR_NOMEM, "Sobolev loop state");
   c->sob_ready = true;
   return ATUS_OK;
 }
@@ -1437,57 +1537,64 @@ int atus_sob_renormalize(atus_ctx* c) {
   return ATUS_OK;
 }
 
-int atus_sob_iter(atus_ctx* c, double* res_tot, int* renormalised, double* N_out) {
+int atus_sob_iterate(atus_ctx* c, int count, double eps, int* performed, double* res_tot, double* log) {
   ATUS_TRY(check_ctx(c));
-  ATUS_TRY(check_single(c, "atus_sob_iter"));
+  ATUS_TRY(check_single(c, "atus_sob_iterate"));
   if (!c->sob_ready) return fail(ATUS_ERR_STATE, "call atus_sob_setup first");
+  if (count < 1 || count > SOB_MAX_CHUNK)
+    return fail(ATUS_ERR_ARG, "count must be 1.." + std::to_string(SOB_MAX_CHUNK));
   const int nw = c->ncomp;
   const long long np = c->npts;
-  // Compute_Sobolev_Psi and Compute_Laplace share the forward transform
-  for (int i = 0; i < nw; ++i) ATUS_TRY(sob_filter(c, i, c->psi, c->sob, c->lap));
-  // Compute_Sobolev_Gradient
-  for (int i = 0; i < nw; ++i) {
-    hipLaunchKernelGGL(k_sob_grad_in, dim3(ew_blocks(np)), dim3(256), 0, c->stream, c->psi, c->lap, c->pot, c->grad,
-                       np, nw, i, c->gs[0 + i * nw], nw > 1 ? c->gs[1 + i * nw] : 0.0);
+  const unsigned nb = red_blocks(np);
+  SobState* st = (SobState*)c->sob_st;
+  auto finalize = [&](int init, int slot) -> int {
+    hipLaunchKernelGGL(k_sob_finalize, dim3(1), dim3(RED_THREADS), 0, c->stream, st, c->sob_part, (int)nb, nw, c->ar(),
+                       eps, init, c->sob_log, slot);
     HIP_TRY(hipGetLastError());
-    ATUS_TRY(sob_filter(c, i, c->grad, c->grad, nullptr));
+    return ATUS_OK;
+  };
+  // N of the current psi, the renormalisation of the first iteration
+  for (int i = 0; i < nw; ++i) {
+    hipLaunchKernelGGL(k_sob_update_dev, dim3(nb), dim3(RED_THREADS), 0, c->stream, c->psi + i * np,
+                       (const double2*)nullptr, (const double2*)nullptr, np, 0.0, (const SobState*)st, i, 1,
+                       c->sob_part + (size_t)i * nb * 2);
+    HIP_TRY(hipGetLastError());
   }
-  // Project_Sobolev_Gradient + Compute_res + particle numbers
-  double res_sq = 0;
-  std::vector<double> N(nw);
-  for (int i = 0; i < nw; ++i) {
-    RedProj fp{c->psi + i * np, c->grad + i * np, c->sob + i * np};
-    double pr[2];
-    ATUS_TRY(reduce(c, fp, np, pr));
-    const double fak = pr[0] / pr[1];
-    const unsigned nb = red_blocks(np);
-    hipLaunchKernelGGL(k_sob_project_res, dim3(nb), dim3(RED_THREADS), 0, c->stream, c->grad + i * np,
-                       c->sob + i * np, c->psi + i * np, np, fak, c->red_part);
-    HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL((k_reduce_final<2>), dim3(1), dim3(RED_THREADS), 0, c->stream, c->red_part, (int)nb, c->red_out);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(c->red_host, c->red_out, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    const double res_i = c->red_host[0] * c->ar();
-    N[i] = c->ar() * c->red_host[1];
-    res_sq += res_i * res_i;
+  ATUS_TRY(finalize(1, 0));
+  for (int k = 0; k < count; ++k) {
+    ATUS_TRY(sob_body(c));
+    for (int i = 0; i < nw; ++i) {
+      hipLaunchKernelGGL(k_sob_update_dev, dim3(nb), dim3(RED_THREADS), 0, c->stream, c->psi + i * np,
+                         (const double2*)c->grad + i * np, (const double2*)c->sob + i * np, np, c->sob_step,
+                         (const SobState*)st, i, 0, c->sob_part + (size_t)i * nb * 2);
+      HIP_TRY(hipGetLastError());
+    }
+    ATUS_TRY(finalize(0, k));
   }
-  bool brenorm = false;
-  for (int i = 0; i < nw; ++i)
-    if (fabs(N[i] - 1) > 1e-5) brenorm = true;
-  for (int i = 0; i < nw; ++i) {
-    const double f = brenorm ? 1.0 / sqrt(N[i]) : 1.0;
-    hipLaunchKernelGGL(k_sob_update, dim3(ew_blocks(np)), dim3(256), 0, c->stream, c->psi + i * np, c->grad + i * np,
-                       np, f, c->sob_step);
-    HIP_TRY(hipGetLastError());
+  SobState h;
+  HIP_TRY(hipMemcpyAsync(&h, st, sizeof(SobState), hipMemcpyDeviceToHost, c->stream));
+  std::vector<double> lg;
+  if (log) {
+    lg.resize((size_t)count * SOB_LOG_VALS);
+    HIP_TRY(hipMemcpyAsync(lg.data(), c->sob_log, sizeof(double) * lg.size(), hipMemcpyDeviceToHost, c->stream));
   }
   HIP_TRY(hipStreamSynchronize(c->stream));
-  const double rt = sqrt(res_sq);
-  if (res_tot) *res_tot = rt;
-  if (renormalised) *renormalised = brenorm ? 1 : 0;
+  if (performed) *performed = (int)h.iters;
+  if (res_tot) *res_tot = h.res_tot;
+  if (log) std::copy(lg.begin(), lg.begin() + (size_t)h.iters * SOB_LOG_VALS, log);
+  if (!std::isfinite(h.res_tot)) return fail(ATUS_ERR_NUMERIC, "Sobolev residual is not finite");
+  return ATUS_OK;
+}
+
+int atus_sob_iter(atus_ctx* c, double* res_tot, int* renormalised, double* N_out) {
+  double e[SOB_LOG_VALS];
+  int done = 0;
+  // eps < 0: the iteration is always performed
+  ATUS_TRY(atus_sob_iterate(c, 1, -1.0, &done, nullptr, e));
+  if (res_tot) *res_tot = e[3];
+  if (renormalised) *renormalised = e[2] != 0.0 ? 1 : 0;
   if (N_out)
-    for (int i = 0; i < nw; ++i) N_out[i] = N[i];
-  if (!std::isfinite(rt)) return fail(ATUS_ERR_NUMERIC, "Sobolev residual is not finite");
+    for (int i = 0; i < c->ncomp; ++i) N_out[i] = e[i];
   return ATUS_OK;
 }
 
@@ -1512,12 +1619,27 @@ int atus_sob_run(atus_ctx* c, double eps, long long maxit, long long* iters, dou
   ATUS_TRY(check_single(c, "atus_sob_run"));
   if (!(eps >= 0.0)) return fail(ATUS_ERR_ARG, "eps must be >= 0");
   long long it = 0;
-  double r = 0.0;
-  do {
-    int rn = 0;
-    ATUS_TRY(atus_sob_iter(c, &r, &rn, nullptr));
-    ++it;
-  } while (r > eps && (maxit <= 0 || it < maxit));
+  double r = 0.0, r_prev = 0.0;
+  int chunk = 16;
+  for (;;) {
+    int n = chunk;
+    if (maxit > 0 && maxit - it < n) n = (int)(maxit - it);
+    int done = 0;
+    ATUS_TRY(atus_sob_iterate(c, n, eps, &done, &r, nullptr));
+    it += done;
+    if (r <= eps || (maxit > 0 && it >= maxit)) break;
+    // the next chunk: about the iterations still needed at the residual's
+    // recent rate of decrease (iterations past convergence are enqueued but
+    // leave psi unchanged), 16 to 512
+    int next = 64;
+    if (r_prev > 0 && r < r_prev && r > 0) {
+      const double rate = std::log(r_prev / r) / done;
+      const double need = std::log(r / eps) / rate;
+      next = (int)std::min(512.0, std::max(16.0, 0.9 * need));
+    }
+    chunk = next;
+    r_prev = r;
+  }
   if (iters) *iters = it;
   if (res_tot) *res_tot = r;
   return ATUS_OK;
@@ -1742,6 +1864,11 @@ struct atus_graph {
   double* dev = nullptr;   // t1 of each step [Nk], then the amplitudes [Nk][nbatch]
   double* host = nullptr;  // pinned staging of the same
   size_t nf = 0;           // amplitudes per block
+  // the context state the captured launches took by value or by pointer:
+  // a replay refuses to run once the context no longer matches it
+  const double* pot = nullptr;
+  double gs[16] = {0};
+  bool alpha2_set = false;
 };
 
 extern "C" {
@@ -1751,6 +1878,7 @@ int atus_graph_destroy(atus_graph* g) {
   if (g->c) {
     (void)hipSetDevice(g->c->device);
     (void)hipStreamSynchronize(g->c->stream);
+    if (g->exec) --g->c->live_graphs;
   }
   if (g->exec) (void)hipGraphExecDestroy(g->exec);
   if (g->graph) (void)hipGraphDestroy(g->graph);
@@ -1815,8 +1943,14 @@ int atus_graph_create(atus_ctx* c, int kind, double dt, const atus_laser* laser,
   g->graph = graph;
   if (rc != ATUS_OK) return bail(rc, "capturing the block: " + err);
   if (ec != hipSuccess || !graph) return bail(ATUS_ERR_HIP, "stream capture did not produce a graph");
-  if (hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0) != hipSuccess)
+  if (hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+    g->exec = nullptr;
     return bail(ATUS_ERR_HIP, "graph instantiation");
+  }
+  g->pot = c->pot;
+  for (int i = 0; i < 16; ++i) g->gs[i] = c->gs[i];
+  g->alpha2_set = c->alpha2_set;
+  ++c->live_graphs;
   *out = g;
   return ATUS_OK;
 }
@@ -1825,6 +1959,15 @@ int atus_graph_launch(atus_graph* g, double* t, const double* F) {
   if (!g || !t) return fail(ATUS_ERR_ARG, "null argument");
   atus_ctx* c = g->c;
   ATUS_TRY(check_ctx(c));
+  if (c->pot != g->pot || c->alpha2_set != g->alpha2_set)
+    return fail(ATUS_ERR_STATE, "the context's potential or species changed since the graph was captured; "
+                                "capture it again");
+  for (int i = 0; i < 16; ++i)
+    if (c->gs[i] != g->gs[i])
+      return fail(ATUS_ERR_STATE, "the context's couplings changed since the graph was captured; capture it again");
+  // the kinetic launches read the context's tables in place: rebuild them
+  // for the graph's dt if another block or atus_set_alpha changed them
+  ATUS_TRY(ensure_kin(c, g->dt));
   // t1 of each step with the block runner's bookkeeping, then amplitudes
   double tt = *t + 0.5 * g->dt;
   for (int m = 1; m <= g->Nk; ++m) {

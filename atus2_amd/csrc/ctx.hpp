@@ -97,6 +97,11 @@ struct atus_ctx {
   double sob_alpha[2] = {0, 0};
   double sob_step = 0;
   double2 *sob = nullptr, *lap = nullptr, *grad = nullptr;
+  // the device-side loop: its state (atus::SobState), the update partials
+  // [2][RED_MAX_BLOCKS][2] and the per-iteration log [SOB_MAX_CHUNK][4]
+  void* sob_st = nullptr;
+  double* sob_part = nullptr;
+  double* sob_log = nullptr;
 
   bool timing = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -132,6 +137,11 @@ struct atus_ctx {
   double* blk_dev = nullptr;
   size_t blk_cap = 0;
   hipEvent_t blk_ev = nullptr;
+
+  // captured graphs (atus_graph_create) not yet destroyed: they hold raw
+  // pointers into this context, so it refuses to free or replace the
+  // buffers they read while any is alive
+  int live_graphs = 0;
 
   // per-launch event pairs while profiling
   bool prof = false;

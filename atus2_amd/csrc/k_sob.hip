@@ -43,6 +43,46 @@ hipError_t launch_sobf(int N, int rm, bool contig, const double2* src, double2* 
   }
 }
 
+template <int N, int RM, int NW>
+static hipError_t gf_go(const double2* lap, const double2* psi, const double* pot, double2* out, long long npts,
+                        int comp, double g0, double g1, const double2* tw, size_t max_lds, hipStream_t s) {
+  PassCfg c = make_pass_cfg(N, RM, 1, npts / N, 1, max_lds);
+  if (c.block.x == 0 || c.block.x > 256 || c.grid.x == 0) return hipErrorInvalidConfiguration;
+  static bool ok = false;
+  hipError_t e = allow_lds(k_sob_gradfuse<N, RM, NW>, c.lds, ok);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_sob_gradfuse<N, RM, NW>), c.grid, c.block, c.lds, s, lap, psi, pot, out, npts, c.g, tw, comp,
+                     g0, g1);
+  return hipGetLastError();
+}
+
+template <int N>
+static hipError_t gf_n(int rm, int nw, const double2* lap, const double2* psi, const double* pot, double2* out,
+                       long long npts, int comp, double g0, double g1, const double2* tw, size_t max_lds,
+                       hipStream_t s) {
+  const bool r8 = rm == 8 && N > 8 && N / 8 <= 256;
+  if (nw == 1)
+    return r8 ? gf_go<N, 8, 1>(lap, psi, pot, out, npts, comp, g0, g1, tw, max_lds, s)
+              : gf_go<N, 16, 1>(lap, psi, pot, out, npts, comp, g0, g1, tw, max_lds, s);
+  if (nw == 2)
+    return r8 ? gf_go<N, 8, 2>(lap, psi, pot, out, npts, comp, g0, g1, tw, max_lds, s)
+              : gf_go<N, 16, 2>(lap, psi, pot, out, npts, comp, g0, g1, tw, max_lds, s);
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_sob_gradfuse(int N, int rm, int nw, const double2* lap, const double2* psi, const double* pot,
+                               double2* out, long long npts, int comp, double g0, double g1, const double2* tw,
+                               size_t max_lds, hipStream_t s) {
+  switch (N) {
+#define ATUS_CASE(n) \
+  case n: return gf_n<n>(rm, nw, lap, psi, pot, out, npts, comp, g0, g1, tw, max_lds, s);
+    ATUS_CASE(2) ATUS_CASE(4) ATUS_CASE(8) ATUS_CASE(16) ATUS_CASE(32) ATUS_CASE(64) ATUS_CASE(128)
+    ATUS_CASE(256) ATUS_CASE(512) ATUS_CASE(1024) ATUS_CASE(2048) ATUS_CASE(4096)
+#undef ATUS_CASE
+    default: return hipErrorInvalidValue;
+  }
+}
+
 static unsigned point_blocks(long long n) {
   long long b = (n + 255) / 256;
   if (b > 8192) b = 8192;

@@ -1047,6 +1047,51 @@ __global__ __launch_bounds__(256) void k_sobf(const double2* src, double2* out_s
   }
 }
 
+// ------------------------------------------------------------ k_sob_gradfuse
+// The L2 gradient between the two Sobolev filters of one iteration
+// (Compute_Sobolev_Gradient, CSOB_Base.h:304-337), fused into the passes
+// along the last (contiguous) axis of a 2D/3D grid: the inverse DFT that
+// finishes the Laplacian of component `comp` (lap holds it transformed
+// along that axis only), then per point
+//   grad_in = -lap + (V + sum_j g_j |psi_j|^2) psi_comp
+// (the operations of k_sob_grad_in, in its order), then the forward DFT
+// that starts the filter of grad_in.  out: the forward-transformed grad_in.
+template <int N, int RM, int NW>
+__global__ __launch_bounds__(256) void k_sob_gradfuse(const double2* __restrict__ lap, const double2* __restrict__ psi,
+                                                      const double* __restrict__ pot, double2* __restrict__ out,
+                                                      long long npts, AxisGeo g, const double2* __restrict__ tw,
+                                                      int comp, double g0, double g1) {
+  extern __shared__ double2 lds_all[];
+  constexpr int R = Plan<N, RM>::R, P = Plan<N, RM>::P;
+  constexpr bool WAVE = P <= 64;
+  PencilMap<N, RM, true> m(g);
+  double2* lds = lds_all + m.p * lds_stride<N>();
+  const double2* twl = stage_table<N>(lds_all + (size_t)g.T * lds_stride<N>(), tw);
+  const long long off = (long long)comp * npts;
+  double2 v[R];
+  load_pencil<N, RM>(v, lap + off, m.base, 1, m.t, m.valid);
+  if constexpr (tables_in_lds<N>()) __syncthreads();
+  fft_pencil<N, RM, +1, WAVE>(v, m.t, lds, twl);
+  const double gj[2] = {g0, g1};
+  if (m.valid) {
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      const long long l = m.base + m.t + P * i;
+      double nl = 0.0;
+#pragma unroll
+      for (int j = 0; j < NW; ++j) {
+        const double2 a = psi[j * npts + l];
+        nl += gj[j] * (a.x * a.x + a.y * a.y);
+      }
+      const double vv = pot[off + l] + nl;
+      const double2 p = psi[off + l];
+      v[i] = make_double2(-v[i].x + vv * p.x, -v[i].y + vv * p.y);
+    }
+  }
+  fft_pencil<N, RM, -1, WAVE>(v, m.t, lds, twl);
+  store_pencil<N, RM>(v, out + off, m.base, 1, m.t, m.valid);
+}
+
 // ------------------------------------------------------------ pointwise
 
 __device__ __forceinline__ double x0_of(long long l, const Grid& gr) {

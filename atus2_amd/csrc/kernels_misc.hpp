@@ -145,6 +145,123 @@ __global__ __launch_bounds__(256) void k_sob_update(double2* __restrict__ psi, c
   }
 }
 
+// ---- the Sobolev loop on the device (CSOB_Base::run, CSOB_Base.h:427-505),
+// iterations enqueued back to back without a host round trip.  An iteration
+// that ends with res_tot <= eps sets `done`; the iterations enqueued after it
+// leave psi and the state as they are, so the loop stops where the
+// reference's do-while stops.
+struct SobState {
+  double proj[2][2];  // <psi,grad>, <psi,psi_sob> of each wave function
+  double f[2];        // scale of psi before this iteration's update (1 or 1/sqrt(N))
+  double N[2];        // particle numbers of the current psi
+  double res_tot;     // residual of the last iteration performed
+  long long iters;    // iterations performed
+  int done;           // an iteration ended with res_tot <= eps
+  int renorm;         // this iteration renormalises (f != 1)
+};
+// per performed iteration: N[0], N[1] (checked before its update), whether
+// it renormalised, res_tot
+constexpr int SOB_LOG_VALS = 4;
+constexpr int SOB_MAX_CHUNK = 1024;
+
+// psi <- f psi - step (grad - fak psi_sob) unless the loop is done (the
+// projection, CSOB_Base.h:360-365; Renormalize_All_Psi and the update,
+// :470-492), with fak = proj[0] / proj[1]; partials of |grad - fak psi_sob|^2
+// and of |psi|^2 after the update.  measure_only: the |psi|^2 partials of
+// the current psi alone (grad and psi_sob are not read).
+__global__ __launch_bounds__(RED_THREADS) void k_sob_update_dev(double2* __restrict__ psi,
+                                                                const double2* __restrict__ grad,
+                                                                const double2* __restrict__ sob, long long n,
+                                                                double step, const SobState* __restrict__ st, int comp,
+                                                                int measure_only, double* __restrict__ partials) {
+  double acc[2] = {0.0, 0.0};
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  if (measure_only) {
+    for (long long l = (long long)blockIdx.x * blockDim.x + threadIdx.x; l < n; l += stride) {
+      const double2 p = psi[l];
+      acc[1] += p.x * p.x + p.y * p.y;
+    }
+  } else {
+    const double fak = st->proj[comp][0] / st->proj[comp][1];
+    const double f = st->f[comp];
+    const bool frozen = st->done != 0;
+    for (long long l = (long long)blockIdx.x * blockDim.x + threadIdx.x; l < n; l += stride) {
+      double2 g = grad[l];
+      const double2 s = sob[l];
+      g.x -= fak * s.x;
+      g.y -= fak * s.y;
+      double2 p = psi[l];
+      if (!frozen) {
+        if (f != 1.0) {
+          p.x *= f;
+          p.y *= f;
+        }
+        p.x -= step * g.x;
+        p.y -= step * g.y;
+        psi[l] = p;
+      }
+      acc[0] += g.x * g.x + g.y * g.y;
+      acc[1] += p.x * p.x + p.y * p.y;
+    }
+  }
+  block_reduce_store<2>(acc, partials + (long long)blockIdx.x * 2);
+}
+
+// one workgroup: sums the update partials of the nw wave functions
+// (partials[i][nb][2], fixed order) and advances the state.  init: N of the
+// current psi only (start of a run of iterations).  Otherwise, unless the
+// loop is done: res_i = ar sum |g|^2 (Compute_res, CSOB_Base.h:370-397),
+// res_tot = sqrt(sum res_i^2), the iteration's log entry, iters + 1, done
+// if res_tot <= eps; then N of the updated psi and the renormalisation of
+// the next iteration (|N_i - 1| > 1e-5 for any i, CSOB_Base.h:470-479).
+__global__ __launch_bounds__(RED_THREADS) void k_sob_finalize(SobState* __restrict__ st,
+                                                              const double* __restrict__ partials, int nb, int nw,
+                                                              double ar, double eps, int init, double* __restrict__ log,
+                                                              int slot) {
+  __shared__ double tot[2][2];
+  for (int i = 0; i < nw; ++i) {
+    double acc[2] = {0.0, 0.0};
+    const double* p = partials + (long long)i * nb * 2;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+      acc[0] += p[(long long)b * 2];
+      acc[1] += p[(long long)b * 2 + 1];
+    }
+    block_reduce_store<2>(acc, tot[i]);
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  if (init) {
+    st->done = 0;
+    st->iters = 0;
+    st->res_tot = 0.0;
+  } else {
+    if (st->done) return;
+    double rsq = 0.0;
+    for (int i = 0; i < nw; ++i) {
+      const double r = tot[i][0] * ar;
+      rsq += r * r;
+    }
+    const double rt = sqrt(rsq);
+    if (log) {
+      double* e = log + (long long)slot * SOB_LOG_VALS;
+      e[0] = st->N[0];
+      e[1] = nw > 1 ? st->N[1] : 0.0;
+      e[2] = st->renorm;
+      e[3] = rt;
+    }
+    st->res_tot = rt;
+    st->iters += 1;
+    if (rt <= eps) st->done = 1;
+  }
+  bool renorm = false;
+  for (int i = 0; i < nw; ++i) {
+    st->N[i] = ar * tot[i][1];
+    if (fabs(st->N[i] - 1) > 1e-5) renorm = true;
+  }
+  for (int i = 0; i < nw; ++i) st->f[i] = renorm ? 1.0 / sqrt(st->N[i]) : 1.0;
+  st->renorm = renorm ? 1 : 0;
+}
+
 __global__ __launch_bounds__(256) void k_scale(double2* __restrict__ psi, long long n, double f) {
   const long long stride = (long long)gridDim.x * blockDim.x;
   for (long long l = (long long)blockIdx.x * blockDim.x + threadIdx.x; l < n; l += stride) psi[l] = cscale(psi[l], f);

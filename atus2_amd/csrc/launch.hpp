@@ -82,6 +82,13 @@ hipError_t launch_sobf(int N, int rm, bool contig, const double2* src, double2* 
                        long long comp_stride, int ncomp, long long inner, const double2* tw, const SobArgs& sa,
                        size_t max_lds, hipStream_t s);
 
+// the Sobolev iteration's L2 gradient of component `comp` fused into the
+// last-axis inverse (Laplacian) and forward (gradient filter) passes of a
+// 2D/3D grid of nw (1 or 2) wave functions
+hipError_t launch_sob_gradfuse(int N, int rm, int nw, const double2* lap, const double2* psi, const double* pot,
+                               double2* out, long long npts, int comp, double g0, double g1, const double2* tw,
+                               size_t max_lds, hipStream_t s);
+
 // pointwise V step on all points (unfused path)
 hipError_t launch_point(int vk, int nc, double2* psi, const Grid& gr, const VArgs& va, hipStream_t s);
 

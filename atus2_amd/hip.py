@@ -8,6 +8,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -23,7 +24,7 @@ EXPORTED = [
     "atus_set_alpha", "atus_ft", "atus_kinetic", "atus_nl_step", "atus_nl_if_step", "atus_bragg_ad",
     "atus_set_momentum", "atus_run_block", "atus_particle_number", "atus_expval_x", "atus_expval_k",
     "atus_rabi_integrals", "atus_energy", "atus_sob_setup", "atus_sob_renormalize", "atus_sob_iter",
-    "atus_sob_mu", "atus_set_timing", "atus_last_block_ms", "atus_run_block_profiled", "atus_copy_bandwidth", "atus_tune",
+    "atus_sob_mu", "atus_sob_iterate", "atus_set_timing", "atus_last_block_ms", "atus_run_block_profiled", "atus_copy_bandwidth", "atus_tune",
     "atus_batch_create", "atus_batch_set_runs", "atus_batch_run_block", "atus_batch_particle_numbers",
     "atus_batch_rabi_integrals", "atus_sob_run", "atus_graph_create", "atus_graph_launch", "atus_graph_destroy",
     "atus_set_alpha2", "atus_run_block_ds", "atus_v_step_ds",
@@ -109,6 +110,7 @@ def load():
         "atus_sob_setup": (I, [P, P, D]),
         "atus_sob_renormalize": (I, [P]),
         "atus_sob_iter": (I, [P, C.POINTER(D), C.POINTER(I), C.POINTER(D)]),
+        "atus_sob_iterate": (I, [P, I, D, C.POINTER(I), C.POINTER(D), P]),
         "atus_sob_mu": (I, [P, P]),
         "atus_set_timing": (I, [P, I]),
         "atus_last_block_ms": (I, [P, C.POINTER(D)]),
@@ -196,6 +198,7 @@ class Context:
         _check(L.atus_ctx_create(C.byref(h), self.dim, n, dxa, dka, self.ncomp))
         self._h = h
         self._amp_keep = None
+        self._graphs = weakref.WeakSet()
 
     @classmethod
     def from_grid(cls, g, ncomp=1):
@@ -204,8 +207,12 @@ class Context:
         return cls(shape, [g.dx[a] for a in range(g.dim)], [g.dk[a] for a in range(g.dim)], ncomp)
 
     def close(self):
+        """Destroys the context, after the graphs captured on it (they hold
+        its device buffers; atus_ctx_destroy refuses while any is alive)."""
         if getattr(self, "_h", None):
-            _lib.atus_ctx_destroy(self._h)
+            for g in list(getattr(self, "_graphs", ())):
+                g.close()
+            _check(_lib.atus_ctx_destroy(self._h))
             self._h = None
 
     def __del__(self):
@@ -403,6 +410,15 @@ class Context:
         ds.duration[0], ds.duration[1] = float(duration[0]), float(duration[1])
         _check(_lib.atus_v_step_ds(self._h, int(kind), float(dt), float(t), C.byref(las), C.byref(ds)))
 
+    def sob_iterate(self, count, eps=-1.0):
+        """Up to `count` loop bodies on the device (atus_sob_iterate):
+        (performed, res_tot, log rows (N0, N1, renormalised, res_tot))."""
+        done = C.c_int()
+        r = C.c_double()
+        log = np.zeros((int(count), 4))
+        _check(_lib.atus_sob_iterate(self._h, int(count), float(eps), C.byref(done), C.byref(r), _dp(log)))
+        return done.value, r.value, log[:done.value]
+
     def sob_run(self, eps, maxit=0):
         """CSOB_Base::run: iterations until res_tot <= eps (maxit > 0 caps
         them); returns (iterations, res_tot)."""
@@ -477,6 +493,7 @@ class BatchContext(Context):
         _check(L.atus_batch_create(C.byref(h), self.dim, n, dxa, dka, 2, self.nbatch))
         self._h = h
         self._amp_keep = None
+        self._graphs = weakref.WeakSet()
 
     def set_runs(self, gs, lasers):
         """gs: nbatch x 4 couplings; lasers: nbatch Laser structures."""
@@ -615,16 +632,24 @@ class Graph:
         _check(L.atus_graph_create(ctx._h, int(kind), float(dt), C.byref(las) if las is not None else None, int(Nk),
                                    C.byref(h)))
         self._h = h
+        # the context outlives its graphs, and closes them before itself
+        self._ctx = ctx
+        ctx._graphs.add(self)
         self.Nk = int(Nk)
         self.dt = float(dt)
+        self.nbatch = getattr(ctx, "nbatch", 1)
 
     def launch(self, t, F=None):
         """F: amplitudes of V steps 1..Nk ((Nk,) or (Nk, nbatch)) or None."""
+        if not getattr(self, "_h", None):
+            raise AtusError("graph is closed")
         tt = C.c_double(t)
         if F is None:
             _check(_lib.atus_graph_launch(self._h, C.byref(tt), None))
         else:
             a = np.ascontiguousarray(F, dtype=np.float64)
+            if a.size != self.Nk * self.nbatch:
+                raise ValueError(f"amplitudes: expected {self.Nk * self.nbatch} values (Nk x nbatch), got shape {a.shape}")
             _check(_lib.atus_graph_launch(self._h, C.byref(tt), _dp(a)))
         return tt.value
 
@@ -632,6 +657,10 @@ class Graph:
         if getattr(self, "_h", None):
             _lib.atus_graph_destroy(self._h)
             self._h = None
+            ctx = getattr(self, "_ctx", None)
+            if ctx is not None:
+                ctx._graphs.discard(self)
+                self._ctx = None
 
     def __del__(self):
         try:

@@ -140,7 +140,12 @@ double CRT_Base_IF<T, dim, no_int_states>::amplitude_at(double t1) {
 template <class T, int dim, int no_int_states>
 double CRT_Base_IF<T, dim, no_int_states>::Amplitude_at_time() {
   if (!amp_is_t) return 1;
-  return amplitude_at(this->Get_t() + 0.5 * this->Get_dt());
+  // host steps written for the reference call this from every thread of a
+  // parallel region (bragg.cpp:201); the evaluator's time variable is shared
+  double F;
+#pragma omp critical(atus_amplitude)
+  F = amplitude_at(this->Get_t() + 0.5 * this->Get_dt());
+  return F;
 }
 
 template <class T, int dim, int no_int_states>

@@ -85,6 +85,10 @@ class CRT_shared {
 
   generic_header m_header;
   int m_no_of_pts;
+  // points of the half spectrum of a real transform along the last axis
+  // (CRT_shared.h:147-169).  No solver here reads it; it stays because it
+  // belongs to the protected members that drivers derived from the solver
+  // templates compile against.
   int m_no_of_pts_red;
   int m_shift_x;
   int m_shift_y;

@@ -11,7 +11,9 @@
 #ifndef ATUS_AMD_CSOB_BASE_H
 #define ATUS_AMD_CSOB_BASE_H
 
+#include <algorithm>
 #include <array>
+#include <cmath>
 #include <cstdio>
 #include <iostream>
 #include <memory>
@@ -162,23 +164,44 @@ template <class T, int dim, int no_wf>
 void CSOB_Base<T, dim, no_wf>::run() {
   if (!m_device_ready) to_device();
   Renormalize_All_Psi();
+  // loop bodies run on the device in chunks (atus_sob_iterate), which stop
+  // after the first body with res <= epsilon as the do-while does; the
+  // per-iteration lines come from the chunk's log
   int counter = 0;
-  do {
-    int renorm = 0;
-    double N[2] = {0, 0};
-    atus_host::check(atus_sob_iter(m_set->device(), &m_res_tot, &renorm, N), "atus_sob_iter");
-    for (int i = 0; i < no_wf; i++) {
-      // the particle numbers the iteration checked, before its update
-      m_N[i] = renorm ? 1.0 : N[i];
-      std::cout << "N[" << i << "] = " << N[i] << std::endl;
+  int chunk = 16;
+  double res_prev = 0.0;
+  std::vector<double> log(4 * 1024);
+  for (;;) {
+    int n = chunk;
+    if (m_max_iter > 0 && m_max_iter - counter < n) n = (int)(m_max_iter - counter);
+    int done = 0;
+    atus_host::check(atus_sob_iterate(m_set->device(), n, m_epsilon, &done, &m_res_tot, log.data()), "atus_sob_iterate");
+    for (int k = 0; k < done; k++) {
+      const double* e = log.data() + 4 * k;
+      const bool renorm = e[2] != 0.0;
+      for (int i = 0; i < no_wf; i++) {
+        // the particle numbers the iteration checked, before its update
+        m_N[i] = renorm ? 1.0 : e[i];
+        std::cout << "N[" << i << "] = " << e[i] << std::endl;
+      }
+      printf("brenorm = %s\n", renorm ? "true" : "false");
+      std::cout << "--- " << counter << std::endl;
+      std::cout << "res " << e[3] << std::endl;
+      counter++;
     }
-    printf("brenorm = %s\n", renorm ? "true" : "false");
-    std::cout << "--- " << counter << std::endl;
-    std::cout << "res " << m_res_tot << std::endl;
-    counter++;
     m_iterations = counter;
+    if (m_res_tot <= m_epsilon) break;
     if (m_max_iter > 0 && counter >= m_max_iter) break;
-  } while (m_res_tot > m_epsilon);
+    // next chunk: about the iterations still needed at the residual's
+    // recent rate of decrease, 16 to 512 (bodies past convergence are
+    // enqueued but change nothing)
+    chunk = 64;
+    if (res_prev > 0 && m_res_tot < res_prev && m_res_tot > 0 && m_epsilon > 0) {
+      const double need = std::log(m_res_tot / m_epsilon) / (std::log(res_prev / m_res_tot) / done);
+      chunk = (int)std::min(512.0, std::max(16.0, 0.9 * need));
+    }
+    res_prev = m_res_tot;
+  }
 }
 
 // CSOB_Base::Compute_mu (CSOB_Base.h:400-424)

@@ -39,23 +39,27 @@ void FieldSet::flush() {
 }
 
 atus_ctx* FieldSet::device() {
+  std::lock_guard<std::mutex> lk(m_mutex);
   flush();
   for (int c = 0; c < m_ncomp; ++c) m_state[c] = DEVICE_NEWER;
   return m_ctx;
 }
 
 atus_ctx* FieldSet::device_readonly() {
+  std::lock_guard<std::mutex> lk(m_mutex);
   flush();
   return m_ctx;
 }
 
 fftw_complex* FieldSet::host(int c) {
+  std::lock_guard<std::mutex> lk(m_mutex);
   if (m_state[c] == DEVICE_NEWER) check(atus_download(m_ctx, c, m_mirror[c].data()), "atus_download");
   m_state[c] = HOST_NEWER;
   return reinterpret_cast<fftw_complex*>(m_mirror[c].data());
 }
 
 const fftw_complex* FieldSet::host_view(int c) {
+  std::lock_guard<std::mutex> lk(m_mutex);
   if (m_state[c] == DEVICE_NEWER) {
     check(atus_download(m_ctx, c, m_mirror[c].data()), "atus_download");
     m_state[c] = IN_SYNC;
@@ -70,6 +74,7 @@ atus_ctx* FieldSet::make_batch(int nbatch) const {
 }
 
 void FieldSet::device_changed() {
+  std::lock_guard<std::mutex> lk(m_mutex);
   for (int c = 0; c < m_ncomp; ++c) m_state[c] = DEVICE_NEWER;
 }
 

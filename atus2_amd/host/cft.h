@@ -12,7 +12,9 @@
 #define ATUS_AMD_CFT_H
 
 #include <cstdint>
+#include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -63,6 +65,10 @@ class FieldSet {
   double m_dx[3] = {0, 0, 0}, m_dk[3] = {0, 0, 0};
   std::vector<std::vector<double>> m_mirror;
   std::vector<int> m_state;
+  // host code written for the reference calls Getp2In() from every thread
+  // of a parallel region (bragg.cpp:187-194): the mirror states change
+  // under this lock
+  std::mutex m_mutex;
   void flush();
 };
 

@@ -148,10 +148,15 @@ def load_net(path=WEIGHTS):
     return net
 
 
-def solve_large(case, start, net=None, eps=EPS, maxit=0):
+def solve_large(case, start, net=None, eps=EPS, maxit=0, state=False):
     """The minimiser on 128^3 points of [-16, 16)^3 from the cold or the
     warm start: (iterations, seconds from the start state to res <= eps,
-    mu, res)."""
+    mu, res); with state=True also (psi, mu, energy) after renormalising
+    the result to N = 1 exactly (the minimiser leaves |N - 1| <= 1e-5
+    alone, CSOB_Base.h:470-479, so two runs end at slightly different N).
+    Two runs stopped at res <= eps agree in psi and mu to first order in
+    the norm of the remaining gradient (sqrt(eps) = 1e-5: mu carries the
+    first variation of the interaction term), in the energy to second."""
     half = (L_BOX / 2,) * 3
     dx, dk, xs = _grid(N_GRID, half)
     V = _potential(case, xs)
@@ -172,7 +177,18 @@ def solve_large(case, start, net=None, eps=EPS, maxit=0):
         ctx.synchronize()
         el = time.perf_counter() - tic
         mu = float(ctx.sob_mu()[0])
-    return its, el, mu, res
+        if not state:
+            return its, el, mu, res
+        ctx.sob_renormalize()
+        return its, el, mu, res, ctx.download(0), float(ctx.sob_mu()[0]), float(ctx.energy())
+
+
+def compare_states(a, b):
+    """Relative L2 difference of two ground states of N = 1 (real up to a
+    global phase, which is removed first)."""
+    ph = np.vdot(b, a)
+    a = a * (abs(ph) / ph) if ph != 0 else a
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
 def main(argv=None):
@@ -186,7 +202,9 @@ def main(argv=None):
     tr.add_argument("--out", default=WEIGHTS)
     ev = sub.add_parser("eval")
     ev.add_argument("--cases", type=int, default=256)
+    ev.add_argument("--first", type=int, default=0, help="start at this held-out case")
     ev.add_argument("--weights", default=WEIGHTS)
+    ev.add_argument("--rows", default=None, help="write one CSV row per case here")
     a = ap.parse_args(argv)
 
     def log(m):
@@ -211,20 +229,33 @@ def main(argv=None):
         return
     net = load_net(a.weights)
     rows = []
-    for i, c in enumerate(cases(a.cases, 55)):
-        ic, tc, mc, _ = solve_large(c, "cold")
-        iw, tw, mw, _ = solve_large(c, "warm", net)
-        rows.append((ic, tc, iw, tw, abs(mc - mw)))
-        if (i + 1) % 16 == 0:
+    fout = open(a.rows, "w") if a.rows else None
+    if fout:
+        fout.write("case,wx,wy,wz,g,cold_iters,cold_s,warm_iters,warm_s,mu_cold_N1,mu_warm_N1,E_cold_N1,E_warm_N1,"
+                   "psi_rel_l2_N1\n")
+    all_cases = cases(a.first + a.cases, 55)
+    for i in range(a.first, a.first + a.cases):
+        c = all_cases[i]
+        ic, tc, _, _, pc, mc, ec = solve_large(c, "cold", state=True)
+        iw, tw, _, _, pw, mw, ew = solve_large(c, "warm", net, state=True)
+        d = compare_states(pw, pc)
+        rows.append((ic, tc, iw, tw, abs(mc - mw) / abs(mc), d, abs(ec - ew) / abs(ec)))
+        if fout:
+            fout.write(f"{i},{c.w[0]:.6f},{c.w[1]:.6f},{c.w[2]:.6f},{c.g:.4f},{ic},{tc:.4f},{iw},{tw:.4f},"
+                       f"{mc:.15f},{mw:.15f},{ec:.15f},{ew:.15f},{d:.3e}\n")
+            fout.flush()
+        if (i + 1 - a.first) % 8 == 0:
             r = np.array(rows)
-            log(f"{i + 1} cases: cold {r[:, 1].sum():.1f} s, warm {r[:, 3].sum():.1f} s")
+            log(f"{i + 1 - a.first} cases: cold {r[:, 1].sum():.1f} s, warm {r[:, 3].sum():.1f} s")
     r = np.array(rows)
-    print(json.dumps({"cases": a.cases, "grid": [N_GRID] * 3, "eps": EPS, "step": STEP_128,
+    print(json.dumps({"cases": a.cases, "first": a.first, "grid": [N_GRID] * 3, "eps": EPS, "step": STEP_128,
                       "cold_s": float(r[:, 1].sum()), "warm_s": float(r[:, 3].sum()),
                       "ratio": float(r[:, 3].sum() / r[:, 1].sum()),
                       "cold_iters_mean": float(r[:, 0].mean()), "warm_iters_mean": float(r[:, 2].mean()),
                       "worst_case_ratio": float(np.max(r[:, 3] / r[:, 1])),
-                      "max_mu_difference": float(r[:, 4].max())}))
+                      "max_rel_mu_difference_N1": float(r[:, 4].max()),
+                      "max_psi_rel_l2_N1": float(r[:, 5].max()),
+                      "max_rel_energy_difference_N1": float(r[:, 6].max())}))
 
 
 if __name__ == "__main__":

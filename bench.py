@@ -20,11 +20,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
-# HBM bytes per launch of each kernel family at 256^3, from rocprofv3
+# HBM bytes per step of the 256^3 workload, per schedule, from rocprofv3
 # FETCH_SIZE / WRITE_SIZE passes scaled by a copy kernel of known size
 TRAFFIC_TABLE = os.path.join(ROOT, "profiles", "hbm_traffic_bragg256.json")
-# the library's values per thread when a family is not tuned
-DEFAULT_RM = {"pass_contig": 16, "pass_strided": 8, "kin": 16, "vfuse": 8, "point": 16}
 
 
 def log(msg):
@@ -47,12 +45,15 @@ def gpu_run(args, wl):
     t = 0.0
     choice = {}
     if args.tuning:
-        # fixed choices, e.g. "pass_strided=8,vfuse=8"
+        # fixed choices, e.g. "pass_strided=8,vfuse=8,slab=-1"
         for item in args.tuning.split(","):
-            fam, rm = item.split("=")
-            ctx.tune(fam.strip(), int(rm))
-            choice[fam.strip()] = int(rm)
-        log(f"[bench] fixed values per thread: {choice}")
+            fam, v = (w.strip() for w in item.split("="))
+            if fam == "slab":
+                ctx.set_slab(int(v))
+            else:
+                ctx.tune(fam, int(v))
+            choice[fam] = int(v)
+        log(f"[bench] fixed choices: {choice}")
     elif not args.no_autotune:
         from atus2_amd.autotune import autotune
         t, choice = autotune(ctx, hip.STEP_BRAGG_AD, wl.dt, t, las, wl.amp, Nk=10, reps=2)
@@ -205,14 +206,19 @@ def scan_run(steps):
             "sample": f"{steps} of the configuration's 3000 steps per run (python -m atus2_amd.scan runs all)"}
 
 
-def measured_traffic(n, family, values_per_thread):
-    """HBM bytes per launch measured by counters for this family, or None
-    when the table does not cover the configuration."""
+def measured_step_traffic(n, tuning):
+    """HBM bytes per Strang step measured by counters (FETCH_SIZE +
+    WRITE_SIZE over every launch of the step's kernels, calibrated by a copy
+    kernel) for the schedule the bench ran, or None when the table holds no
+    measurement of that schedule."""
     if n != 256 or not os.path.exists(TRAFFIC_TABLE):
         return None
     with open(TRAFFIC_TABLE) as f:
         tab = json.load(f)
-    return tab["families"].get(family, {}).get(str(values_per_thread))
+    for entry in tab.get("steps", []):
+        if entry["tuning"] == {k: int(v) for k, v in tuning.items()}:
+            return entry["bytes_per_step"]
+    return None
 
 
 def ground_state_run(ncases):
@@ -297,17 +303,12 @@ def main():
     r = gpu_run(args, wl)
     npts = float(math.prod(wl.shape))
     value = npts * args.steps / r["elapsed"] / 1e9
-    traffic = measured_traffic(wl.n, r["dominant"], r["tuning"].get(r["dominant"], DEFAULT_RM[r["dominant"]]))
-    if traffic is not None:
-        traffic *= r["dom_share"]   # the table holds whole-grid launches
     # a step is four passes, each reading and writing both components once
-    # (256 B per point and step); its HBM traffic is the four families'
-    # measured whole-grid traffic (the y pass twice)
+    # (256 B per point and step); its HBM traffic is measured by counters
+    # for the schedule that ran (None if that schedule was not measured)
     step_bytes = 4 * 2 * 32.0 * math.prod(wl.shape)
     step_gbps = step_bytes / (r["elapsed"] / args.steps) / 1e9
-    fam_traffic = [measured_traffic(wl.n, f, r["tuning"].get(f, DEFAULT_RM[f])) for f in ("vfuse", "kin")]
-    ys = measured_traffic(wl.n, "pass_strided", r["tuning"].get("pass_strided", DEFAULT_RM["pass_strided"]))
-    step_traffic = None if None in fam_traffic or ys is None else sum(fam_traffic) + 2 * ys
+    step_traffic = measured_step_traffic(wl.n, r["tuning"])
     gs = ground_state_run(args.gs_cases) if args.gs_cases > 0 else None
     others = None if args.no_other_configs else other_configs_run(not (args.no_autotune or args.tuning))
     scan = scan_run(args.scan_steps) if args.scan_steps > 0 else None
@@ -347,8 +348,7 @@ def main():
                                          "effective_GBps": 448 * npts / (r["elapsed"] / args.steps) / 1e9},
                      "dominant_kernel": {"kernel": r["dominant"], "avg_us": 1e3 * r["dom_avg_ms"],
                                          "algorithmic_bytes_per_launch": r["bytes_per_launch"],
-                                         "achieved": r["achieved"], "frac": r["achieved"] / HBM_PEAK_GBS,
-                                         "traffic": traffic}},
+                                         "achieved": r["achieved"], "frac": r["achieved"] / HBM_PEAK_GBS}},
         "cpu_baseline": cpu,
         "ground_state": gs,
         "other_configs": others,

@@ -63,7 +63,8 @@ int atus_snapshot_save(atus_ctx *ctx, int slot);
 int atus_snapshot_restore(atus_ctx *ctx, int slot);
 /* device copy of src components [src_comp, src_comp + n) into dst
  * components [dst_comp, dst_comp + n); same device and grid size (e.g. one
- * run's state into the runs of a batch) */
+ * run's state into the runs of a batch).  Within one context the two
+ * ranges must not overlap (ATUS_ERR_ARG); identical ranges are a no-op. */
 int atus_copy_components(atus_ctx *dst, int dst_comp, atus_ctx *src, int src_comp, int n);
 /* CRT_Base::Init_Potential + Setup_Potential (CRT_Base.h:277-298):
  * npts doubles for component comp; host == NULL removes all potentials
@@ -208,11 +209,22 @@ int atus_sob_renormalize(atus_ctx *ctx);
  * optional outputs: res_tot, renormalised (0/1) and N (ncomp particle
  * numbers checked before the update, CSOB_Base.h:470-474) */
 int atus_sob_iter(atus_ctx *ctx, double *res_tot, int *renormalised, double *N);
+/* up to `count` (1..1024) bodies of that loop enqueued back to back, with
+ * no host round trip between them: the projection factor, the residual,
+ * the particle numbers and the renormalisation decision stay on the
+ * device.  The first body whose res_tot <= eps ends the loop as the
+ * reference's do-while does: the bodies enqueued after it leave psi
+ * unchanged.  Outputs: *performed (bodies that updated psi), *res_tot of
+ * the last of them, and if log != NULL four values per performed body:
+ * N[0], N[1] (checked before its update), renormalised (0/1), res_tot --
+ * what CSOB_Base::run prints per iteration.  eps < 0: all count bodies. */
+int atus_sob_iterate(atus_ctx *ctx, int count, double eps, int *performed, double *res_tot, double *log);
 /* Compute_mu (CSOB_Base.h:400-424, with Get_Ar() = m_ar): ncomp values */
 int atus_sob_mu(atus_ctx *ctx, double *mu);
 /* CSOB_Base::run (CSOB_Base.h:427-505) after Renormalize_All_Psi: loop
  * bodies until res_tot <= eps; maxit > 0 caps the count (the reference
- * loops without a cap).  Outputs: iterations done, last res_tot. */
+ * loops without a cap).  Outputs: iterations done, last res_tot.  Runs
+ * atus_sob_iterate in chunks sized from the residual's rate of decrease. */
 int atus_sob_run(atus_ctx *ctx, double eps, long long maxit, long long *iters, double *res_tot);
 
 /* ---- timing helper: average device time (ms) of the last atus_run_block
@@ -300,7 +312,22 @@ int atus_batch_rabi_integrals(atus_ctx *ctx, int comp, const double *k0, int n, 
  * capture) or atus_batch_run_block (batches; the run parameters stay those
  * of atus_batch_set_runs).  Each launch takes the amplitudes of the block,
  * F[m-1] (single) or F[(m-1)*nbatch + r] (batch), for V step m = 1..Nk at
- * the t1 the block runner would use (NULL: all 1), and advances *t. */
+ * the t1 the block runner would use (NULL: all 1), and advances *t.
+ *
+ * What a graph freezes at capture, and what it checks at each launch:
+ *  - dt and Nk, the step kind and (single runs) the laser parameters are
+ *    the graph's own; they never change.
+ *  - The kinetic launches read the context's kinetic tables in place.  A
+ *    launch rebuilds them for the graph's dt first, so blocks of another dt
+ *    (or atus_set_alpha) in between do not leak into a replay; a replay
+ *    uses the context's current alpha.
+ *  - The V launches took the couplings (atus_set_g) by value and the
+ *    potential buffer by address.  A launch returns ATUS_ERR_STATE if the
+ *    couplings, the potential buffer or the species split (atus_set_alpha2)
+ *    differ from capture; capture again.
+ *  - While a graph of a context exists, atus_ctx_destroy, freeing or first
+ *    allocating its potential (atus_set_potential) and atus_set_alpha2
+ *    return ATUS_ERR_STATE: destroy the context's graphs first. */
 typedef struct atus_graph atus_graph;
 int atus_graph_create(atus_ctx *ctx, int kind, double dt, const atus_laser *laser, int Nk, atus_graph **out);
 int atus_graph_launch(atus_graph *graph, double *t, const double *F);

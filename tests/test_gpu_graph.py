@@ -147,3 +147,89 @@ def test_graph_of_a_slab_chained_3d_block(hip, orc):
     finally:
         for c in ctxs:
             c.close()
+
+
+@pytest.mark.parametrize("shape", [(4096,), (32, 64), (16, 32, 64)])
+def test_graph_rebuilds_its_kinetic_tables_after_a_block_of_another_dt(hip, orc, shape):
+    # run_sequence gives every sequence item its own dt (Set_dt,
+    # CRT_Base.h:757-758): an eager block at dt2 between two replays of a
+    # graph captured at dt1 must not change what the replay computes
+    dim = len(shape)
+    g = orc.make_grid(shape, L=[40.0, 30.0, 36.0][:dim])
+    psis = _state(orc, g, 2, 21)
+    las = hip.laser(amp0=4.0, laser_k=3 * 2 * np.pi / 40.0, laser_domh=8.0, deltaL=(0.0, 0.3))
+    dt1, dt2, Nk = 1e-3, 3e-3, 10
+    ctxs = []
+    for _ in range(2):
+        ctx = hip.Context.from_grid(g, 2)
+        for c in range(2):
+            ctx.upload(c, psis[c])
+        ctx.set_alpha([0.5] * dim)
+        ctx.set_g([100.0, 95.0, 95.0, 100.0])
+        ctxs.append(ctx)
+    eager, rep = ctxs
+    try:
+        with hip.Graph(rep, hip.STEP_BRAGG_AD, dt1, Nk, las) as G:
+            te = tr = 0.0
+            for blk in range(3):
+                # the dt2 block on both, then the dt1 block: eager vs replay
+                te = eager.run_block(hip.STEP_FREEPROP_IF, dt2, 4, te, las)
+                tr = rep.run_block(hip.STEP_FREEPROP_IF, dt2, 4, tr, las)
+                te = eager.run_block(hip.STEP_BRAGG_AD, dt1, Nk, te, las, _amp)
+                F = np.array([_amp(t1) for t1 in hip.BatchContext.step_times(tr, dt1, Nk)])
+                tr = G.launch(tr, F)
+                assert tr == te
+                for c in range(2):
+                    assert np.array_equal(rep.download(c), eager.download(c)), (blk, c)
+    finally:
+        for c in ctxs:
+            c.close()
+
+
+def test_graph_refuses_a_changed_context_and_outlives_nothing(hip, orc):
+    g = orc.make_grid((32, 64), L=[40.0, 30.0])
+    psis = _state(orc, g, 1, 3)
+    V = 0.5 * sum(x ** 2 for x in orc.coords_x(g)) * np.ones((32, 64))
+    ctx = hip.Context.from_grid(g, 1)
+    ctx.upload(0, psis[0])
+    ctx.set_alpha([0.5, 0.5])
+    ctx.set_g([10.0])
+    ctx.set_potential(0, V)
+    G = hip.Graph(ctx, hip.STEP_FREEPROP, 1e-3, 5)
+    t = G.launch(0.0)
+    # amplitudes of the wrong length are refused before the C call
+    with pytest.raises(ValueError):
+        G.launch(t, np.ones(4))
+    # the potential buffer the graph reads cannot be freed under it
+    with pytest.raises(hip.AtusError, match="captured graphs"):
+        ctx.set_potential(0, None)
+    # new values in the same buffer are fine; new couplings are refused
+    ctx.set_potential(0, 2.0 * V)
+    t = G.launch(t)
+    ctx.set_g([11.0])
+    with pytest.raises(hip.AtusError, match="couplings changed"):
+        G.launch(t)
+    ctx.set_g([10.0])
+    t = G.launch(t)
+    # the C-ABI refuses to destroy a context with a live graph ...
+    rc = hip._lib.atus_ctx_destroy(ctx._h)
+    assert rc == -5 and "graph" in hip._lib.atus_last_error().decode()
+    # ... and the Python context closes its graphs before itself
+    ctx.close()
+    with pytest.raises(hip.AtusError, match="closed"):
+        G.launch(t)
+
+
+def test_copy_components_refuses_overlapping_ranges(hip, orc):
+    g = orc.make_grid((64,), L=20.0)
+    with hip.Context.from_grid(g, 3) as ctx:
+        rng = np.random.default_rng(4)
+        ps = [rng.standard_normal(64) + 1j * rng.standard_normal(64) for _ in range(3)]
+        for c in range(3):
+            ctx.upload(c, ps[c])
+        with pytest.raises(hip.AtusError, match="overlapping"):
+            ctx.copy_components(1, ctx, 0, 2)
+        ctx.copy_components(0, ctx, 0, 3)  # identical ranges: nothing to do
+        ctx.copy_components(2, ctx, 0, 1)  # disjoint ranges
+        assert np.array_equal(ctx.download(2), ps[0])
+        assert np.array_equal(ctx.download(1), ps[1])

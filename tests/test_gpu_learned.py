@@ -134,19 +134,30 @@ def test_sob_run_converged_mu_matches_oracle(hip, orc):
 
 def test_trained_warm_start_gains_over_the_cold_start(hip):
     # the committed network on held-out traps (seed 55, never trained on):
-    # both starts reach res <= 1e-10 at 128^3; the warm one in far fewer
-    # iterations, to the same ground state
+    # both starts reach res <= 1e-10 at 128^3; the warm one, network
+    # inference and interpolation included, in at most half the GPU wall
+    # time of the cold one over these cases, and to the same ground state
     from atus2_amd import learned as Lr
     net = Lr.load_net()
+    tc_sum = tw_sum = 0.0
     try:
-        for c in Lr.cases(2, 55):
-            ic, tc, mc, rc = Lr.solve_large(c, "cold")
-            iw, tw, mw, rw = Lr.solve_large(c, "warm", net)
-            print(f"w={c.w} g={c.g:.0f}: cold {ic} its {tc:.2f} s, warm {iw} its {tw:.2f} s, mu {mc:.8f} / {mw:.8f}")
+        for c in Lr.cases(4, 55):
+            ic, tc, _, rc, pc, mc, ec = Lr.solve_large(c, "cold", state=True)
+            iw, tw, _, rw, pw, mw, ew = Lr.solve_large(c, "warm", net, state=True)
+            d = Lr.compare_states(pw, pc)
+            print(f"w={c.w} g={c.g:.0f}: cold {ic} its {tc:.2f} s, warm {iw} its {tw:.2f} s, at N=1: "
+                  f"mu {mc:.12f} / {mw:.12f}, E {ec:.12f} / {ew:.12f}, psi rel L2 {d:.2e}")
             assert rc <= 1e-10 and rw <= 1e-10
-            assert iw < 0.7 * ic, (iw, ic)
-            # the same state: mu agrees within the minimiser's renormalisation
-            # tolerance (|N - 1| <= 1e-5 is left alone, CSOB_Base.h:472)
-            assert abs(mw - mc) <= 1e-4 * abs(mc) + 1e-5
+            tc_sum += tc
+            tw_sum += tw
+            # the same state: res <= 1e-10 leaves a gradient of norm about
+            # 1e-5, to which psi and mu (its interaction term has a first
+            # variation) are determined at equal N, and the energy to the
+            # square of it
+            assert d <= 2e-4, d
+            assert abs(mw - mc) <= 5e-5 * abs(mc), (mw, mc)
+            assert abs(ew - ec) <= 1e-8 * abs(ec), (ew, ec)
     finally:
         net.close()
+    print(f"warm / cold GPU wall time over 4 held-out cases: {tw_sum / tc_sum:.3f}")
+    assert tw_sum <= 0.5 * tc_sum, (tw_sum, tc_sum)
