@@ -1133,16 +1133,46 @@ static void kin_step(orc_rt *s, int full) {
     orc_ft_step(&s->g, s->ncomp, s->psi, full ? s->full_tab : s->half_tab);
 }
 
+/* SplitMix64 (Steele, Lea & Flood 2014) of one 64-bit counter */
+static unsigned long long splitmix64(unsigned long long z) {
+  z += 0x9e3779b97f4a7c15ULL;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+
+/* the rounding model of orc_rt (jitter > 0) */
+static void jitter_all(orc_rt *s) {
+  if (!(s->jitter > 0.0)) return;
+  const long long np = orc_npts(&s->g);
+  const unsigned long long key = splitmix64(s->jitter_seed ^ splitmix64((unsigned long long)s->jitter_count));
+  s->jitter_count++;
+  for (int c = 0; c < s->ncomp; c++) {
+    double *p = s->psi[c];
+    const unsigned long long kc = splitmix64(key + (unsigned long long)c);
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < 2 * np; i++) {
+      const double u = (double)(splitmix64(kc + (unsigned long long)i) >> 11) * 0x1.0p-52 - 1.0;
+      p[i] *= 1.0 + s->jitter * u;
+    }
+  }
+}
+
 void orc_rt_block(orc_rt *s, int kind, int Nk) {
   kin_step(s, 0);
+  jitter_all(s);
   s->t += 0.5 * s->dt;
   for (int j = 2; j <= Nk; j++) {
     v_step(s, kind);
+    jitter_all(s);
     kin_step(s, 1);
+    jitter_all(s);
     s->t += s->dt;
   }
   v_step(s, kind);
+  jitter_all(s);
   kin_step(s, 0);
+  jitter_all(s);
   s->t += 0.5 * s->dt;
 }
 

@@ -131,6 +131,15 @@ typedef struct {
   /* three-state steps: Amp[1] and the whole Beta vector */
   double amp1;
   double beta[3];
+  /* rounding model (not part of the reference): if jitter > 0, every
+   * element of every component is multiplied by (1 + jitter * u), u
+   * uniform in [-1, 1) from a counter-based generator keyed by jitter_seed
+   * and the number of perturbations so far, after each kinetic and each V
+   * application of orc_rt_block.  Two runs that differ only by it show how
+   * far last-bit rounding differences at every step carry. */
+  double jitter;
+  unsigned long long jitter_seed;
+  long long jitter_count;
 } orc_rt;
 
 /* CRT_Base_2::Do_FT_Step_*: table 1 on components [0, ncomp/2), table 2 on

@@ -56,7 +56,8 @@ class RT(C.Structure):
                 ("beta2", C.c_double), ("amp2", C.c_double), ("laser_k2", C.c_double),
                 ("laser_domh2", C.c_double), ("laser_dk2", C.c_double), ("chirp_rate2", C.c_double),
                 ("phase2", C.c_double), ("seq_time", C.c_double), ("duration", C.c_double * 2),
-                ("amp1", C.c_double), ("beta", C.c_double * 3)]
+                ("amp1", C.c_double), ("beta", C.c_double * 3),
+                ("jitter", C.c_double), ("jitter_seed", C.c_ulonglong), ("jitter_count", C.c_longlong)]
 
 
 class SOB(C.Structure):
@@ -313,11 +314,13 @@ def energy(g, psis, gs, alpha, pots=None):
 class RTSolver:
     """Owns the arrays of an orc_rt and runs reference blocks on them."""
 
-    def __init__(self, g, psis, gs, alpha, dt, t=0.0, pots=None, amp=None, alpha2=None, species2=None, **ifp):
+    def __init__(self, g, psis, gs, alpha, dt, t=0.0, pots=None, amp=None, alpha2=None, species2=None,
+                 jitter=0.0, jitter_seed=1, **ifp):
         """alpha2: kinetic coefficients of species 2 (components [n/2, n),
         CRT_Base_2); species2: dict of beta2, amp2, laser_k2, laser_domh2,
         laser_dk2, chirp_rate2, phase2, seq_time, duration (CRT_Base_IF_2,
-        bragg_ds)."""
+        bragg_ds); jitter: relative size of the rounding model applied after
+        every kinetic and V application of block() (0: none)."""
         self.g = g
         self.psis = [_c128(p).copy() for p in psis]
         self.pots = None
@@ -341,6 +344,8 @@ class RTSolver:
         for a in range(len(alpha)):
             s.alpha[a] = float(alpha[a])
         s.t = float(t)
+        s.jitter = float(jitter)
+        s.jitter_seed = int(jitter_seed)
         s.half_tab = self.half.ctypes.data
         s.full_tab = self.full.ctypes.data
         dl = ifp.pop("deltaL", (0.0, 0.0, 0.0, 0.0))

@@ -124,7 +124,7 @@ def _state(orc, g, seed, ncomp):
 @pytest.mark.parametrize("shape,rm", [((4096,), 16), ((256,), 8), ((16384,), 16), ((8192,), 8), ((64, 128), 16),
                                       ((32, 16, 64), 16), ((32, 16, 64), 8)])
 @pytest.mark.parametrize("kind", ["freeprop", "freeprop_if", "bragg_ad"])
-def test_run_block_matches_oracle(hip, orc, shape, rm, kind):
+def test_run_block_matches_oracle(hip, orc, margin, shape, rm, kind):
     dim = len(shape)
     g = orc.make_grid(shape, L=[40.0, 30.0, 36.0][:dim])
     ncomp = 1 if kind == "freeprop" else 2
@@ -159,12 +159,19 @@ def test_run_block_matches_oracle(hip, orc, shape, rm, kind):
     # nodes of Omega: V11-E+ is a cancellation with absolute error ~eps|E|
     # that enters O12 as ~eps|E|/|Omega|.  Any change of rounding upstream
     # (a different FFT) is amplified the same way, so the bound for the
-    # Bragg blocks is the oracle's own spread under a 2e-16 perturbation.
-    Sp = None
+    # Bragg blocks is 20x the oracle's own spread under last-bit rounding
+    # differences: the largest of three oracle runs that differ from it by a
+    # 2e-16 perturbation of the start state, or by a 2.2e-16 perturbation
+    # after every kinetic and V application (two seeds).  The spread of such
+    # a run varies by up to 10x with the draw, hence the largest of three.
+    Sps = []
     if kind == "bragg_ad":
         rng = np.random.default_rng(99)
         pert = [p * (1 + 2e-16 * rng.standard_normal(p.shape)) for p in psis]
-        Sp = orc.RTSolver(g, pert, gs, alpha, dt=dt, t=t0, pots=pots, amp=amp, **dict(ifp))
+        Sps.append(orc.RTSolver(g, pert, gs, alpha, dt=dt, t=t0, pots=pots, amp=amp, **dict(ifp)))
+        for seed in (7, 8):
+            Sps.append(orc.RTSolver(g, psis, gs, alpha, dt=dt, t=t0, pots=pots, amp=amp, jitter=2.2e-16,
+                                    jitter_seed=seed, **dict(ifp)))
     with hip.Context.from_grid(g, ncomp) as ctx:
         for f in hip.KFAM:
             ctx.tune(f, rm)
@@ -183,11 +190,13 @@ def test_run_block_matches_oracle(hip, orc, shape, rm, kind):
             out = [ctx.download(c) for c in range(ncomp)]
             err = rel_l2(out, S.psis)
             tol = 1e-12
-            if Sp is not None:
-                Sp.block(okind, Nk)
-                floor = rel_l2(Sp.psis, S.psis)
-                tol = max(tol, 20 * floor)
-                print(f"block {b}: gpu-vs-oracle {err:.2e}, oracle noise floor {floor:.2e}")
+            if Sps:
+                for X in Sps:
+                    X.block(okind, Nk)
+                floor = max(rel_l2(X.psis, S.psis) for X in Sps)
+                tol = min(1e-10, max(tol, 20 * floor))
+                print(f"block {b}: gpu-vs-oracle {err:.2e}, oracle rounding spread {floor:.2e}, bound {tol:.2e}")
+            margin(f"block {b}", err, tol)
             assert err < tol, (b, err, tol)
             Ns = [ctx.particle_number(c) for c in range(ncomp)]
             Ns_ref = [orc.particle_number(g, S.psis[c]) for c in range(ncomp)]
