@@ -47,6 +47,36 @@ unsigned ew_blocks(long long n) {
 
 }  // namespace
 
+// Device scratch of atus_batch_particle_numbers_noalloc, the entry point for
+// callers that cannot allocate device memory: the partial sums of its
+// segmented reduction and their per-segment results.  Each GPU holds its
+// own copy for the life of the process.  Host code never names the array
+// itself (in host code the name is a stand-in, not a device address); it
+// uses the per-device address hipGetSymbolAddress returns, taken once per
+// device when the first context is created there.  Calls on one GPU share
+// the array: calls that overlap in time on one GPU race.
+constexpr size_t NOALLOC_SCRATCH_DOUBLES = 32768;  // 256 KiB
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_DOUBLES];
+
+namespace {
+
+constexpr int MAX_SCRATCH_DEVICES = 64;
+double* g_scratch_ptr[MAX_SCRATCH_DEVICES] = {};
+
+// the address of `scratch` on the current device `dev`, looked up once
+int ensure_scratch_symbol(int dev) {
+  if (dev < 0 || dev >= MAX_SCRATCH_DEVICES) return fail(ATUS_ERR_UNSUPPORTED, "device index beyond 63");
+  if (g_scratch_ptr[dev]) return ATUS_OK;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return fail(ATUS_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch_ptr[dev] = (double*)p;
+  return ATUS_OK;
+}
+
+}  // namespace
+
 namespace {
 
 int check_ctx(atus_ctx* c) {
@@ -514,6 +544,10 @@ int ctx_create(atus_ctx** out, int dim, const long long n[3], const double dx[3]
     c->fs_n2 = c->n[0] / c->fs_n1;
   }
   if (hipGetDevice(&c->device) != hipSuccess) return bail(ATUS_ERR_HIP, "no HIP device");
+  if (ensure_scratch_symbol(c->device) != ATUS_OK) {
+    std::string m = g_err;
+    return bail(ATUS_ERR_HIP, m);
+  }
   int lds = 0;
   if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) == hipSuccess && lds > 0)
     c->max_lds = std::min<size_t>((size_t)lds, 163840);
@@ -1818,6 +1852,30 @@ int atus_batch_particle_numbers(atus_ctx* c, double* out) {
   if (!out) return fail(ATUS_ERR_ARG, "null output");
   RedNorm fn{c->psi};
   ATUS_TRY(reduce_seg(c, fn, c->npts, c->npts, c->ncomp, out));
+  for (int i = 0; i < c->ncomp; ++i) {
+    out[i] *= c->ar();
+    if (!std::isfinite(out[i])) return fail(ATUS_ERR_NUMERIC, "particle number is not finite");
+  }
+  return ATUS_OK;
+}
+
+int atus_batch_particle_numbers_noalloc(atus_ctx* c, double* out) {
+  ATUS_TRY(check_ctx(c));
+  if (!c->runs) return fail(ATUS_ERR_STATE, "not a batch context (atus_batch_create)");
+  if (!out) return fail(ATUS_ERR_ARG, "null output");
+  double* ptr = (c->device >= 0 && c->device < MAX_SCRATCH_DEVICES) ? g_scratch_ptr[c->device] : nullptr;
+  if (!ptr) return fail(ATUS_ERR_STATE, "no device scratch on this context's device");
+  // the same kernels and workgroup count as atus_batch_particle_numbers:
+  // nseg * nb partials, then nseg results after them
+  const int nseg = c->ncomp;
+  const unsigned nb = seg_blocks(c->npts, nseg);
+  const size_t need = (size_t)nseg * nb + (size_t)nseg;
+  if (need > NOALLOC_SCRATCH_DOUBLES)
+    return fail(ATUS_ERR_ARG, "batch of " + std::to_string(nseg) + " components needs " + std::to_string(need) +
+                                  " doubles of device scratch; the allocation-free entry point holds " +
+                                  std::to_string(NOALLOC_SCRATCH_DOUBLES) + " (use atus_batch_particle_numbers)");
+  RedNorm fn{c->psi};
+  ATUS_TRY(reduce_seg_into(c, fn, c->npts, c->npts, nseg, nb, ptr, ptr + (size_t)nseg * nb, out));
   for (int i = 0; i < c->ncomp; ++i) {
     out[i] *= c->ar();
     if (!std::isfinite(out[i])) return fail(ATUS_ERR_NUMERIC, "particle number is not finite");

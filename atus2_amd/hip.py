@@ -26,6 +26,7 @@ EXPORTED = [
     "atus_rabi_integrals", "atus_energy", "atus_sob_setup", "atus_sob_renormalize", "atus_sob_iter",
     "atus_sob_mu", "atus_sob_iterate", "atus_set_timing", "atus_last_block_ms", "atus_run_block_profiled", "atus_copy_bandwidth", "atus_tune",
     "atus_batch_create", "atus_batch_set_runs", "atus_batch_run_block", "atus_batch_particle_numbers",
+    "atus_batch_particle_numbers_noalloc",
     "atus_batch_rabi_integrals", "atus_sob_run", "atus_graph_create", "atus_graph_launch", "atus_graph_destroy",
     "atus_set_alpha2", "atus_run_block_ds", "atus_v_step_ds",
     "atus_snapshot_save", "atus_snapshot_restore", "atus_copy_components", "atus_v_step",
@@ -122,6 +123,7 @@ def load():
         "atus_batch_set_runs": (I, [P, C.POINTER(BatchRun), I]),
         "atus_batch_run_block": (I, [P, I, D, I, C.POINTER(D), P]),
         "atus_batch_particle_numbers": (I, [P, P]),
+        "atus_batch_particle_numbers_noalloc": (I, [P, P]),
         "atus_batch_rabi_integrals": (I, [P, I, P, I, D, P]),
         "atus_sob_run": (I, [P, D, C.c_longlong, C.POINTER(C.c_longlong), C.POINTER(D)]),
         "atus_graph_create": (I, [P, I, D, C.POINTER(Laser), I, pctx]),

@@ -302,6 +302,17 @@ int atus_batch_set_runs(atus_ctx *ctx, const atus_batch_run *runs, int nbatch);
 int atus_batch_run_block(atus_ctx *ctx, int kind, double dt, int Nk, double *t, const double *F);
 /* Get_Particle_Number of every component: out[r*ncomp_per_run + c] */
 int atus_batch_particle_numbers(atus_ctx *ctx, double *out);
+/* No counterpart in the reference.  atus_batch_particle_numbers for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc): the same kernels and the
+ * same results, with the partial sums in a fixed 256 KiB device array of
+ * the library (one per GPU, held until the process ends) instead of buffers
+ * the context allocates on first use.  The call allocates nothing.  A batch
+ * needing more than 32768 doubles of partials and results (more than 16384
+ * components, i.e. 8192 runs) is refused with ATUS_ERR_ARG.  Calls on one
+ * GPU share that array: calls that overlap in time on one GPU (several host
+ * threads, or contexts driven concurrently) race. */
+int atus_batch_particle_numbers_noalloc(atus_ctx *ctx, double *out);
 /* compute_rabi_integrals of component `comp` of every run, transformed in
  * place and back as in CRT_Base_IF.h:208-281: out[r*n + q] */
 int atus_batch_rabi_integrals(atus_ctx *ctx, int comp, const double *k0, int n, double threshold, double *out);
