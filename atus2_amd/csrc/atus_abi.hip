@@ -79,6 +79,10 @@ int ensure_scratch_symbol(int dev) {
 
 namespace {
 
+// launch limits of kernel family fam: the device's LDS per workgroup and
+// the family's tuned threads per workgroup (atus_tune_threads)
+Limits lim(const atus_ctx* c, int fam) { return Limits{c->max_lds, c->wg[fam]}; }
+
 int check_ctx(atus_ctx* c) {
   if (!c) return fail(ATUS_ERR_ARG, "null context");
   return hipSetDevice(c->device) == hipSuccess ? ATUS_OK : fail(ATUS_ERR_HIP, "hipSetDevice failed");
@@ -231,7 +235,7 @@ int pass(atus_ctx* c, int a, int dir, const double2* src, double2* dst, int ncom
   const int fam = inner == 1 ? ATUS_KFAM_PASS_CONTIG : ATUS_KFAM_PASS_STRIDED;
   ProfScope ps(c, fam);
   HIP_TRY(launch_pass(N, c->rm[fam], inner == 1, dir, src, dst, cs ? cs : c->npts, ncomp, inner, outer, c->tw[N], scale,
-                      c->max_lds, c->stream));
+                      lim(c, fam), c->stream));
   return ATUS_OK;
 }
 
@@ -245,7 +249,7 @@ int pass_slab(atus_ctx* c, int a, int dir, int s0, int S, hipStream_t st) {
   const int fam = inner == 1 ? ATUS_KFAM_PASS_CONTIG : ATUS_KFAM_PASS_STRIDED;
   double2* p = c->psi + (long long)s0 * plane;
   ProfScope ps(c, fam, (double)S / c->n[0], st);
-  HIP_TRY(launch_pass(N, c->rm[fam], inner == 1, dir, p, p, c->npts, c->ncomp, inner, outer, c->tw[N], 1.0, c->max_lds,
+  HIP_TRY(launch_pass(N, c->rm[fam], inner == 1, dir, p, p, c->npts, c->ncomp, inner, outer, c->tw[N], 1.0, lim(c, fam),
                       st));
   return ATUS_OK;
 }
@@ -260,7 +264,7 @@ int vfuse_slab(atus_ctx* c, int vk, const VArgs& va, int s0, int S, hipStream_t 
   if (vs.pot) vs.pot += (long long)s0 * plane;
   ProfScope ps(c, ATUS_KFAM_VFUSE, (double)S / c->n[0], st);
   HIP_TRY(launch_vfuse(NL, c->rm[ATUS_KFAM_VFUSE], vk, c->ncomp, c->psi + (long long)s0 * plane, c->npts,
-                       (long long)S * c->n[1], c->tw[NL], vs, c->n[1], c->n[0] - 2 * s0, c->dx[0], c->max_lds,
+                       (long long)S * c->n[1], c->tw[NL], vs, c->n[1], c->n[0] - 2 * s0, c->dx[0], lim(c, ATUS_KFAM_VFUSE),
                        st));
   return ATUS_OK;
 }
@@ -272,7 +276,7 @@ int fs_col(atus_ctx* c, int mode, double2* p, int ycount, long long cs, int vk, 
   ProfScope ps(c, mode == FS_STEP ? ATUS_KFAM_VFUSE : ATUS_KFAM_PASS_STRIDED);
   HIP_TRY(launch_fs_col(c->fs_n1, c->rm[mode == FS_STEP ? ATUS_KFAM_VFUSE : ATUS_KFAM_PASS_STRIDED], mode, vk, nc, batch,
                         p, cs, ycount, c->fs_n2, c->tw[c->fs_n1], c->tw[(int)c->npts], va, c->dx[0], scale,
-                        c->max_lds, c->stream));
+                        lim(c, mode == FS_STEP ? ATUS_KFAM_VFUSE : ATUS_KFAM_PASS_STRIDED), c->stream));
   return ATUS_OK;
 }
 
@@ -280,7 +284,7 @@ int fs_col(atus_ctx* c, int mode, double2* p, int ycount, long long cs, int vk, 
 int fs_row_kin(atus_ctx* c, int f, const VArgs& va) {
   ProfScope ps(c, ATUS_KFAM_KIN);
   HIP_TRY(launch_kin(c->fs_n2, c->rm[ATUS_KFAM_KIN], true, V_NONE, 1, c->psi, c->npts, c->ncomp, 1, c->tw[c->fs_n2],
-                     kin_tabs(c, f), va, c->dx[0], c->max_lds, c->stream, c->fs_n1));
+                     kin_tabs(c, f), va, c->dx[0], lim(c, ATUS_KFAM_KIN), c->stream, c->fs_n1));
   return ATUS_OK;
 }
 
@@ -319,7 +323,7 @@ int full_ft(atus_ctx* c, int dir, const double2* src, double2* dst, int ncomp, l
       {
         ProfScope ps(c, ATUS_KFAM_PASS_CONTIG);
         HIP_TRY(launch_pass(c->fs_n2, c->rm[ATUS_KFAM_PASS_CONTIG], true, -1, dst, dst, s_, ncomp, 1, c->fs_n1,
-                            c->tw[c->fs_n2], 1.0, c->max_lds, c->stream));
+                            c->tw[c->fs_n2], 1.0, lim(c, ATUS_KFAM_PASS_CONTIG), c->stream));
       }
       ATUS_TRY(transpose(c, dst, t, c->fs_n1, c->fs_n2, ncomp, s_, scale));
       HIP_TRY(hipMemcpy2DAsync(dst, s_ * sizeof(double2), t, s_ * sizeof(double2), c->npts * sizeof(double2), ncomp,
@@ -329,7 +333,7 @@ int full_ft(atus_ctx* c, int dir, const double2* src, double2* dst, int ncomp, l
       {
         ProfScope ps(c, ATUS_KFAM_PASS_CONTIG);
         HIP_TRY(launch_pass(c->fs_n2, c->rm[ATUS_KFAM_PASS_CONTIG], true, +1, t, t, s_, ncomp, 1, c->fs_n1,
-                            c->tw[c->fs_n2], 1.0, c->max_lds, c->stream));
+                            c->tw[c->fs_n2], 1.0, lim(c, ATUS_KFAM_PASS_CONTIG), c->stream));
       }
       ATUS_TRY(fs_col(c, FS_EXIT, t, ncomp, s_, V_NONE, 1, false, none, 1.0));
       HIP_TRY(hipMemcpy2DAsync(dst, s_ * sizeof(double2), t, s_ * sizeof(double2), c->npts * sizeof(double2), ncomp,
@@ -362,7 +366,7 @@ int kinetic_unfused(atus_ctx* c, int f) {
   {
     ProfScope ps(c, ATUS_KFAM_KIN);
     HIP_TRY(launch_kin(N0, c->rm[ATUS_KFAM_KIN], c->dim == 1, V_NONE, 1, c->psi, c->npts, c->ncomp, c->inner_of(0), c->tw[N0],
-                       kin_tabs(c, f), va, c->dx[0], c->max_lds, c->stream));
+                       kin_tabs(c, f), va, c->dx[0], lim(c, ATUS_KFAM_KIN), c->stream));
   }
   for (int a = 1; a < c->dim; ++a) ATUS_TRY(pass(c, a, +1, c->psi, c->psi, c->ncomp, 1.0));
   return ATUS_OK;
@@ -391,12 +395,12 @@ unsigned seg_blocks(long long n, int nseg) {
   return nb > want ? want : nb;
 }
 
-// the launches of reduce_seg with caller-given device buffers: `part`
-// holds nseg * nb * NV partials, `res` nseg * NV results; out (host)
-// receives res
+// the two launches of a segmented reduction into caller-given device
+// buffers: `part` holds nseg * nb * NV partials, `res` nseg * NV results
+// (no host copy: capturable into a graph)
 template <class Fn>
-int reduce_seg_into(atus_ctx* c, const Fn& fn, long long n, long long seg_stride, int nseg, unsigned nb, double* part,
-                    double* res, double* out) {
+int reduce_seg_launch(atus_ctx* c, const Fn& fn, long long n, long long seg_stride, int nseg, unsigned nb, double* part,
+                      double* res) {
   constexpr int NV = Fn::NV;
   hipLaunchKernelGGL((k_reduce_seg<Fn>), dim3(nb, (unsigned)nseg), dim3(RED_THREADS), 0, c->stream, fn, n, seg_stride,
                      part);
@@ -404,8 +408,39 @@ int reduce_seg_into(atus_ctx* c, const Fn& fn, long long n, long long seg_stride
   hipLaunchKernelGGL((k_reduce_seg_final<NV>), dim3((unsigned)nseg), dim3(RED_THREADS), 0, c->stream, part, (int)nb,
                      res);
   HIP_TRY(hipGetLastError());
+  return ATUS_OK;
+}
+
+// the launches of reduce_seg with caller-given device buffers: `part`
+// holds nseg * nb * NV partials, `res` nseg * NV results; out (host)
+// receives res
+template <class Fn>
+int reduce_seg_into(atus_ctx* c, const Fn& fn, long long n, long long seg_stride, int nseg, unsigned nb, double* part,
+                    double* res, double* out) {
+  constexpr int NV = Fn::NV;
+  ATUS_TRY(reduce_seg_launch(c, fn, n, seg_stride, nseg, nb, part, res));
   HIP_TRY(hipMemcpyAsync(out, res, sizeof(double) * (size_t)nseg * NV, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
+  return ATUS_OK;
+}
+
+// room for `need` partials and nseg * RED_MAX_NV results in the
+// segmented-reduction buffers (allocated outside any graph capture)
+int ensure_sred(atus_ctx* c, size_t need, int nseg) {
+  if (c->sred_cap >= need && c->sred_nseg >= nseg) return ATUS_OK;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  need = std::max(need, c->sred_cap);
+  nseg = std::max(nseg, c->sred_nseg);
+  (void)hipFree(c->sred_part);
+  (void)hipFree(c->sred_out);
+  c->sred_part = c->sred_out = nullptr;
+  c->sred_cap = 0;
+  c->sred_nseg = 0;
+  if (hipMalloc(&c->sred_part, sizeof(double) * need) != hipSuccess ||
+      hipMalloc(&c->sred_out, sizeof(double) * (size_t)nseg * RED_MAX_NV) != hipSuccess)
+    return fail(ATUS_ERR_NOMEM, "segmented reduction buffers");
+  c->sred_cap = need;
+  c->sred_nseg = nseg;
   return ATUS_OK;
 }
 
@@ -415,17 +450,7 @@ template <class Fn>
 int reduce_seg(atus_ctx* c, const Fn& fn, long long n, long long seg_stride, int nseg, double* out) {
   constexpr int NV = Fn::NV;
   const unsigned nb = seg_blocks(n, nseg);
-  const size_t need = (size_t)nseg * nb * NV;
-  if (c->sred_cap < need) {
-    (void)hipFree(c->sred_part);
-    (void)hipFree(c->sred_out);
-    c->sred_part = c->sred_out = nullptr;
-    c->sred_cap = 0;
-    if (hipMalloc(&c->sred_part, sizeof(double) * need) != hipSuccess ||
-        hipMalloc(&c->sred_out, sizeof(double) * (size_t)nseg * RED_MAX_NV) != hipSuccess)
-      return fail(ATUS_ERR_NOMEM, "segmented reduction buffers");
-    c->sred_cap = need;
-  }
+  ATUS_TRY(ensure_sred(c, (size_t)nseg * nb * NV, nseg));
   return reduce_seg_into(c, fn, n, seg_stride, nseg, nb, c->sred_part, c->sred_out, out);
 }
 
@@ -869,7 +894,7 @@ int single_block_slabs(atus_ctx* c, int vk, const VArgs& base, int Nk, double dt
   auto kin = [&](int f, const VArgs& va) -> int {
     ProfScope ps(c, ATUS_KFAM_KIN);
     HIP_TRY(launch_kin(N0, c->rm[ATUS_KFAM_KIN], false, V_NONE, 1, c->psi, c->npts, c->ncomp, inner0, c->tw[N0],
-                       kin_tabs(c, f), va, c->dx[0], c->max_lds, c->stream));
+                       kin_tabs(c, f), va, c->dx[0], lim(c, ATUS_KFAM_KIN), c->stream));
     return ATUS_OK;
   };
   // odd slabs on the second stream (profiled blocks record their events on
@@ -1000,7 +1025,7 @@ int resident_block(atus_ctx* c, int vk, const VArgs& base, int Nk, double dt, do
     ProfScope ps(c, ATUS_KFAM_KIN, Nk + 1.0);
     HIP_TRY(launch_block1d(N0, rm, vk, batch ? 2 : c->ncomp, batch, c->psi, c->npts,
                            batch ? c->nbatch : 1, c->tw[N0], kin_tabs(c, 0).e0, kin_tabs(c, 1).e0, base, t1tab, Ftab,
-                           c->dx[0], Nk, c->max_lds, c->stream));
+                           c->dx[0], Nk, lim(c, ATUS_KFAM_KIN), c->stream));
   }
   for (int m = 1; m <= Nk; ++m) tt += (m < Nk) ? dt : 0.5 * dt;
   *t = tt;
@@ -1027,7 +1052,7 @@ int single_block(atus_ctx* c, int vk, const VArgs& base, int Nk, double dt, doub
     {
       ProfScope ps(c, ATUS_KFAM_KIN);
       HIP_TRY(launch_kin(N0, c->rm[ATUS_KFAM_KIN], false, V_NONE, 1, c->psi, c->npts, c->ncomp, inner0, c->tw[N0], kin_tabs(c, 0), base,
-                         c->dx[0], c->max_lds, c->stream));
+                         c->dx[0], lim(c, ATUS_KFAM_KIN), c->stream));
     }
     if (c->dim == 3) ATUS_TRY(pass(c, 1, +1, c->psi, c->psi, c->ncomp, 1.0));
   } else {
@@ -1050,20 +1075,20 @@ int single_block(atus_ctx* c, int vk, const VArgs& base, int Nk, double dt, doub
     } else if (c->dim == 1) {
       ProfScope ps(c, ATUS_KFAM_KIN);
       HIP_TRY(launch_kin(N0, c->rm[ATUS_KFAM_KIN], true, vk, c->ncomp, c->psi, c->npts, c->ncomp, 1, c->tw[N0], kin_tabs(c, f), va,
-                         c->dx[0], c->max_lds, c->stream));
+                         c->dx[0], lim(c, ATUS_KFAM_KIN), c->stream));
     } else {
       const long long outer = c->npts / NL;
       const long long ppx = c->dim == 3 ? c->n[1] : 1;
       {
         ProfScope ps(c, ATUS_KFAM_VFUSE);
         HIP_TRY(launch_vfuse(NL, c->rm[ATUS_KFAM_VFUSE], vk, c->ncomp, c->psi, c->npts, outer, c->tw[NL], va, ppx, c->n[0], c->dx[0],
-                             c->max_lds, c->stream));
+                             lim(c, ATUS_KFAM_VFUSE), c->stream));
       }
       if (c->dim == 3) ATUS_TRY(pass(c, 1, -1, c->psi, c->psi, c->ncomp, 1.0));
       {
         ProfScope ps(c, ATUS_KFAM_KIN);
         HIP_TRY(launch_kin(N0, c->rm[ATUS_KFAM_KIN], false, V_NONE, 1, c->psi, c->npts, c->ncomp, inner0, c->tw[N0], kin_tabs(c, f), va,
-                           c->dx[0], c->max_lds, c->stream));
+                           c->dx[0], lim(c, ATUS_KFAM_KIN), c->stream));
       }
       if (c->dim == 3) ATUS_TRY(pass(c, 1, +1, c->psi, c->psi, c->ncomp, 1.0));
     }
@@ -1101,7 +1126,7 @@ int batch_block(atus_ctx* c, int vk, const VArgs& base, int Nk, double dt, doubl
     {
       ProfScope ps(c, ATUS_KFAM_KIN);
       HIP_TRY(launch_kin(N0, c->rm[ATUS_KFAM_KIN], false, V_NONE, 1, c->psi, c->npts, c->ncomp, inner0, c->tw[N0],
-                         kin_tabs(c, 0), base, c->dx[0], c->max_lds, c->stream));
+                         kin_tabs(c, 0), base, c->dx[0], lim(c, ATUS_KFAM_KIN), c->stream));
     }
     if (c->dim == 3) ATUS_TRY(pass(c, 1, +1, c->psi, c->psi, c->ncomp, 1.0));
   } else {
@@ -1120,20 +1145,20 @@ int batch_block(atus_ctx* c, int vk, const VArgs& base, int Nk, double dt, doubl
     } else if (c->dim == 1) {
       ProfScope ps(c, ATUS_KFAM_KIN);
       HIP_TRY(launch_vstep_batch(N0, c->rm[ATUS_KFAM_KIN], true, vk, c->psi, c->npts, B, c->tw[N0], kin_tabs(c, f), va,
-                                 1, N0, c->dx[0], c->max_lds, c->stream));
+                                 1, N0, c->dx[0], lim(c, ATUS_KFAM_KIN), c->stream));
     } else {
       const long long ppx = c->dim == 3 ? c->n[1] : 1;
       {
         ProfScope ps(c, ATUS_KFAM_VFUSE);
         HIP_TRY(launch_vstep_batch(NL, c->rm[ATUS_KFAM_VFUSE], false, vk, c->psi, c->npts,
                                    (long long)B * base.pencils_per_run, c->tw[NL], kin_tabs(c, f), va, ppx, c->n[0],
-                                   c->dx[0], c->max_lds, c->stream));
+                                   c->dx[0], lim(c, ATUS_KFAM_VFUSE), c->stream));
       }
       if (c->dim == 3) ATUS_TRY(pass(c, 1, -1, c->psi, c->psi, c->ncomp, 1.0));
       {
         ProfScope ps(c, ATUS_KFAM_KIN);
         HIP_TRY(launch_kin(N0, c->rm[ATUS_KFAM_KIN], false, V_NONE, 1, c->psi, c->npts, c->ncomp, inner0, c->tw[N0],
-                           kin_tabs(c, f), va, c->dx[0], c->max_lds, c->stream));
+                           kin_tabs(c, f), va, c->dx[0], lim(c, ATUS_KFAM_KIN), c->stream));
       }
       if (c->dim == 3) ATUS_TRY(pass(c, 1, +1, c->psi, c->psi, c->ncomp, 1.0));
     }
@@ -1438,7 +1463,7 @@ int sob_filter(atus_ctx* c, int comp, const double2* src, double2* out_sob, doub
   const int N0 = c->n[0];
   ATUS_TRY(ensure_twiddles(c, N0));
   HIP_TRY(launch_sobf(N0, c->rm[ATUS_KFAM_KIN], c->dim == 1, s, w, out_lap ? out_lap + off : nullptr, c->npts, 1, c->inner_of(0),
-                      c->tw[N0], sa, c->max_lds, c->stream));
+                      c->tw[N0], sa, lim(c, ATUS_KFAM_KIN), c->stream));
   for (int a = 1; a < c->dim; ++a) {
     ATUS_TRY(pass(c, a, +1, w, w, 1, 1.0));
     if (out_lap) ATUS_TRY(pass(c, a, +1, out_lap + off, out_lap + off, 1, 1.0));
@@ -1486,7 +1511,7 @@ int sob_body(atus_ctx* c) {
         s = w;
       }
       HIP_TRY(launch_sobf(N0, c->rm[ATUS_KFAM_KIN], false, s, w, c->lap + off, np, 1, c->inner_of(0), c->tw[N0], sa,
-                          c->max_lds, c->stream));
+                          lim(c, ATUS_KFAM_KIN), c->stream));
       for (int a = 1; a <= last; ++a) ATUS_TRY(pass(c, a, +1, w, w, 1, 1.0));
       for (int a = 1; a < last; ++a) ATUS_TRY(pass(c, a, +1, c->lap + off, c->lap + off, 1, 1.0));
     }
@@ -1495,11 +1520,11 @@ int sob_body(atus_ctx* c) {
       sa.alpha = c->sob_alpha[i];
       double2* w = c->grad + off;
       HIP_TRY(launch_sob_gradfuse(NL, c->rm[ATUS_KFAM_VFUSE], nw, c->lap, c->psi, c->pot, c->grad, np, i,
-                                  c->gs[0 + i * nw], nw > 1 ? c->gs[1 + i * nw] : 0.0, c->tw[NL], c->max_lds,
+                                  c->gs[0 + i * nw], nw > 1 ? c->gs[1 + i * nw] : 0.0, c->tw[NL], lim(c, ATUS_KFAM_VFUSE),
                                   c->stream));
       for (int a = last - 1; a >= 1; --a) ATUS_TRY(pass(c, a, -1, w, w, 1, 1.0));
       HIP_TRY(launch_sobf(N0, c->rm[ATUS_KFAM_KIN], false, w, w, nullptr, np, 1, c->inner_of(0), c->tw[N0], sa,
-                          c->max_lds, c->stream));
+                          lim(c, ATUS_KFAM_KIN), c->stream));
       for (int a = 1; a <= last; ++a) ATUS_TRY(pass(c, a, +1, w, w, 1, 1.0));
     }
   }
@@ -1679,8 +1704,14 @@ int atus_sob_run(atus_ctx* c, double eps, long long maxit, long long* iters, dou
   return ATUS_OK;
 }
 
-int atus_run_block_profiled(atus_ctx* c, const atus_step_params* sp, int Nk, double* t, double* ms, int* count,
-                            double* grid) {
+}  // extern "C"
+
+namespace {
+
+// runs `block` with every launch bracketed by an event pair on its stream;
+// per kernel family: summed ms, launches and grid share
+template <class Block>
+int profiled(atus_ctx* c, Block block, double* ms, int* count, double* grid) {
   ATUS_TRY(check_ctx(c));
   if (!ms || !count) return fail(ATUS_ERR_ARG, "null output");
   for (int f = 0; f < ATUS_NKFAM; ++f) {
@@ -1692,7 +1723,7 @@ int atus_run_block_profiled(atus_ctx* c, const atus_step_params* sp, int Nk, dou
   c->prof_ev.clear();
   c->prof_fam.clear();
   c->prof_frac.clear();
-  int rc = atus_run_block(c, sp, Nk, t);
+  int rc = block();
   c->prof = false;
   if (rc == ATUS_OK) {
     if (hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(ATUS_ERR_HIP, "synchronize after profiled block");
@@ -1713,6 +1744,20 @@ int atus_run_block_profiled(atus_ctx* c, const atus_step_params* sp, int Nk, dou
   c->prof_fam.clear();
   c->prof_frac.clear();
   return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int atus_run_block_profiled(atus_ctx* c, const atus_step_params* sp, int Nk, double* t, double* ms, int* count,
+                            double* grid) {
+  return profiled(c, [&]() { return atus_run_block(c, sp, Nk, t); }, ms, count, grid);
+}
+
+int atus_batch_run_block_profiled(atus_ctx* c, int kind, double dt, int Nk, double* t, const double* F, double* ms,
+                                  int* count, double* grid) {
+  return profiled(c, [&]() { return atus_batch_run_block(c, kind, dt, Nk, t, F); }, ms, count, grid);
 }
 
 int atus_copy_bandwidth(atus_ctx* c, int reps, double* gbps) {
@@ -1758,6 +1803,15 @@ int atus_tune(atus_ctx* c, int family, int values_per_thread) {
   if (family < 0 || family >= ATUS_NKFAM) return fail(ATUS_ERR_ARG, "unknown kernel family");
   if (values_per_thread != 8 && values_per_thread != 16) return fail(ATUS_ERR_ARG, "values per thread must be 8 or 16");
   c->rm[family] = values_per_thread;
+  return ATUS_OK;
+}
+
+int atus_tune_threads(atus_ctx* c, int family, int threads) {
+  ATUS_TRY(check_ctx(c));
+  if (family < 0 || family >= ATUS_NKFAM) return fail(ATUS_ERR_ARG, "unknown kernel family");
+  if (threads != 0 && threads != 64 && threads != 128 && threads != 256 && threads != 512)
+    return fail(ATUS_ERR_ARG, "threads per workgroup must be 0 (automatic), 64, 128, 256 or 512");
+  c->wg[family] = threads;
   return ATUS_OK;
 }
 
@@ -2040,6 +2094,233 @@ int atus_graph_launch(atus_graph* g, double* t, const double* F) {
   HIP_TRY(hipMemcpyAsync(g->dev, g->host, sizeof(double) * (g->Nk + g->nf), hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipGraphLaunch(g->exec, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
+  *t = tt;
+  return ATUS_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ scan graphs
+
+struct atus_scan_graph {
+  std::vector<atus_ctx*> ctxs;
+  int vk = 0, Nk = 0, nk0 = 0;
+  double dt = 0;
+  int device = 0;
+  hipStream_t master = nullptr;
+  hipEvent_t fork = nullptr;
+  std::vector<hipEvent_t> joins;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  // device tables: t1 [Nk], then per context its amplitudes [Nk][nbatch]
+  double* dev = nullptr;
+  double* host = nullptr;
+  size_t ntab = 0;
+  std::vector<size_t> foff;  // offset of each context's amplitudes in the tables
+  // Rabi integrals: per context nbatch x RED_MAX_NV sums (device, pinned host)
+  double* rdev = nullptr;
+  double* rhost = nullptr;
+  std::vector<size_t> roff;
+  size_t nr = 0;
+  bool counted = false;
+};
+
+namespace {
+
+// compute_rabi_integrals of component 0 of every run of a batch
+// (CRT_Base_IF.h:208-281), transformed in place and back, the sums left in
+// device memory at `res` (nbatch x RED_MAX_NV): the launches of
+// atus_batch_rabi_integrals without its host copy
+int batch_rabi_launch(atus_ctx* c, const double* k0, int n, double threshold, double* res) {
+  const long long cs = (long long)c->nc_run * c->npts;
+  double2* p = c->psi;
+  ATUS_TRY(full_ft(c, -1, p, p, c->nbatch, cs));
+  RedRabi fn{};
+  fn.psik = p;
+  fn.gr = c->grid();
+  fn.nk0 = n;
+  for (int q = 0; q < n; ++q)
+    for (int a = 0; a < 3; ++a) fn.k0[q][a] = k0[3 * q + a];
+  fn.thr = threshold;
+  const unsigned nb = seg_blocks(c->npts, c->nbatch);
+  if (c->sred_cap < (size_t)c->nbatch * nb * RED_MAX_NV) return fail(ATUS_ERR_STATE, "reduction buffers not sized");
+  ATUS_TRY(reduce_seg_launch(c, fn, c->npts, cs, c->nbatch, nb, c->sred_part, res));
+  ATUS_TRY(full_ft(c, +1, p, p, c->nbatch, cs));
+  return ATUS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int atus_scan_graph_destroy(atus_scan_graph* g) {
+  if (!g) return ATUS_OK;
+  (void)hipSetDevice(g->device);
+  if (g->master) (void)hipStreamSynchronize(g->master);
+  if (g->counted)
+    for (atus_ctx* c : g->ctxs) --c->live_graphs;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  (void)hipFree(g->dev);
+  (void)hipFree(g->rdev);
+  if (g->host) (void)hipHostFree(g->host);
+  if (g->rhost) (void)hipHostFree(g->rhost);
+  for (hipEvent_t e : g->joins)
+    if (e) (void)hipEventDestroy(e);
+  if (g->fork) (void)hipEventDestroy(g->fork);
+  if (g->ev0) (void)hipEventDestroy(g->ev0);
+  if (g->ev1) (void)hipEventDestroy(g->ev1);
+  if (g->master) (void)hipStreamDestroy(g->master);
+  delete g;
+  return ATUS_OK;
+}
+
+int atus_scan_graph_create(atus_ctx* const* ctxs, int nctx, int kind, double dt, int Nk, const double* k0, int nk0,
+                           double threshold, atus_scan_graph** out) {
+  if (!out) return fail(ATUS_ERR_ARG, "null output pointer");
+  *out = nullptr;
+  if (!ctxs || nctx < 1) return fail(ATUS_ERR_ARG, "need at least one batch context");
+  if (Nk < 1) return fail(ATUS_ERR_ARG, "Nk must be >= 1");
+  if (nk0 < 0 || nk0 > RED_MAX_NV || (nk0 > 0 && !k0)) return fail(ATUS_ERR_ARG, "need 0..8 momentum points");
+  const int vk = vkind_of(kind);
+  if (vk != V_BRAGG && vk != V_NL_IF) return fail(ATUS_ERR_ARG, "batches run bragg_ad or freeprop (IF) steps");
+  for (int i = 0; i < nctx; ++i) {
+    atus_ctx* c = ctxs[i];
+    ATUS_TRY(check_ctx(c));
+    if (!c->runs || c->nc_run != 2) return fail(ATUS_ERR_STATE, "not a batch context (atus_batch_create)");
+    if (!c->runs_set) return fail(ATUS_ERR_STATE, "run parameters not set (atus_batch_set_runs)");
+    if (c->device != ctxs[0]->device) return fail(ATUS_ERR_ARG, "contexts on different devices");
+    for (int j = 0; j < i; ++j)
+      if (ctxs[j] == c) return fail(ATUS_ERR_ARG, "a context appears twice");
+  }
+  atus_scan_graph* g = new atus_scan_graph();
+  auto bail = [&](int code, const std::string& m) {
+    atus_scan_graph_destroy(g);
+    return fail(code, m);
+  };
+  g->ctxs.assign(ctxs, ctxs + nctx);
+  g->vk = vk;
+  g->Nk = Nk;
+  g->nk0 = nk0;
+  g->dt = dt;
+  g->device = ctxs[0]->device;
+  // tables and buffers, everything the captured launches read, allocated
+  // before the capture
+  g->ntab = (size_t)Nk;
+  for (atus_ctx* c : g->ctxs) {
+    g->foff.push_back(g->ntab);
+    g->ntab += (size_t)Nk * c->nbatch;
+    g->roff.push_back(g->nr);
+    g->nr += (size_t)c->nbatch * RED_MAX_NV;
+    int rc = ensure_kin(c, dt);
+    if (rc == ATUS_OK && c->fs) rc = ensure_tbuf(c);
+    if (rc == ATUS_OK && nk0 > 0)
+      rc = ensure_sred(c, (size_t)c->nbatch * seg_blocks(c->npts, c->nbatch) * RED_MAX_NV, c->nbatch);
+    if (rc != ATUS_OK) {
+      const std::string m = g_err;
+      return bail(rc, m);
+    }
+  }
+  if (hipMalloc(&g->dev, sizeof(double) * g->ntab) != hipSuccess ||
+      hipHostMalloc(&g->host, sizeof(double) * g->ntab, hipHostMallocDefault) != hipSuccess ||
+      hipMalloc(&g->rdev, sizeof(double) * std::max<size_t>(g->nr, 1)) != hipSuccess ||
+      hipHostMalloc(&g->rhost, sizeof(double) * std::max<size_t>(g->nr, 1), hipHostMallocDefault) != hipSuccess)
+    return bail(ATUS_ERR_NOMEM, "scan graph tables");
+  g->joins.assign(nctx, nullptr);
+  if (hipStreamCreateWithFlags(&g->master, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&g->fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreate(&g->ev0) != hipSuccess || hipEventCreate(&g->ev1) != hipSuccess)
+    return bail(ATUS_ERR_HIP, "scan graph stream and events");
+  for (int i = 0; i < nctx; ++i)
+    if (hipEventCreateWithFlags(&g->joins[i], hipEventDisableTiming) != hipSuccess)
+      return bail(ATUS_ERR_HIP, "scan graph events");
+  for (atus_ctx* c : g->ctxs)
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return bail(ATUS_ERR_HIP, "synchronize");
+  // capture: every context's block and Rabi integrals on its own stream,
+  // forked from and joined back into the master stream
+  std::vector<bool> prof(nctx);
+  for (int i = 0; i < nctx; ++i) {
+    prof[i] = g->ctxs[i]->prof;
+    g->ctxs[i]->prof = false;
+  }
+  int rc = ATUS_OK;
+  std::string err;
+  if (hipStreamBeginCapture(g->master, hipStreamCaptureModeRelaxed) != hipSuccess)
+    return bail(ATUS_ERR_HIP, "stream capture");
+  if (hipEventRecord(g->fork, g->master) != hipSuccess) rc = fail(ATUS_ERR_HIP, "fork event");
+  for (int i = 0; rc == ATUS_OK && i < nctx; ++i) {
+    atus_ctx* c = g->ctxs[i];
+    if (hipStreamWaitEvent(c->stream, g->fork, 0) != hipSuccess) {
+      rc = fail(ATUS_ERR_HIP, "fork");
+      break;
+    }
+    VArgs base;
+    batch_vargs(c, base, dt);
+    StepFeed fd;
+    fd.t1dev = g->dev;
+    fd.fbatch = g->dev + g->foff[i];
+    double t = 0.0;
+    rc = batch_block(c, vk, base, Nk, dt, &t, fd);
+    if (rc == ATUS_OK && nk0 > 0) rc = batch_rabi_launch(c, k0, nk0, threshold, g->rdev + g->roff[i]);
+    if (rc == ATUS_OK && (hipEventRecord(g->joins[i], c->stream) != hipSuccess ||
+                          hipStreamWaitEvent(g->master, g->joins[i], 0) != hipSuccess))
+      rc = fail(ATUS_ERR_HIP, "join");
+  }
+  err = g_err;
+  hipGraph_t graph = nullptr;
+  const hipError_t ec = hipStreamEndCapture(g->master, &graph);
+  g->graph = graph;
+  for (int i = 0; i < nctx; ++i) g->ctxs[i]->prof = prof[i];
+  if (rc != ATUS_OK) return bail(rc, "capturing the scan: " + err);
+  if (ec != hipSuccess || !graph) return bail(ATUS_ERR_HIP, "stream capture did not produce a graph");
+  if (hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+    g->exec = nullptr;
+    return bail(ATUS_ERR_HIP, "graph instantiation");
+  }
+  for (atus_ctx* c : g->ctxs) ++c->live_graphs;
+  g->counted = true;
+  *out = g;
+  return ATUS_OK;
+}
+
+int atus_scan_graph_launch(atus_scan_graph* g, double* t, const double* F, double* rabi, double* gpu_ms) {
+  if (!g || !t) return fail(ATUS_ERR_ARG, "null argument");
+  if (hipSetDevice(g->device) != hipSuccess) return fail(ATUS_ERR_HIP, "hipSetDevice failed");
+  for (atus_ctx* c : g->ctxs) ATUS_TRY(ensure_kin(c, g->dt));
+  // t1 of each step with the block runner's bookkeeping, then amplitudes
+  double tt = *t + 0.5 * g->dt;
+  for (int m = 1; m <= g->Nk; ++m) {
+    g->host[m - 1] = tt + 0.5 * g->dt;
+    tt += (m < g->Nk) ? g->dt : 0.5 * g->dt;
+  }
+  for (size_t i = (size_t)g->Nk; i < g->ntab; ++i) {
+    const double f = F ? F[i - g->Nk] : 1.0;
+    if (!std::isfinite(f)) return fail(ATUS_ERR_NUMERIC, "non-finite amplitude in the table");
+    g->host[i] = f;
+  }
+  // the contexts' own pending work first
+  for (atus_ctx* c : g->ctxs) HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipMemcpyAsync(g->dev, g->host, sizeof(double) * g->ntab, hipMemcpyHostToDevice, g->master));
+  HIP_TRY(hipEventRecord(g->ev0, g->master));
+  HIP_TRY(hipGraphLaunch(g->exec, g->master));
+  HIP_TRY(hipEventRecord(g->ev1, g->master));
+  if (g->nk0 > 0)
+    HIP_TRY(hipMemcpyAsync(g->rhost, g->rdev, sizeof(double) * g->nr, hipMemcpyDeviceToHost, g->master));
+  HIP_TRY(hipStreamSynchronize(g->master));
+  if (gpu_ms) {
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, g->ev0, g->ev1));
+    *gpu_ms = ms;
+  }
+  if (rabi && g->nk0 > 0) {
+    size_t o = 0;
+    for (size_t i = 0; i < g->ctxs.size(); ++i) {
+      const atus_ctx* c = g->ctxs[i];
+      for (int r = 0; r < c->nbatch; ++r)
+        for (int q = 0; q < g->nk0; ++q) rabi[o++] = c->ar_k() * g->rhost[g->roff[i] + (size_t)r * RED_MAX_NV + q];
+    }
+  }
   *t = tt;
   return ATUS_OK;
 }

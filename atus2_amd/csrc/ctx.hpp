@@ -92,6 +92,7 @@ struct atus_ctx {
   double* sred_part = nullptr;
   double* sred_out = nullptr;
   size_t sred_cap = 0;
+  int sred_nseg = 0;
 
   bool sob_ready = false;
   double sob_alpha[2] = {0, 0};
@@ -111,6 +112,9 @@ struct atus_ctx {
   // defaults: what the autotuner picks at 256^3 (8 for the strided passes
   // and for the register-heavy V pass, which spills at 16)
   int rm[ATUS_NKFAM] = {16, 8, 16, 8, 16};
+  // threads per workgroup at most, per kernel family (0: as many as the
+  // kernel's launch bounds allow); sets the pencils per workgroup
+  int wg[ATUS_NKFAM] = {0, 0, 0, 0, 0};
   // 3D blocks: the y and z passes between two kinetic steps run slab by
   // slab of this many x-planes, alternate slabs on two streams (0: over the
   // whole grid each; -1: chosen from the size of the device's last-level

@@ -7,7 +7,7 @@ namespace atus {
 
 template <int N, int RM, int VK>
 static hipError_t kinb_go(double2* psi, long long cs, long long nbatch, const double2* tw, const KinTabs& kt,
-                          const VArgs& va, double dx0, size_t max_lds, hipStream_t s) {
+                          const VArgs& va, double dx0, const Limits& max_lds, hipStream_t s) {
   PassCfg c = make_pass_cfg(N, RM, 1, nbatch, 1, max_lds, 2);
   if (c.block.x == 0 || c.block.x > 256 || c.grid.x == 0) return hipErrorInvalidConfiguration;
   static bool ok = false;
@@ -19,7 +19,7 @@ static hipError_t kinb_go(double2* psi, long long cs, long long nbatch, const do
 
 template <int N, int RM, int VK>
 static hipError_t vfb_go(double2* psi, long long cs, long long outer, const double2* tw, const VArgs& va,
-                         long long ppx, int n0, double dx0, size_t max_lds, hipStream_t s) {
+                         long long ppx, int n0, double dx0, const Limits& max_lds, hipStream_t s) {
   PassCfg c = make_pass_cfg(N, RM, 1, outer, 1, max_lds);
   if (c.block.x == 0 || c.block.x > 256 || c.grid.x == 0) return hipErrorInvalidConfiguration;
   static bool ok = false;
@@ -31,7 +31,7 @@ static hipError_t vfb_go(double2* psi, long long cs, long long outer, const doub
 
 template <int N, int RM>
 static hipError_t batch_nr(bool one_d, int vk, double2* psi, long long cs, long long count, const double2* tw,
-                           const KinTabs& kt, const VArgs& va, long long ppx, int n0, double dx0, size_t max_lds,
+                           const KinTabs& kt, const VArgs& va, long long ppx, int n0, double dx0, const Limits& max_lds,
                            hipStream_t s) {
   if (one_d) {
     if (vk == V_BRAGG) return kinb_go<N, RM, V_BRAGG>(psi, cs, count, tw, kt, va, dx0, max_lds, s);
@@ -45,7 +45,7 @@ static hipError_t batch_nr(bool one_d, int vk, double2* psi, long long cs, long 
 
 template <int N>
 static hipError_t batch_n(int rm, bool one_d, int vk, double2* psi, long long cs, long long count, const double2* tw,
-                          const KinTabs& kt, const VArgs& va, long long ppx, int n0, double dx0, size_t max_lds,
+                          const KinTabs& kt, const VArgs& va, long long ppx, int n0, double dx0, const Limits& max_lds,
                           hipStream_t s) {
   if (rm == 8 && N > 8 && N / 8 <= 256)
     return batch_nr<N, 8>(one_d, vk, psi, cs, count, tw, kt, va, ppx, n0, dx0, max_lds, s);
@@ -54,7 +54,7 @@ static hipError_t batch_n(int rm, bool one_d, int vk, double2* psi, long long cs
 
 hipError_t launch_vstep_batch(int N, int rm, bool one_d, int vk, double2* psi, long long comp_stride, long long count,
                               const double2* tw, const KinTabs& kt, const VArgs& va, long long pencils_per_x0, int n0,
-                              double dx0, size_t max_lds, hipStream_t s) {
+                              double dx0, const Limits& max_lds, hipStream_t s) {
   switch (N) {
 #define ATUS_CASE(n) \
   case n: return batch_n<n>(rm, one_d, vk, psi, comp_stride, count, tw, kt, va, pencils_per_x0, n0, dx0, max_lds, s);

@@ -6,7 +6,7 @@ namespace atus {
 template <int N, int RM, int VK, int NC, bool BATCH>
 static hipError_t block_go(double2* psi, long long cs, long long runs, const double2* tw, const double2* eh,
                            const double2* ef, const VArgs& va, const double* t1tab, const double* Ftab, double dx0,
-                           int Nk, size_t max_lds, hipStream_t s) {
+                           int Nk, const Limits& max_lds, hipStream_t s) {
   // pencils of more than 256 threads (4096 points at 8 values per thread)
   // take workgroups of 512
   constexpr int LB = Plan<N, RM>::P > 256 ? 512 : 256;
@@ -26,7 +26,7 @@ static hipError_t block_go(double2* psi, long long cs, long long runs, const dou
 template <int N, int RM>
 static hipError_t block_nr(int vk, int nc, bool batch, double2* psi, long long cs, long long runs,
                            const double2* tw, const double2* eh, const double2* ef, const VArgs& va,
-                           const double* t1tab, const double* Ftab, double dx0, int Nk, size_t max_lds,
+                           const double* t1tab, const double* Ftab, double dx0, int Nk, const Limits& max_lds,
                            hipStream_t s) {
   if (batch) {
     if (nc != 2) return hipErrorInvalidValue;
@@ -55,7 +55,7 @@ static hipError_t block_nr(int vk, int nc, bool batch, double2* psi, long long c
 template <int N>
 static hipError_t block_n(int rm, int vk, int nc, bool batch, double2* psi, long long cs, long long runs,
                           const double2* tw, const double2* eh, const double2* ef, const VArgs& va,
-                          const double* t1tab, const double* Ftab, double dx0, int Nk, size_t max_lds,
+                          const double* t1tab, const double* Ftab, double dx0, int Nk, const Limits& max_lds,
                           hipStream_t s) {
   if constexpr (N > 8 && N / 8 <= 512) {
     if (rm == 8) return block_nr<N, 8>(vk, nc, batch, psi, cs, runs, tw, eh, ef, va, t1tab, Ftab, dx0, Nk, max_lds, s);
@@ -65,7 +65,7 @@ static hipError_t block_n(int rm, int vk, int nc, bool batch, double2* psi, long
 
 hipError_t launch_block1d(int N, int rm, int vk, int nc, bool batch, double2* psi, long long comp_stride,
                           long long runs, const double2* tw, const double2* eh, const double2* ef, const VArgs& va,
-                          const double* t1tab, const double* Ftab, double dx0, int Nk, size_t max_lds,
+                          const double* t1tab, const double* Ftab, double dx0, int Nk, const Limits& max_lds,
                           hipStream_t s) {
   switch (N) {
 #define ATUS_CASE(n)                                                                                              \

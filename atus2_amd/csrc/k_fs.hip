@@ -6,7 +6,7 @@ namespace atus {
 
 template <int N1, int RM, int MODE, int VK, int NC, bool BATCH>
 static hipError_t fs_go(double2* psi, long long cs, int ycount, long long n2, const double2* tw1,
-                        const double2* wN, const VArgs& va, double dx, double scale, size_t max_lds,
+                        const double2* wN, const VArgs& va, double dx, double scale, const Limits& max_lds,
                         hipStream_t s) {
   PassCfg c = make_pass_cfg(N1, RM, n2, 1, ycount, max_lds);
   if (c.block.x == 0 || c.block.x > 256 || c.grid.x == 0) return hipErrorInvalidConfiguration;
@@ -21,7 +21,7 @@ static hipError_t fs_go(double2* psi, long long cs, int ycount, long long n2, co
 template <int N1, int RM>
 static hipError_t fs_nr(int mode, int vk, int nc, bool batch, double2* psi, long long cs, int ycount, long long n2,
                         const double2* tw1, const double2* wN, const VArgs& va, double dx, double scale,
-                        size_t max_lds, hipStream_t s) {
+                        const Limits& max_lds, hipStream_t s) {
 #define ATUS_FS(M, V, C, B) fs_go<N1, RM, M, V, C, B>(psi, cs, ycount, n2, tw1, wN, va, dx, scale, max_lds, s)
   if (mode == FS_ENTER) return ATUS_FS(FS_ENTER, V_NONE, 1, false);
   if (mode == FS_EXIT) return ATUS_FS(FS_EXIT, V_NONE, 1, false);
@@ -47,14 +47,14 @@ static hipError_t fs_nr(int mode, int vk, int nc, bool batch, double2* psi, long
 template <int N1>
 static hipError_t fs_n(int rm, int mode, int vk, int nc, bool batch, double2* psi, long long cs, int ycount,
                        long long n2, const double2* tw1, const double2* wN, const VArgs& va, double dx, double scale,
-                       size_t max_lds, hipStream_t s) {
+                       const Limits& max_lds, hipStream_t s) {
   if (rm == 8) return fs_nr<N1, 8>(mode, vk, nc, batch, psi, cs, ycount, n2, tw1, wN, va, dx, scale, max_lds, s);
   return fs_nr<N1, 16>(mode, vk, nc, batch, psi, cs, ycount, n2, tw1, wN, va, dx, scale, max_lds, s);
 }
 
 hipError_t launch_fs_col(int n1, int rm, int mode, int vk, int nc, bool batch, double2* psi, long long comp_stride,
                          int ycount, long long n2, const double2* tw1, const double2* wN, const VArgs& va, double dx,
-                         double scale, size_t max_lds, hipStream_t s) {
+                         double scale, const Limits& max_lds, hipStream_t s) {
   switch (n1) {
 #define ATUS_CASE(n) \
   case n: return fs_n<n>(rm, mode, vk, nc, batch, psi, comp_stride, ycount, n2, tw1, wN, va, dx, scale, max_lds, s);

@@ -6,7 +6,7 @@ namespace atus {
 // LB: threads per workgroup at most (the kernel's launch bounds)
 template <int N, int RM, bool CONTIG, int VK, int NC, int LB = 256>
 static hipError_t kin_go(double2* psi, long long cs, int ycount, long long inner, long long outer,
-                         const double2* tw, const KinTabs& kt, const VArgs& va, double dx0, size_t max_lds,
+                         const double2* tw, const KinTabs& kt, const VArgs& va, double dx0, const Limits& max_lds,
                          hipStream_t s) {
   PassCfg c = make_pass_cfg(N, RM, inner, outer, ycount, max_lds, 2, LB);
   if (c.block.x == 0 || c.block.x > LB || c.grid.x == 0) return hipErrorInvalidConfiguration;
@@ -20,7 +20,7 @@ static hipError_t kin_go(double2* psi, long long cs, int ycount, long long inner
 
 template <int N, int RM>
 static hipError_t kin_nr(bool contig, int vk, int nc, double2* psi, long long cs, int ncomp, long long inner,
-                        const double2* tw, const KinTabs& kt, const VArgs& va, double dx0, size_t max_lds,
+                        const double2* tw, const KinTabs& kt, const VArgs& va, double dx0, const Limits& max_lds,
                         hipStream_t s, long long rows) {
   if (!contig) {
     // 2D/3D: axis 0 is strided, one component per blockIdx.y, no V prologue
@@ -46,7 +46,7 @@ static hipError_t kin_nr(bool contig, int vk, int nc, double2* psi, long long cs
 
 template <int N>
 static hipError_t kin_n(int rm, bool contig, int vk, int nc, double2* psi, long long cs, int ncomp, long long inner,
-                        const double2* tw, const KinTabs& kt, const VArgs& va, double dx0, size_t max_lds,
+                        const double2* tw, const KinTabs& kt, const VArgs& va, double dx0, const Limits& max_lds,
                         hipStream_t s, long long rows) {
   if (rm == 8 && N > 8 && N / 8 <= 256)
     return kin_nr<N, 8>(contig, vk, nc, psi, cs, ncomp, inner, tw, kt, va, dx0, max_lds, s, rows);
@@ -55,7 +55,7 @@ static hipError_t kin_n(int rm, bool contig, int vk, int nc, double2* psi, long 
 
 hipError_t launch_kin(int N, int rm, bool contig, int vk, int nc, double2* psi, long long comp_stride, int ncomp,
                       long long inner, const double2* tw, const KinTabs& kt, const VArgs& va, double dx0,
-                      size_t max_lds, hipStream_t s, long long rows) {
+                      const Limits& max_lds, hipStream_t s, long long rows) {
   switch (N) {
 #define ATUS_CASE(n) \
   case n: return kin_n<n>(rm, contig, vk, nc, psi, comp_stride, ncomp, inner, tw, kt, va, dx0, max_lds, s, rows);

@@ -41,7 +41,7 @@ PassCfg make_pass_cfg(int N, int rm, long long inner, long long outer, int ycoun
 
 template <int N, int RM>
 static hipError_t pass_nr(bool contig, int dir, const double2* src, double2* dst, long long cs, int ncomp,
-                          long long inner, long long outer, const double2* tw, double scale, size_t max_lds,
+                          long long inner, long long outer, const double2* tw, double scale, const Limits& max_lds,
                           hipStream_t s) {
   PassCfg c = make_pass_cfg(N, RM, inner, outer, ncomp, max_lds);
   if (c.block.x == 0 || c.block.x > 256 || c.grid.x == 0) return hipErrorInvalidConfiguration;
@@ -74,7 +74,7 @@ static hipError_t pass_nr(bool contig, int dir, const double2* src, double2* dst
 
 template <int N>
 static hipError_t pass_n(int rm, bool contig, int dir, const double2* src, double2* dst, long long cs, int ncomp,
-                         long long inner, long long outer, const double2* tw, double scale, size_t max_lds,
+                         long long inner, long long outer, const double2* tw, double scale, const Limits& max_lds,
                          hipStream_t s) {
   if (rm == 8 && N > 8 && N / 8 <= 256) return pass_nr<N, 8>(contig, dir, src, dst, cs, ncomp, inner, outer, tw, scale, max_lds, s);
   return pass_nr<N, 16>(contig, dir, src, dst, cs, ncomp, inner, outer, tw, scale, max_lds, s);
@@ -82,7 +82,7 @@ static hipError_t pass_n(int rm, bool contig, int dir, const double2* src, doubl
 
 hipError_t launch_pass(int N, int rm, bool contig, int dir, const double2* src, double2* dst, long long comp_stride,
                        int ncomp, long long inner, long long outer, const double2* tw, double scale,
-                       size_t max_lds, hipStream_t s) {
+                       const Limits& max_lds, hipStream_t s) {
   switch (N) {
 #define ATUS_CASE(n) \
   case n: return pass_n<n>(rm, contig, dir, src, dst, comp_stride, ncomp, inner, outer, tw, scale, max_lds, s);

@@ -5,7 +5,7 @@ namespace atus {
 
 template <int N, int RM>
 static hipError_t sob_nr(bool contig, const double2* src, double2* o1, double2* o2, long long cs, int ncomp,
-                         long long inner, const double2* tw, const SobArgs& sa, size_t max_lds, hipStream_t s) {
+                         long long inner, const double2* tw, const SobArgs& sa, const Limits& max_lds, hipStream_t s) {
   PassCfg c = make_pass_cfg(N, RM, contig ? 1 : inner, 1, ncomp, max_lds);
   if (c.block.x == 0 || c.block.x > 256 || c.grid.x == 0) return hipErrorInvalidConfiguration;
   static bool ok[2] = {false, false};
@@ -25,14 +25,14 @@ static hipError_t sob_nr(bool contig, const double2* src, double2* o1, double2* 
 
 template <int N>
 static hipError_t sob_n(int rm, bool contig, const double2* src, double2* o1, double2* o2, long long cs, int ncomp,
-                        long long inner, const double2* tw, const SobArgs& sa, size_t max_lds, hipStream_t s) {
+                        long long inner, const double2* tw, const SobArgs& sa, const Limits& max_lds, hipStream_t s) {
   if (rm == 8 && N > 8 && N / 8 <= 256) return sob_nr<N, 8>(contig, src, o1, o2, cs, ncomp, inner, tw, sa, max_lds, s);
   return sob_nr<N, 16>(contig, src, o1, o2, cs, ncomp, inner, tw, sa, max_lds, s);
 }
 
 hipError_t launch_sobf(int N, int rm, bool contig, const double2* src, double2* out_sob, double2* out_lap,
                        long long comp_stride, int ncomp, long long inner, const double2* tw, const SobArgs& sa,
-                       size_t max_lds, hipStream_t s) {
+                       const Limits& max_lds, hipStream_t s) {
   switch (N) {
 #define ATUS_CASE(n) \
   case n: return sob_n<n>(rm, contig, src, out_sob, out_lap, comp_stride, ncomp, inner, tw, sa, max_lds, s);
@@ -45,7 +45,7 @@ hipError_t launch_sobf(int N, int rm, bool contig, const double2* src, double2* 
 
 template <int N, int RM, int NW>
 static hipError_t gf_go(const double2* lap, const double2* psi, const double* pot, double2* out, long long npts,
-                        int comp, double g0, double g1, const double2* tw, size_t max_lds, hipStream_t s) {
+                        int comp, double g0, double g1, const double2* tw, const Limits& max_lds, hipStream_t s) {
   PassCfg c = make_pass_cfg(N, RM, 1, npts / N, 1, max_lds);
   if (c.block.x == 0 || c.block.x > 256 || c.grid.x == 0) return hipErrorInvalidConfiguration;
   static bool ok = false;
@@ -58,7 +58,7 @@ static hipError_t gf_go(const double2* lap, const double2* psi, const double* po
 
 template <int N>
 static hipError_t gf_n(int rm, int nw, const double2* lap, const double2* psi, const double* pot, double2* out,
-                       long long npts, int comp, double g0, double g1, const double2* tw, size_t max_lds,
+                       long long npts, int comp, double g0, double g1, const double2* tw, const Limits& max_lds,
                        hipStream_t s) {
   const bool r8 = rm == 8 && N > 8 && N / 8 <= 256;
   if (nw == 1)
@@ -72,7 +72,7 @@ static hipError_t gf_n(int rm, int nw, const double2* lap, const double2* psi, c
 
 hipError_t launch_sob_gradfuse(int N, int rm, int nw, const double2* lap, const double2* psi, const double* pot,
                                double2* out, long long npts, int comp, double g0, double g1, const double2* tw,
-                               size_t max_lds, hipStream_t s) {
+                               const Limits& max_lds, hipStream_t s) {
   switch (N) {
 #define ATUS_CASE(n) \
   case n: return gf_n<n>(rm, nw, lap, psi, pot, out, npts, comp, g0, g1, tw, max_lds, s);

@@ -18,11 +18,30 @@ struct PassCfg {
 // values per thread R = min(rm, N); threads per pencil P = N / R
 inline int threads_per_pencil(int N, int rm) { return N / (N < rm ? N : rm); }
 
+// what a launch may use: LDS bytes per workgroup, and threads per
+// workgroup at most (0: as many as the kernel's launch bounds allow; the
+// pencils per workgroup follow as threads / P, at least one pencil)
+struct Limits {
+  size_t lds;
+  int threads;
+};
+
 // contiguous pencils if inner == 1; the pass covers `outer` slabs of
 // [N][inner]; ycount = gridDim.y (components)
 // ntables: lookup tables of length N the kernel stages in LDS (N <= 1024)
 PassCfg make_pass_cfg(int N, int rm, long long inner, long long outer, int ycount, size_t max_lds,
                       int ntables = 1, int max_threads = 256);
+// the same within Limits (max_threads: the kernel's launch bounds)
+inline PassCfg make_pass_cfg(int N, int rm, long long inner, long long outer, int ycount, const Limits& lim,
+                             int ntables = 1, int max_threads = 256) {
+  int t = max_threads;
+  if (lim.threads > 0) {
+    const int P = threads_per_pencil(N, rm);
+    t = lim.threads < max_threads ? lim.threads : max_threads;
+    if (t < P) t = P < max_threads ? P : max_threads;
+  }
+  return make_pass_cfg(N, rm, inner, outer, ycount, lim.lds, ntables, t);
+}
 
 bool supported_len(int N);  // power of two in [2, 4096]
 
@@ -40,17 +59,17 @@ inline hipError_t allow_lds(K kernel, size_t bytes, bool& done) {
 // rm: values per thread (8 or 16) of the FFT kernels
 hipError_t launch_pass(int N, int rm, bool contig, int dir, const double2* src, double2* dst, long long comp_stride,
                        int ncomp, long long inner, long long outer, const double2* tw, double scale,
-                       size_t max_lds, hipStream_t s);
+                       const Limits& max_lds, hipStream_t s);
 
 // kinetic step along axis 0; vk/nc select the fused V prologue (1D only)
 hipError_t launch_kin(int N, int rm, bool contig, int vk, int nc, double2* psi, long long comp_stride, int ncomp,
                       long long inner, const double2* tw, const KinTabs& kt, const VArgs& va, double dx0,
-                      size_t max_lds, hipStream_t s, long long rows = 1);
+                      const Limits& max_lds, hipStream_t s, long long rows = 1);
 
 // inverse / V / forward along the last (contiguous) axis of a 2D/3D grid
 hipError_t launch_vfuse(int N, int rm, int vk, int nc, double2* psi, long long comp_stride, long long outer,
                         const double2* tw, const VArgs& va, long long pencils_per_x0, int n0, double dx0,
-                        size_t max_lds, hipStream_t s);
+                        const Limits& max_lds, hipStream_t s);
 
 // V step of a batch of independent two-component runs (VArgs::runs, ::Fm,
 // ::pencils_per_run, ::run_shift set).  one_d: the fused 1D step over
@@ -59,7 +78,7 @@ hipError_t launch_vfuse(int N, int rm, int vk, int nc, double2* psi, long long c
 // pencils of all runs.
 hipError_t launch_vstep_batch(int N, int rm, bool one_d, int vk, double2* psi, long long comp_stride, long long count,
                               const double2* tw, const KinTabs& kt, const VArgs& va, long long pencils_per_x0, int n0,
-                              double dx0, size_t max_lds, hipStream_t s);
+                              double dx0, const Limits& max_lds, hipStream_t s);
 
 // a whole block of a 1D grid (N <= 4096) in one launch: one run (nc
 // components, runs = 1) or a batch of `runs` two-component runs; eh / ef
@@ -67,7 +86,7 @@ hipError_t launch_vstep_batch(int N, int rm, bool one_d, int vk, double2* psi, l
 // [Nk][runs]) the per-step t1 and amplitudes in device memory
 hipError_t launch_block1d(int N, int rm, int vk, int nc, bool batch, double2* psi, long long comp_stride,
                           long long runs, const double2* tw, const double2* eh, const double2* ef, const VArgs& va,
-                          const double* t1tab, const double* Ftab, double dx0, int Nk, size_t max_lds,
+                          const double* t1tab, const double* Ftab, double dx0, int Nk, const Limits& max_lds,
                           hipStream_t s);
 
 // column pass of the four-step 1D transform (k_fs_col): n1 in 64..1024,
@@ -75,19 +94,19 @@ hipError_t launch_block1d(int N, int rm, int vk, int nc, bool batch, double2* ps
 // batch), runs (batch) or 1
 hipError_t launch_fs_col(int n1, int rm, int mode, int vk, int nc, bool batch, double2* psi, long long comp_stride,
                          int ycount, long long n2, const double2* tw1, const double2* wN, const VArgs& va, double dx,
-                         double scale, size_t max_lds, hipStream_t s);
+                         double scale, const Limits& max_lds, hipStream_t s);
 
 // Sobolev filters along axis 0
 hipError_t launch_sobf(int N, int rm, bool contig, const double2* src, double2* out_sob, double2* out_lap,
                        long long comp_stride, int ncomp, long long inner, const double2* tw, const SobArgs& sa,
-                       size_t max_lds, hipStream_t s);
+                       const Limits& max_lds, hipStream_t s);
 
 // the Sobolev iteration's L2 gradient of component `comp` fused into the
 // last-axis inverse (Laplacian) and forward (gradient filter) passes of a
 // 2D/3D grid of nw (1 or 2) wave functions
 hipError_t launch_sob_gradfuse(int N, int rm, int nw, const double2* lap, const double2* psi, const double* pot,
                                double2* out, long long npts, int comp, double g0, double g1, const double2* tw,
-                               size_t max_lds, hipStream_t s);
+                               const Limits& max_lds, hipStream_t s);
 
 // pointwise V step on all points (unfused path)
 hipError_t launch_point(int vk, int nc, double2* psi, const Grid& gr, const VArgs& va, hipStream_t s);

@@ -24,10 +24,12 @@ EXPORTED = [
     "atus_set_alpha", "atus_ft", "atus_kinetic", "atus_nl_step", "atus_nl_if_step", "atus_bragg_ad",
     "atus_set_momentum", "atus_run_block", "atus_particle_number", "atus_expval_x", "atus_expval_k",
     "atus_rabi_integrals", "atus_energy", "atus_sob_setup", "atus_sob_renormalize", "atus_sob_iter",
-    "atus_sob_mu", "atus_sob_iterate", "atus_set_timing", "atus_last_block_ms", "atus_run_block_profiled", "atus_copy_bandwidth", "atus_tune",
+    "atus_sob_mu", "atus_sob_iterate", "atus_set_timing", "atus_last_block_ms", "atus_run_block_profiled", "atus_copy_bandwidth", "atus_tune", "atus_tune_threads",
     "atus_batch_create", "atus_batch_set_runs", "atus_batch_run_block", "atus_batch_particle_numbers",
+    "atus_batch_run_block_profiled",
     "atus_batch_particle_numbers_noalloc",
     "atus_batch_rabi_integrals", "atus_sob_run", "atus_graph_create", "atus_graph_launch", "atus_graph_destroy",
+    "atus_scan_graph_create", "atus_scan_graph_launch", "atus_scan_graph_destroy",
     "atus_set_alpha2", "atus_run_block_ds", "atus_v_step_ds",
     "atus_snapshot_save", "atus_snapshot_restore", "atus_copy_components", "atus_v_step",
     "atus_get_moments", "atus_inflate", "atus_set_slab", "atus_set_resident",
@@ -119,9 +121,12 @@ def load():
                                         C.POINTER(I), C.POINTER(D)]),
         "atus_copy_bandwidth": (I, [P, I, C.POINTER(D)]),
         "atus_tune": (I, [P, I, I]),
+        "atus_tune_threads": (I, [P, I, I]),
         "atus_batch_create": (I, [pctx, I, C.POINTER(C.c_longlong), C.POINTER(D), C.POINTER(D), I, I]),
         "atus_batch_set_runs": (I, [P, C.POINTER(BatchRun), I]),
         "atus_batch_run_block": (I, [P, I, D, I, C.POINTER(D), P]),
+        "atus_batch_run_block_profiled": (I, [P, I, D, I, C.POINTER(D), P, C.POINTER(D), C.POINTER(I),
+                                              C.POINTER(D)]),
         "atus_batch_particle_numbers": (I, [P, P]),
         "atus_batch_particle_numbers_noalloc": (I, [P, P]),
         "atus_batch_rabi_integrals": (I, [P, I, P, I, D, P]),
@@ -129,6 +134,9 @@ def load():
         "atus_graph_create": (I, [P, I, D, C.POINTER(Laser), I, pctx]),
         "atus_graph_launch": (I, [P, C.POINTER(D), P]),
         "atus_graph_destroy": (I, [P]),
+        "atus_scan_graph_create": (I, [P, I, I, D, I, P, I, D, pctx]),
+        "atus_scan_graph_launch": (I, [P, C.POINTER(D), P, P, C.POINTER(D)]),
+        "atus_scan_graph_destroy": (I, [P]),
         "atus_set_alpha2": (I, [P, C.POINTER(D)]),
         "atus_run_block_ds": (I, [P, C.POINTER(StepParams), C.POINTER(DsParams), I, C.POINTER(D)]),
         "atus_v_step_ds": (I, [P, I, D, D, C.POINTER(Laser), C.POINTER(DsParams)]),
@@ -451,6 +459,11 @@ class Context:
         f = KFAM.index(family) if isinstance(family, str) else int(family)
         _check(_lib.atus_tune(self._h, f, int(values_per_thread)))
 
+    def tune_threads(self, family, threads):
+        """Threads per workgroup at most for a kernel family (0: automatic)."""
+        f = KFAM.index(family) if isinstance(family, str) else int(family)
+        _check(_lib.atus_tune_threads(self._h, f, int(threads)))
+
     def set_slab(self, planes):
         """3D blocks: y/z passes per slab of `planes` x-planes (-1 automatic,
         0 whole grid)."""
@@ -532,6 +545,20 @@ class BatchContext(Context):
                 raise ValueError(f"amplitude table shape {a.shape} != {(Nk, self.nbatch)}")
             _check(_lib.atus_batch_run_block(self._h, kind, dt, Nk, C.byref(tt), _dp(a)))
         return tt.value
+
+    def run_batch_block_profiled(self, kind, dt, Nk, t, F=None):
+        """One block with per-family device times: (t, {family: (ms,
+        launches, grid share)})."""
+        tt = C.c_double(t)
+        ms = (C.c_double * len(KFAM))()
+        cnt = (C.c_int * len(KFAM))()
+        grid = (C.c_double * len(KFAM))()
+        a = None if F is None else np.ascontiguousarray(F, dtype=np.float64)
+        if a is not None and a.shape != (Nk, self.nbatch):
+            raise ValueError(f"amplitude table shape {a.shape} != {(Nk, self.nbatch)}")
+        _check(_lib.atus_batch_run_block_profiled(self._h, int(kind), float(dt), int(Nk), C.byref(tt),
+                                                  None if a is None else _dp(a), ms, cnt, grid))
+        return tt.value, {KFAM[i]: (ms[i], cnt[i], grid[i]) for i in range(len(KFAM))}
 
     def particle_numbers(self):
         out = np.empty((self.nbatch, 2))
@@ -663,6 +690,78 @@ class Graph:
             if ctx is not None:
                 ctx._graphs.discard(self)
                 self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class ScanGraph:
+    """The blocks and Rabi integrals of several BatchContexts (runs of
+    different grids) captured as one HIP graph (atus_scan_graph_*)."""
+
+    def __init__(self, ctxs, kind, dt, Nk, k0list=(), threshold=0.0):
+        L = load()
+        self.ctxs = list(ctxs)
+        arr = (C.c_void_p * len(self.ctxs))(*[c._h.value for c in self.ctxs])
+        k0 = np.zeros((max(1, len(k0list)), 3))
+        for i, k in enumerate(k0list):
+            k0[i, :len(k)] = k
+        k0 = np.ascontiguousarray(k0)
+        h = C.c_void_p()
+        _check(L.atus_scan_graph_create(C.cast(arr, C.c_void_p), len(self.ctxs), int(kind), float(dt), int(Nk),
+                                        _dp(k0), len(k0list), float(threshold), C.byref(h)))
+        self._h = h
+        self.Nk = int(Nk)
+        self.nk0 = len(k0list)
+        for c in self.ctxs:
+            c._graphs.add(self)
+
+    def launch(self, t, F=None):
+        """F: a list of (Nk, nbatch) amplitude tables, one per context, or
+        None.  Returns (t, [Rabi populations (nbatch, nk0) per context],
+        device ms of the graph)."""
+        if not getattr(self, "_h", None):
+            raise AtusError("scan graph is closed")
+        tt = C.c_double(t)
+        ms = C.c_double()
+        fa = None
+        if F is not None:
+            parts = []
+            for c, f in zip(self.ctxs, F):
+                a = np.asarray(f, dtype=np.float64)
+                if a.shape != (self.Nk, c.nbatch):
+                    raise ValueError(f"amplitude table shape {a.shape} != {(self.Nk, c.nbatch)}")
+                parts.append(a.ravel())
+            if len(parts) != len(self.ctxs):
+                raise ValueError("one amplitude table per context")
+            fa = np.ascontiguousarray(np.concatenate(parts))
+        nr = sum(c.nbatch for c in self.ctxs) * self.nk0
+        rabi = np.zeros(max(nr, 1))
+        _check(_lib.atus_scan_graph_launch(self._h, C.byref(tt), None if fa is None else _dp(fa), _dp(rabi),
+                                           C.byref(ms)))
+        out, o = [], 0
+        for c in self.ctxs:
+            n = c.nbatch * self.nk0
+            out.append(rabi[o:o + n].reshape(c.nbatch, self.nk0))
+            o += n
+        return tt.value, out, ms.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.atus_scan_graph_destroy(self._h)
+            self._h = None
+            for c in self.ctxs:
+                c._graphs.discard(self)
+            self.ctxs = []
 
     def __del__(self):
         try:

@@ -8,10 +8,15 @@ bragg_ad sequence of bragg.cpp with the Rabi populations of component 0
 after every block (CRT_Base_IF.h:208-314).  Here the runs of one grid shape
 share a batch context (atus_batch_*), so that every launch advances all of
 them, and the amplitudes of a block are evaluated for all runs at once.
+The blocks of all shapes, with their Rabi populations, are captured as one
+HIP graph (atus_scan_graph_*): each shape's launches on its own stream, so
+the device runs the small 1D groups beside the large 2D ones, and a block
+of the whole scan is one graph launch.  Each shape's kernels take the
+autotuner's values per thread and threads per workgroup.
 
-`python -m atus2_amd.scan` times the 512-run scan both ways (batched, and
-the same runs one after another through the single-run path) and prints
-one JSON line.
+`python -m atus2_amd.scan` times the 512-run scan three ways (the graph,
+the shape groups one after another with eager launches, and every run
+one after another through the single-run path) and prints one JSON line.
 """
 from __future__ import annotations
 
@@ -132,6 +137,79 @@ def run_batched(runs, dt=2e-4, Nk=100, steps=3000, max_batch=4096):
     return out, elapsed
 
 
+def make_contexts(runs, dt=2e-4, Nk=100, tune=True, log=None):
+    """One batch context per grid shape, loaded with its runs' initial
+    states; with tune=True each takes the autotuner's choice (values per
+    thread and threads per workgroup per kernel family), measured on a
+    scratch context of the same runs.  Returns [(shape, run indices, ctx)]."""
+    from .autotune import _apply, autotune
+    groups = {}
+    for i, r in enumerate(runs):
+        groups.setdefault(r.shape, []).append(i)
+    out = []
+    for shape, idx in sorted(groups.items()):
+        rs = [runs[i] for i in idx]
+        dx, dk = grid_spacings(shape, rs[0].L)
+
+        def load(B):
+            for j, r in enumerate(rs):
+                p1, p2 = r.initial_state()
+                B.upload(2 * j, p1)
+                B.upload(2 * j + 1, p2)
+            B.set_alpha([0.5] * len(shape))
+            B.set_runs([r.gs for r in rs], [r.laser() for r in rs])
+
+        B = hip.BatchContext(shape, dx, dk, len(rs))
+        load(B)
+        if tune:
+            with hip.BatchContext(shape, dx, dk, len(rs)) as T:
+                load(T)
+                _, choice = autotune(T, hip.STEP_BRAGG_AD, dt, 0.0, Nk=10, reps=2)
+            _apply(B, choice)
+            if log is not None:
+                log(f"{shape} x {len(rs)}: {choice}")
+        out.append((shape, idx, B))
+    return out
+
+
+def run_graph(runs, dt=2e-4, Nk=100, steps=3000, tune=True, log=None):
+    """All runs at once: one batch context per grid shape, and a block of
+    every context (with its Rabi populations) captured as one HIP graph
+    whose branches the device runs side by side (atus_scan_graph_*).
+    Returns ({run index: Rabi populations per block}, seconds of the block
+    loop, seconds of device time inside the graph launches)."""
+    groups = make_contexts(runs, dt, Nk, tune, log)
+    ctxs = [B for _, _, B in groups]
+    series = [[] for _ in groups]
+    try:
+        k0 = runs[0].rabi_k0()
+        sig = [np.array([runs[i].sigma for i in idx]) for _, idx, _ in groups]
+        t0s = [np.array([runs[i].t0 for i in idx]) for _, idx, _ in groups]
+        with hip.ScanGraph(ctxs, hip.STEP_BRAGG_AD, dt, Nk, k0, RABI_THRESHOLD) as G:
+            for B in ctxs:
+                B.synchronize()
+            gpu = 0.0
+            tic = time.perf_counter()
+            t = 0.0
+            for _ in range(steps // Nk):
+                t1 = np.array(hip.BatchContext.step_times(t, dt, Nk))[:, None]
+                F = [np.exp(-0.5 * ((t1 - t0s[g][None, :]) / sig[g][None, :]) ** 2) for g in range(len(groups))]
+                t, rab, ms = G.launch(t, F)
+                gpu += ms * 1e-3
+                for g in range(len(groups)):
+                    series[g].append(rab[g])
+            elapsed = time.perf_counter() - tic
+    finally:
+        for B in ctxs:
+            B.close()
+    out = {}
+    for g, (_, idx, _) in enumerate(groups):
+        s_ = np.stack(series[g], axis=1)  # runs x blocks x 3
+        for j, i in enumerate(idx):
+            out[i] = s_[j]
+    return out, elapsed, gpu
+
+
 def run_single(runs, dt=2e-4, Nk=100, steps=3000, log=None):
     """The same runs one after another through the single-run path (the
     amplitude by callback per step, as CRT_Base_IF evaluates AMP_T)."""
@@ -168,21 +246,32 @@ def main(argv=None):
     ap.add_argument("--n1d", type=int, default=384)
     ap.add_argument("--n2d", type=int, default=128)
     ap.add_argument("--no-single", action="store_true", help="skip the one-after-another baseline")
+    ap.add_argument("--no-batched", action="store_true", help="skip the eager group-by-group batched run")
+    ap.add_argument("--no-tune", action="store_true", help="library defaults instead of the autotuner's choices")
     a = ap.parse_args(argv)
     runs = scan_runs(n1d=a.n1d, n2d=a.n2d)
     pts = sum(math.prod(r.shape) for r in runs) * a.steps
-    print(f"[scan] {len(runs)} runs, {a.steps} steps each", file=sys.stderr, flush=True)
-    rb, tb = run_batched(runs, steps=a.steps)
-    print(f"[scan] batched: {tb:.2f} s", file=sys.stderr, flush=True)
+
+    def log(m):
+        print(f"[scan] {m}", file=sys.stderr, flush=True)
+
+    log(f"{len(runs)} runs, {a.steps} steps each")
+    rg, tg, gpu = run_graph(runs, steps=a.steps, tune=not a.no_tune, log=log)
+    log(f"one graph per block: {tg:.2f} s, device busy {gpu / tg:.3f}")
     res = {"workload": f"{len(runs)} Bragg runs ({a.n1d} 1D n=1024..16384, {a.n2d} 2D 256^2..1024^2), "
                        f"{a.steps} steps of dt=2e-4, Rabi populations every 100 steps",
-           "batched_s": tb, "batched_Gpt_steps_per_s": pts / tb / 1e9}
+           "graph_s": tg, "graph_Gpt_steps_per_s": pts / tg / 1e9, "gpu_busy": gpu / tg}
+    if not a.no_batched:
+        rb, tb = run_batched(runs, steps=a.steps)
+        log(f"groups one after another, eager: {tb:.2f} s")
+        res.update({"batched_s": tb, "batched_Gpt_steps_per_s": pts / tb / 1e9,
+                    "graph_vs_batched_rabi_difference": max(float(np.max(np.abs(rg[i] - rb[i])))
+                                                            for i in range(len(runs)))})
     if not a.no_single:
-        rs, ts = run_single(runs, steps=a.steps, log=lambda m: print(f"[scan] one after another: {m}",
-                                                                     file=sys.stderr, flush=True))
-        print(f"[scan] one after another: {ts:.2f} s", file=sys.stderr, flush=True)
-        diff = max(float(np.max(np.abs(rb[i] - rs[i]))) for i in range(len(runs)))
-        res.update({"single_s": ts, "single_Gpt_steps_per_s": pts / ts / 1e9, "speedup": ts / tb,
+        rs, ts = run_single(runs, steps=a.steps, log=lambda m: log(f"one after another: {m}"))
+        log(f"one after another: {ts:.2f} s")
+        diff = max(float(np.max(np.abs(rg[i] - rs[i]))) for i in range(len(runs)))
+        res.update({"single_s": ts, "single_Gpt_steps_per_s": pts / ts / 1e9, "speedup": ts / tg,
                     "max_rabi_difference": diff})
     print(json.dumps(res))
 

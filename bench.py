@@ -191,19 +191,22 @@ def other_configs_run(tune=True):
 
 
 def scan_run(steps):
-    """The 512-run batched Bragg scan (seed 4: 384 1D and 128 2D runs) for
-    `steps` steps per run, batched and one run after another."""
+    """The 512-run Bragg scan (seed 4: 384 1D and 128 2D runs) for `steps`
+    steps per run: all shape groups in one captured graph per block
+    (autotuned), and the same runs one after another."""
     import numpy as np
     from atus2_amd import scan
     runs = scan.scan_runs()
     pts = sum(math.prod(r.shape) for r in runs) * steps
-    rb, tb = scan.run_batched(runs, steps=steps)
+    rg, tg, gpu = scan.run_graph(runs, steps=steps)
     rs, ts = scan.run_single(runs, steps=steps)
-    diff = max(float(np.max(np.abs(rb[i] - rs[i]))) for i in range(len(runs)))
-    log(f"[bench] scan of {len(runs)} runs x {steps} steps: batched {tb:.2f} s, one after another {ts:.2f} s")
-    return {"runs": len(runs), "steps_per_run": steps, "batched_s": tb, "single_s": ts, "speedup": ts / tb,
-            "batched_Gpt_steps_per_s": pts / tb / 1e9, "max_rabi_difference": diff,
-            "sample": f"{steps} of the configuration's 3000 steps per run (python -m atus2_amd.scan runs all)"}
+    diff = max(float(np.max(np.abs(rg[i] - rs[i]))) for i in range(len(runs)))
+    log(f"[bench] scan of {len(runs)} runs x {steps} steps: one graph per block {tg:.2f} s "
+        f"(device busy {gpu / tg:.3f}), one after another {ts:.2f} s")
+    return {"runs": len(runs), "steps_per_run": steps, "graph_s": tg, "gpu_busy": gpu / tg, "single_s": ts,
+            "speedup": ts / tg, "graph_Gpt_steps_per_s": pts / tg / 1e9, "max_rabi_difference": diff,
+            "sample": f"{steps} of the configuration's 3000 steps per run (python -m atus2_amd.scan runs all); "
+                      f"autotuning of each shape group not timed"}
 
 
 def measured_step_traffic(n, tuning):
@@ -215,8 +218,13 @@ def measured_step_traffic(n, tuning):
         return None
     with open(TRAFFIC_TABLE) as f:
         tab = json.load(f)
+    def norm(t):
+        # threads per workgroup default to 0 (automatic) for every family
+        out = {k: int(v) for k, v in t.items() if k != "threads"}
+        out["threads"] = {f: int(v) for f, v in t.get("threads", {}).items() if int(v) != 0}
+        return out
     for entry in tab.get("steps", []):
-        if entry["tuning"] == {k: int(v) for k, v in tuning.items()}:
+        if norm(entry["tuning"]) == norm(tuning):
             return entry["bytes_per_step"]
     return None
 

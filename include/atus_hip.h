@@ -254,6 +254,13 @@ int atus_run_block_profiled(atus_ctx *ctx, const atus_step_params *sp, int Nk, d
  * the contiguous passes and the kinetic step, 8 for the strided passes and
  * the fused V pass. */
 int atus_tune(atus_ctx *ctx, int family, int values_per_thread);
+/* tile choice of a kernel family (no reference counterpart): at most
+ * `threads` threads per workgroup (64, 128, 256 or 512; 0 = as many as the
+ * kernel allows, the default), i.e. threads / (N / values_per_thread)
+ * pencils per workgroup, at least one.  Fewer pencils per workgroup need
+ * less LDS, so more workgroups fit a CU, but strided passes then read rows
+ * in shorter segments.  The results do not change. */
+int atus_tune_threads(atus_ctx *ctx, int family, int threads);
 /* scheduling of 3D blocks (no reference counterpart): the y and z passes
  * between two kinetic steps only couple points of one x-plane, so they run
  * slab by slab of `planes` x-planes, each slab's passes back to back while
@@ -300,6 +307,10 @@ int atus_batch_set_runs(atus_ctx *ctx, const atus_batch_run *runs, int nbatch);
  * in atus_run_block: tt = *t + dt/2, then per step t1 = tt + dt/2, tt += dt
  * (mathematically *t + m*dt). */
 int atus_batch_run_block(atus_ctx *ctx, int kind, double dt, int Nk, double *t, const double *F);
+/* the same with the per-family device times of atus_run_block_profiled
+ * (no reference counterpart; the autotuner's measurement) */
+int atus_batch_run_block_profiled(atus_ctx *ctx, int kind, double dt, int Nk, double *t, const double *F, double *ms,
+                                  int *count, double *grid);
 /* Get_Particle_Number of every component: out[r*ncomp_per_run + c] */
 int atus_batch_particle_numbers(atus_ctx *ctx, double *out);
 /* No counterpart in the reference.  atus_batch_particle_numbers for callers
@@ -343,6 +354,27 @@ typedef struct atus_graph atus_graph;
 int atus_graph_create(atus_ctx *ctx, int kind, double dt, const atus_laser *laser, int Nk, atus_graph **out);
 int atus_graph_launch(atus_graph *graph, double *t, const double *F);
 int atus_graph_destroy(atus_graph *graph);
+
+/* ---- a parameter scan of runs on different grids as one HIP graph (no
+ * reference counterpart: the reference runs one process per case,
+ * tammy/src/batch_case_generation_4_tammy.cpp:45-291).  The block of every
+ * batch context in ctxs (atus_batch_run_block) and the Rabi integrals of
+ * component 0 of all its runs (atus_batch_rabi_integrals; none if nk0 ==
+ * 0) are captured once, each context's launches on its own stream, forked
+ * from and joined into the graph's stream: the device runs the contexts'
+ * launches side by side, and a block of the whole scan is one graph
+ * launch.  Launch: F holds each context's amplitudes [Nk][nbatch], the
+ * contexts in order (NULL: all 1); rabi receives sum_i nbatch_i x nk0
+ * populations in the same order; gpu_ms (optional) the device time of the
+ * graph (HIP events around it).  The contexts' state, the run parameters
+ * (atus_batch_set_runs) and alpha are read at each launch; dt, Nk and the
+ * momentum points are fixed.  While the graph exists its contexts refuse
+ * atus_ctx_destroy. */
+typedef struct atus_scan_graph atus_scan_graph;
+int atus_scan_graph_create(atus_ctx *const *ctxs, int nctx, int kind, double dt, int Nk, const double *k0, int nk0,
+                           double threshold, atus_scan_graph **out);
+int atus_scan_graph_launch(atus_scan_graph *graph, double *t, const double *F, double *rabi, double *gpu_ms);
+int atus_scan_graph_destroy(atus_scan_graph *graph);
 
 /* ---- learned warm start of the Sobolev minimiser (no reference
  * counterpart: the network only proposes the start state, which

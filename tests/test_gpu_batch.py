@@ -141,3 +141,63 @@ def test_scan_batched_matches_one_after_another(hip):
         # populations agree; the amplitudes come from numpy's and libm's exp
         # (a last-bit difference, amplified near the nodes of Omega)
         assert np.max(np.abs(rb[i] - rs[i])) < 1e-11, (i, runs[i].shape)
+
+
+def test_scan_graph_equals_eager_batches(hip, orc):
+    """Batch contexts of three grids (a resident 1D group, a four-step 1D
+    group and a 2D group) advanced by one captured scan graph per block:
+    states and Rabi populations bit for bit those of the eager batch blocks
+    and batched Rabi integrals, block after block."""
+    shapes = [(1024,), (8192,), (64, 128)]
+    nb, dt, Nk, blocks = 3, 1e-3, 20, 3
+    k0 = [(0.0,), (8.0,), (-8.0,)]
+    made = []
+    try:
+        for s_i, shape in enumerate(shapes):
+            dim = len(shape)
+            L = [40.0, 30.0][:dim]
+            g = orc.make_grid(shape, L=L)
+            runs = _runs(nb, 30 + s_i)
+            lasers = [hip.laser(amp0=r["amp0"], laser_k=4.0, laser_domh=r["laser_domh"], beta0=r["beta0"],
+                                deltaL=tuple(r["deltaL"])) for r in runs]
+            pair = []
+            for _ in range(2):
+                B = hip.BatchContext(shape, [g.dx[a] for a in range(dim)], [g.dk[a] for a in range(dim)], nb)
+                for r in range(nb):
+                    p1, p2 = _initial(orc, g, r)
+                    B.upload(2 * r, p1)
+                    B.upload(2 * r + 1, p2 + 0.1 * p1)
+                B.set_alpha([0.5] * dim)
+                B.set_runs([r["gs"] for r in runs], lasers)
+                pair.append(B)
+            made.append((runs, pair))
+        eager = [p[0] for _, p in made]
+        graphed = [p[1] for _, p in made]
+        G = hip.ScanGraph(graphed, hip.STEP_BRAGG_AD, dt, Nk, k0, 2.0)
+        t = 0.02
+        for blk in range(blocks):
+            t1 = np.array(hip.BatchContext.step_times(t, dt, Nk))
+            F = [np.array([[_amp(r, 0.04)(v) for r in runs] for v in t1]) for runs, _ in made]
+            te = None
+            rab_e = []
+            for B, f in zip(eager, F):
+                te = B.run_batch_block(hip.STEP_BRAGG_AD, dt, Nk, t, f)
+                rab_e.append(B.batch_rabi_integrals(0, k0, 2.0))
+            tg, rab_g, ms = G.launch(t, F)
+            assert tg == te and ms > 0
+            for s_i, (Be, Bg) in enumerate(zip(eager, graphed)):
+                assert np.array_equal(rab_g[s_i], rab_e[s_i]), (blk, shapes[s_i])
+                for c in range(2 * nb):
+                    assert np.array_equal(Bg.download(c), Be.download(c)), (blk, shapes[s_i], c)
+            t = te
+        # the graph keeps its contexts: the C-ABI refuses to destroy one,
+        # and the Python context closes the graph before itself
+        rc = hip._lib.atus_ctx_destroy(graphed[0]._h)
+        assert rc == -5
+        graphed[0].close()
+        with pytest.raises(hip.AtusError, match="closed"):
+            G.launch(t, F)
+    finally:
+        for _, pair in made:
+            for B in pair:
+                B.close()

@@ -148,3 +148,100 @@ def test_c4_scan_runs_match_oracle(hip, orc, margin, shape):
             margin(f"full run: scan run {i} {shape}, 300 steps batched, psi rel L2", err, 1e-10)
             assert err <= 1e-10
         margin(f"scan runs {shape} Rabi", worst, 1e-10)
+
+
+class SobolevRun:
+    """The ground state of held-out trap 105 of seed 55 (w = (1.82, 1.68,
+    1.47), g = 3082) on 128^3 points of [-16, 16)^3, from the learned warm
+    start: the HIP minimiser and the oracle's (CSOB_Base.h:427-505)
+    iterate side by side from the same start state."""
+
+    CASE = 105
+
+    def __init__(self, hip, orc):
+        import numpy as np
+        from atus2_amd import learned as Lr
+        self.np = np
+        self.orc = orc
+        c = Lr.cases(self.CASE + 1, 55)[self.CASE]
+        half = (Lr.L_BOX / 2,) * 3
+        dx, dk, xs = Lr._grid(Lr.N_GRID, half)
+        V = Lr._potential(c, xs)
+        self.ctx = hip.Context((Lr.N_GRID,) * 3, dx, dk, 1)
+        Lr._setup(self.ctx, c, V, Lr.STEP_128)
+        net = Lr.load_net()
+        try:
+            net.warm_start(c.features(), c.half_widths(), self.ctx, 0)
+        finally:
+            net.close()
+        start = self.ctx.download(0)
+        # the iteration count of the whole polish, from the same start
+        self.ctx.sob_renormalize()
+        self.its, res = self.ctx.sob_run(Lr.EPS, 100000)
+        assert res <= Lr.EPS
+        self.ctx.upload(0, start)
+        self.ctx.sob_renormalize()
+        g = orc.make_grid((Lr.N_GRID,) * 3, L=Lr.L_BOX)
+        self.S = orc.SobSolver(g, [start], [V], [c.g], [Lr.ALPHA], stepsize=Lr.STEP_128)
+        self.S.renormalize()
+        self.done = 0
+        self.first_below = None
+
+    def advance(self, n):
+        res = []
+        for _ in range(n):
+            r = self.S.iterate()
+            res.append(r)
+            self.done += 1
+            if self.first_below is None and r <= 1e-10:
+                self.first_below = self.done
+        got, r_gpu, _ = self.ctx.sob_iterate(n)
+        assert got == n
+        return res[-1], r_gpu
+
+    def compare(self):
+        np = self.np
+        psi = self.ctx.download(0)
+        err = float(np.linalg.norm(psi - self.S.psi[0]) / np.linalg.norm(self.S.psi[0]))
+        mu_gpu = float(self.ctx.sob_mu()[0])
+        mu_orc = float(self.S.mu()[0])
+        return err, mu_gpu, mu_orc
+
+    def close(self):
+        self.ctx.close()
+        self.S = None
+
+
+C5 = [(0, 200), (200, 1100), (1100, 2000), (2000, None)]
+
+
+@pytest.mark.parametrize("start,stop", C5)
+def test_c5_128_warm_start_polish_matches_oracle(hip, orc, margin, start, stop):
+    run = _get("c5", lambda: SobolevRun(hip, orc))
+    try:
+        assert run.its <= 3000, run.its
+        end = run.its if stop is None else stop
+        if run.done < start:
+            run.advance(start - run.done)
+        r_orc, r_gpu = run.advance(end - run.done)
+        assert abs(r_gpu / r_orc - 1) < 1e-8, (r_gpu, r_orc)
+        err, mu_gpu, mu_orc = run.compare()
+        print(f"128^3 Sobolev, {run.done} iterations of {run.its}: res {r_gpu:.3e} / {r_orc:.3e}, "
+              f"psi rel L2 {err:.2e}, mu {mu_gpu:.15f} / {mu_orc:.15f}")
+        margin(f"128^3 Sobolev after {run.done} iterations: psi rel L2", err, 1e-10)
+        margin(f"128^3 Sobolev after {run.done} iterations: mu", abs(mu_gpu - mu_orc), 1e-9)
+        assert err <= 1e-10
+        assert abs(mu_gpu - mu_orc) <= 1e-9
+        if start == 0:
+            margin("full run: ground state 128^3, first 200 Sobolev iterations from the warm start, mu",
+                   abs(mu_gpu - mu_orc), 1e-9)
+        if stop is None:
+            # the oracle's do-while stops at the same iteration
+            assert run.first_below == run.its, (run.first_below, run.its)
+            margin(f"full run: ground state 128^3, warm start polished to res 1e-10 ({run.its} iterations), mu",
+                   abs(mu_gpu - mu_orc), 1e-9)
+            margin(f"full run: ground state 128^3, warm start polished to res 1e-10 ({run.its} iterations), "
+                   "psi rel L2", err, 1e-10)
+    finally:
+        if stop is None:
+            _finish("c5")

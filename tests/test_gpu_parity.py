@@ -336,13 +336,39 @@ def test_autotune_choices_keep_parity(hip, orc):
         ctx.set_g(gs)
         ctx.set_alpha([0.5] * 3)
         t, choice = autotune(ctx, hip.STEP_BRAGG_AD, 1e-3, 0.0, las, amp, Nk=5, reps=1)
-        assert {v for f, v in choice.items() if f != "slab"} <= {8, 16} and choice["slab"] in (0, -1)
+        assert {v for f, v in choice.items() if f not in ("slab", "threads")} <= {8, 16}
+        assert choice["slab"] in (0, -1) and set(choice["threads"].values()) <= {0, 64, 128}
         start = [ctx.download(c) for c in range(2)]
         S = orc.RTSolver(g, start, gs, [0.5] * 3, dt=1e-3, t=t, amp=amp, deltaL=[0.0, 0.0], amp0=2.0,
                          laser_k=2 * np.pi / 24.0 * 4, laser_domh=1.0)
         S.block(orc.BRAGG_AD, 20)
         t = ctx.run_block(hip.STEP_BRAGG_AD, 1e-3, 20, t, las, amp)
         assert rel_l2([ctx.download(c) for c in range(2)], S.psis) < 1e-12
+
+
+@pytest.mark.parametrize("shape", [(64, 128, 32), (1024, 64), (2048,), (16384,)])
+def test_threads_per_workgroup_do_not_change_results(hip, orc, shape):
+    # the tile choice (pencils per workgroup) of every family leaves each
+    # pencil's arithmetic unchanged: blocks are bit-identical
+    dim = len(shape)
+    g = orc.make_grid(shape, L=[24.0, 20.0, 16.0][:dim])
+    psis = _state(orc, g, 43, 2)
+    las = hip.laser(amp0=3.0, laser_k=2.0, laser_domh=1.0, deltaL=(0.0, 0.2))
+    outs = []
+    for th in (0, 128, 64):
+        with hip.Context.from_grid(g, 2) as ctx:
+            for c in range(2):
+                ctx.upload(c, psis[c])
+            ctx.set_g([10.0, 9.5, 9.5, 10.0])
+            ctx.set_alpha([0.5] * dim)
+            ctx.set_slab(0)
+            for f in hip.KFAM:
+                ctx.tune_threads(f, th)
+            ctx.run_block(hip.STEP_BRAGG_AD, 1e-3, 6, 0.0, las, lambda t: 1.0)
+            outs.append([ctx.download(c) for c in range(2)])
+    for o in outs[1:]:
+        for c in range(2):
+            assert np.array_equal(o[c], outs[0][c])
 
 
 @pytest.mark.parametrize("slab", [1, 5, 8, 31])
