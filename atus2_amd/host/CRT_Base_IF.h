@@ -92,6 +92,11 @@ class CRT_Base_IF : public CRT_Base<T, dim, no_int_states> {
   atus_host::Expr m_amp_expr;
   double m_amp_time = 0.0;
   double amplitude_at(double t1) override;
+  // ATUS_DEVICE_STEPS (comma-separated sequence names): the step function a
+  // driver registered under "bragg_ad" (the reference's Do_Bragg_ad,
+  // bragg.cpp:186-304, written on the host mirrors) runs as the device step
+  // of the same formula, so an unchanged driver gets the fused path
+  void device_step_by_name(const std::string& name, StepFunction step);
   bool has_amplitude() const override { return amp_is_t; }
   void fill_step_params(atus_step_params& sp, const sequence_item& seq) override;
   atus_laser laser_params() const;
@@ -134,6 +139,21 @@ template <class T, int dim, int no_int_states>
 double CRT_Base_IF<T, dim, no_int_states>::amplitude_at(double t1) {
   m_amp_time = t1;
   return m_amp_expr.Eval();
+}
+
+template <class T, int dim, int no_int_states>
+void CRT_Base_IF<T, dim, no_int_states>::device_step_by_name(const std::string& name, StepFunction step) {
+  if (this->m_device_steps.count(step) || no_int_states != 2 || name != "bragg_ad") return;
+  const char* e = std::getenv("ATUS_DEVICE_STEPS");
+  if (!e) return;
+  std::stringstream ss(e);
+  std::string item;
+  while (std::getline(ss, item, ','))
+    if (item == name) {
+      this->m_device_steps[step] = ATUS_STEP_BRAGG_AD;
+      std::cout << "FYI: " << name << " runs as the device step (ATUS_DEVICE_STEPS)\n";
+      return;
+    }
 }
 
 // CRT_Base_IF::Amplitude_at_time (CRT_Base_IF.h:171-198)
@@ -522,6 +542,7 @@ void CRT_Base_IF<T, dim, no_int_states>::run_sequence() {
     auto it = this->m_map_stepfcts.find(seq.name);
     if (it == this->m_map_stepfcts.end()) throw std::string("Critical Error: Invalid squence name " + seq.name);
     step_fct = it->second;
+    device_step_by_name(seq.name, step_fct);
     if (seq.name == "freeprop") seq.no_of_chirps = 1;
     const double backup_t = m_header.t;
     double backup_end_t = m_header.t;

@@ -68,11 +68,23 @@ class Case:
 
 
 def cases(n, seed):
-    """w_a ~ U[0.5, 2], g ~ log-U[10, 5000]."""
+    """w_a ~ U[0.5, 2], g ~ log-U[10, 5000] (the n values of g are drawn
+    after all the w, so the first cases of a longer draw differ in g)."""
     rng = np.random.default_rng(seed)
     w = rng.uniform(0.5, 2.0, (n, 3))
     g = np.exp(rng.uniform(math.log(10.0), math.log(5000.0), n))
     return [Case(tuple(float(v) for v in w[i]), float(g[i])) for i in range(n)]
+
+
+HELD_OUT = 256
+
+
+def held_out(first=0, count=HELD_OUT):
+    """Cases first .. first+count-1 of the held-out set: cases(256, 55)
+    (never trained on; the training set is cases(4096, 5))."""
+    if first < 0 or first + count > HELD_OUT:
+        raise ValueError(f"the held-out set has {HELD_OUT} cases")
+    return cases(HELD_OUT, 55)[first:first + count]
 
 
 def _grid(n, half):
@@ -201,7 +213,7 @@ def main(argv=None):
     tr.add_argument("--lr", type=float, default=2e-3)
     tr.add_argument("--out", default=WEIGHTS)
     ev = sub.add_parser("eval")
-    ev.add_argument("--cases", type=int, default=256)
+    ev.add_argument("--cases", type=int, default=HELD_OUT)
     ev.add_argument("--first", type=int, default=0, help="start at this held-out case")
     ev.add_argument("--weights", default=WEIGHTS)
     ev.add_argument("--rows", default=None, help="write one CSV row per case here")
@@ -233,9 +245,7 @@ def main(argv=None):
     if fout:
         fout.write("case,wx,wy,wz,g,cold_iters,cold_s,warm_iters,warm_s,mu_cold_N1,mu_warm_N1,E_cold_N1,E_warm_N1,"
                    "psi_rel_l2_N1\n")
-    all_cases = cases(a.first + a.cases, 55)
-    for i in range(a.first, a.first + a.cases):
-        c = all_cases[i]
+    for i, c in enumerate(held_out(a.first, a.cases), start=a.first):
         ic, tc, _, _, pc, mc, ec = solve_large(c, "cold", state=True)
         iw, tw, _, _, pw, mw, ew = solve_large(c, "warm", net, state=True)
         d = compare_states(pw, pc)

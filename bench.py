@@ -236,7 +236,7 @@ def ground_state_run(ncases):
     net = Lr.load_net()
     rows = []
     try:
-        for c in Lr.cases(ncases, 55):
+        for c in Lr.held_out(0, ncases):
             ic, tc, _, _ = Lr.solve_large(c, "cold")
             iw, tw, _, _ = Lr.solve_large(c, "warm", net)
             rows.append((ic, tc, iw, tw))

@@ -35,17 +35,22 @@ def ref_tool(name):
     return p
 
 
-def _both(name, xml, files, tmp_path):
-    """Runs the reference's driver and the rebuilt one, each in its own
-    directory holding the same inputs; returns the two directories."""
+def _both(name, xml, files, tmp_path, ref_env=None, tag="ref"):
+    """Runs the reference's driver (with ref_env added to its environment)
+    and the rebuilt one, each in its own directory holding the same inputs;
+    returns the two directories."""
     dirs = []
-    for kind, exe in (("ref", ref_tool(name)), ("ours", tool(name))):
+    for kind, exe, extra in ((tag, ref_tool(name), ref_env or {}), ("ours", tool(name), {})):
         d = tmp_path / kind
+        if d.exists():
+            dirs.append(d)
+            continue
         d.mkdir()
         for fname, writer in files.items():
             writer(d / fname)
         (d / "run.xml").write_text(xml)
-        r = subprocess.run([exe, "run.xml"], cwd=d, capture_output=True, text=True, timeout=300)
+        r = subprocess.run([exe, "run.xml"], cwd=d, capture_output=True, text=True, timeout=300,
+                           env=dict(os.environ, **extra))
         assert r.returncode == 0, (kind, r.stdout[-3000:], r.stderr[-3000:])
         (d / "stdout.txt").write_text(r.stdout)
         dirs.append(d)
@@ -121,3 +126,10 @@ def test_reference_bragg_runs_unchanged_with_its_host_step(tmp_path):
     assert len(ra) == 5 and ra[0] == rb[0]
     for la, lb in zip(ra[1:], rb[1:]):
         assert np.allclose([float(v) for v in la.split()], [float(v) for v in lb.split()], rtol=1e-5, atol=1e-9)
+    # ATUS_DEVICE_STEPS=bragg_ad: the same unchanged driver takes the
+    # device step, and matches the rebuilt driver bit for bit
+    dev, ours = _both("bragg", xml, files, tmp_path, {"ATUS_DEVICE_STEPS": "bragg_ad"}, tag="ref_device")
+    assert "bragg_ad runs as the device step" in (dev / "stdout.txt").read_text()
+    for c in (1, 2):
+        assert (dev / f"0.040_{c}.bin").read_bytes() == (ours / f"0.040_{c}.bin").read_bytes()
+    assert (dev / "Rabi_1_0.txt").read_text() == (ours / "Rabi_1_0.txt").read_text()

@@ -151,19 +151,20 @@ def test_c4_scan_runs_match_oracle(hip, orc, margin, shape):
 
 
 class SobolevRun:
-    """The ground state of held-out trap 105 of seed 55 (w = (1.82, 1.68,
-    1.47), g = 3082) on 128^3 points of [-16, 16)^3, from the learned warm
-    start: the HIP minimiser and the oracle's (CSOB_Base.h:427-505)
-    iterate side by side from the same start state."""
+    """The ground state of held-out trap 238 (w = (1.25, 1.14, 1.26),
+    g = 4482; about 3000 iterations from the warm start) on 128^3 points of
+    [-16, 16)^3, from the learned warm start: the HIP minimiser and the
+    oracle's (CSOB_Base.h:427-505) iterate side by side from the same start
+    state."""
 
-    CASE = 105
+    CASE = 238
 
     def __init__(self, hip, orc):
         import numpy as np
         from atus2_amd import learned as Lr
         self.np = np
         self.orc = orc
-        c = Lr.cases(self.CASE + 1, 55)[self.CASE]
+        c = Lr.held_out(self.CASE, 1)[0]
         half = (Lr.L_BOX / 2,) * 3
         dx, dk, xs = Lr._grid(Lr.N_GRID, half)
         V = Lr._potential(c, xs)
@@ -212,14 +213,14 @@ class SobolevRun:
         self.S = None
 
 
-C5 = [(0, 200), (200, 1100), (1100, 2000), (2000, None)]
+C5 = [(0, 200), (200, 1100), (1100, 2000), (2000, 2900), (2900, None)]
 
 
 @pytest.mark.parametrize("start,stop", C5)
 def test_c5_128_warm_start_polish_matches_oracle(hip, orc, margin, start, stop):
     run = _get("c5", lambda: SobolevRun(hip, orc))
     try:
-        assert run.its <= 3000, run.its
+        assert 2900 < run.its <= 2900 + 1024, run.its
         end = run.its if stop is None else stop
         if run.done < start:
             run.advance(start - run.done)

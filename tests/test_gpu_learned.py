@@ -141,7 +141,7 @@ def test_trained_warm_start_gains_over_the_cold_start(hip):
     net = Lr.load_net()
     tc_sum = tw_sum = 0.0
     try:
-        for c in Lr.cases(4, 55):
+        for c in Lr.held_out(0, 4):
             ic, tc, _, rc, pc, mc, ec = Lr.solve_large(c, "cold", state=True)
             iw, tw, _, rw, pw, mw, ew = Lr.solve_large(c, "warm", net, state=True)
             d = Lr.compare_states(pw, pc)
