@@ -132,16 +132,42 @@ struct ProfScope {
   }
 };
 
+// twiddle tables of length N: W^j = e^{-2 pi i j / N} (j < N), then the
+// stage-major tables of the plans of 8 and 16 values per thread (fft_tw):
+// the same values, in the order the stages read them (lengths beyond the
+// largest pencil, 4096, only serve as four-step factors: natural only)
 int ensure_twiddles(atus_ctx* c, int N) {
   if (c->tw.count(N)) return ATUS_OK;
-  std::vector<double2> h(N);
+  const int ntab = N <= 4096 ? TW_TABLES : 1;
+  std::vector<double2> h((size_t)ntab * N);
   for (int j = 0; j < N; ++j) {
     long double a = -2.0L * 3.141592653589793238462643383279502884L * (long double)j / (long double)N;
     h[j] = make_double2((double)cosl(a), (double)sinl(a));
   }
+  for (int slot = 1; slot < ntab; ++slot) {
+    const int rm = slot == 1 ? 8 : 16;
+    const int R = N < rm ? N : rm;
+    int A = 0, RA = 1;
+    while (RA * R <= N) { RA *= R; ++A; }
+    const int Q = N / RA;
+    const int S = A + (Q > 1 ? 1 : 0);
+    double2* t = h.data() + (size_t)slot * N;
+    int off = 0, Ls = 1;
+    for (int s = 0; s < S; ++s) {
+      const int r = s < A ? R : Q;
+      if (s >= 1) {
+        const int step = N / (Ls * r);
+        for (int m = 1; m < r; ++m)
+          for (int k = 0; k < Ls; ++k) t[off + (m - 1) * Ls + k] = h[(size_t)k * m * step];
+        off += (r - 1) * Ls;
+      }
+      Ls *= r;
+    }
+    if (off > N - 1) return fail(ATUS_ERR_UNSUPPORTED, "stage-major twiddle table overflow");
+  }
   double2* d = nullptr;
-  if (hipMalloc(&d, sizeof(double2) * N) != hipSuccess) return fail(ATUS_ERR_NOMEM, "twiddle table allocation");
-  HIP_TRY(hipMemcpy(d, h.data(), sizeof(double2) * N, hipMemcpyHostToDevice));
+  if (hipMalloc(&d, sizeof(double2) * h.size()) != hipSuccess) return fail(ATUS_ERR_NOMEM, "twiddle table allocation");
+  HIP_TRY(hipMemcpy(d, h.data(), sizeof(double2) * h.size(), hipMemcpyHostToDevice));
   c->tw[N] = d;
   return ATUS_OK;
 }
@@ -149,21 +175,25 @@ int ensure_twiddles(atus_ctx* c, int N) {
 // per-axis kinetic phase tables for frac 0.5 (slot 0) and 1 (slot 1):
 //   E_a[j] = exp(i * s * phi_a(j)),  phi_a = -dt * (k_a alpha_a) k_a
 // (CRT_Base::Init, CRT_Base.h:253-274, factorised over the axes); axis 0
-// carries the ft(-1) * ft(+1) scale.
+// carries the ft(-1) * ft(+1) scale.  Slot f = 0: half step, 1: full step,
+// 2: half step for a transform pair along one axis alone (every axis's
+// table carries that axis's share dx dk / 2pi of the scale).
 void kin_tables(const atus_ctx* c, const double* alpha, double dt, std::vector<double2>& h) {
   const int nm = c->kin_nmax;
-  h.assign((size_t)2 * 3 * nm, make_double2(1.0, 0.0));
+  h.assign((size_t)atus_ctx::KIN_SLOTS_N * 3 * nm, make_double2(1.0, 0.0));
   const double scale = ft_factor(c->dim, c->dx) * ft_factor(c->dim, c->dk);
   const double mdt = -dt;
-  for (int f = 0; f < 2; ++f) {
+  for (int f = 0; f < atus_ctx::KIN_SLOTS_N; ++f) {
     for (int a = 0; a < c->dim; ++a) {
       const int N = c->n[a], sh = N / 2;
+      const double sa = 0.5 * c->dx[a] * c->dk[a] / M_PI;
       for (int j = 0; j < N; ++j) {
         const double k = c->dk[a] * (double)(((j + sh) % N) - sh);
         const double phi = mdt * ((k * alpha[a]) * k);
-        const double ph = (f == 0) ? 0.5 * phi : phi;
+        const double ph = (f == 1) ? phi : 0.5 * phi;
         double2 e = make_double2(cos(ph), sin(ph));
-        if (a == 0) e = make_double2(e.x * scale, e.y * scale);
+        if (f == 2) e = make_double2(e.x * sa, e.y * sa);
+        else if (a == 0) e = make_double2(e.x * scale, e.y * scale);
         h[((size_t)f * 3 + a) * nm + j] = e;
       }
     }
@@ -178,7 +208,7 @@ int ensure_kin(atus_ctx* c, double dt) {
   HIP_TRY(hipMemcpyAsync(c->kin, h.data(), sizeof(double2) * h.size(), hipMemcpyHostToDevice, c->stream));
   if (c->alpha2_set) {
     // CRT_Base_2::Init (CRT_Base_2.h:195-221): the tables of Alpha_2
-    if (!c->kin2 && hipMalloc(&c->kin2, sizeof(double2) * 2 * 3 * nm) != hipSuccess)
+    if (!c->kin2 && hipMalloc(&c->kin2, sizeof(double2) * atus_ctx::KIN_SLOTS_N * 3 * nm) != hipSuccess)
       return fail(ATUS_ERR_NOMEM, "second-species tables");
     kin_tables(c, c->alpha2, dt, h2);
     HIP_TRY(hipMemcpyAsync(c->kin2, h2.data(), sizeof(double2) * h2.size(), hipMemcpyHostToDevice, c->stream));
@@ -216,6 +246,14 @@ KinTabs kin_tabs(atus_ctx* c, int f) {
     kt.f2 = c->dim > 2 ? c->kin2 + ((size_t)f * 3 + 2) * nm : nullptr;
     kt.split = c->ncomp / 2;
   }
+  return kt;
+}
+
+// the half step of axis a alone (slot 2): the pencil's own axis only
+KinTabs axis_tabs(atus_ctx* c, int a) {
+  KinTabs kt{};
+  kt.e0 = c->kin + ((size_t)2 * 3 + a) * c->kin_nmax;
+  kt.n2 = 1;
   return kt;
 }
 
@@ -257,7 +295,9 @@ int pass_slab(atus_ctx* c, int a, int dir, int s0, int S, hipStream_t st) {
 // the fused inverse-z / V / forward-z pass over the x-planes [s0, s0 + S):
 // the kernel's x index counts from the slab, so n0 is shifted to keep
 // x0 = (i0 - n0/2) dx0 (n0 even), and the potential pointer moves along
-int vfuse_slab(atus_ctx* c, int vk, const VArgs& va, int s0, int S, hipStream_t st) {
+// edge: 1 / 2 / 3 adds z's share of the half kinetic step before / after /
+// on both sides of the V step (the first and last steps of a block)
+int vfuse_slab(atus_ctx* c, int vk, const VArgs& va, int s0, int S, hipStream_t st, int edge = 0) {
   const int NL = c->n[2];
   const long long plane = (long long)c->n[1] * c->n[2];
   VArgs vs = va;
@@ -265,7 +305,19 @@ int vfuse_slab(atus_ctx* c, int vk, const VArgs& va, int s0, int S, hipStream_t 
   ProfScope ps(c, ATUS_KFAM_VFUSE, (double)S / c->n[0], st);
   HIP_TRY(launch_vfuse(NL, c->rm[ATUS_KFAM_VFUSE], vk, c->ncomp, c->psi + (long long)s0 * plane, c->npts,
                        (long long)S * c->n[1], c->tw[NL], vs, c->n[1], c->n[0] - 2 * s0, c->dx[0], lim(c, ATUS_KFAM_VFUSE),
-                       st));
+                       st, edge ? axis_tabs(c, 2).e0 : nullptr, edge));
+  return ATUS_OK;
+}
+
+// y's share of the half kinetic step (forward DFT, phase, inverse DFT along
+// axis 1) over the x-planes [s0, s0 + S) of all components
+int kin_y_slab(atus_ctx* c, const VArgs& va, int s0, int S, hipStream_t st) {
+  const int N = c->n[1];
+  ATUS_TRY(ensure_twiddles(c, N));
+  const long long plane = (long long)c->n[1] * c->n[2];
+  ProfScope ps(c, ATUS_KFAM_KIN, (double)S / c->n[0], st);
+  HIP_TRY(launch_kin(N, c->rm[ATUS_KFAM_KIN], false, V_NONE, 1, c->psi + (long long)s0 * plane, c->npts, c->ncomp,
+                     c->n[2], c->tw[N], axis_tabs(c, 1), va, c->dx[0], lim(c, ATUS_KFAM_KIN), st, S));
   return ATUS_OK;
 }
 
@@ -454,6 +506,69 @@ int reduce_seg(atus_ctx* c, const Fn& fn, long long n, long long seg_stride, int
   return reduce_seg_into(c, fn, n, seg_stride, nseg, nb, c->sred_part, c->sred_out, out);
 }
 
+// The Rabi integrals' reduction (RedRabi): over the index box of the balls
+// |k - k0_q| < thr when that box is under half the grid (RedRabiBox, the
+// same sums over far fewer points), else over every point.  Per axis the
+// box spans the centred indices m with |dk m - k0_a| < thr for some q, one
+// index of margin on each side.
+struct RabiReduction {
+  RedRabi full{};
+  RedRabiBox box{};
+  bool use_box = false;
+  long long npts = 0;  // points the reduction visits
+};
+
+RabiReduction rabi_reduction(const atus_ctx* c, const double2* psik, const double* k0, int n, double thr) {
+  RabiReduction r;
+  r.full.psik = psik;
+  r.full.gr = c->grid();
+  r.full.nk0 = n;
+  for (int q = 0; q < n; ++q)
+    for (int a = 0; a < 3; ++a) r.full.k0[q][a] = k0[3 * q + a];
+  r.full.thr = thr;
+  r.npts = c->npts;
+  long long vol = 1;
+  bool ok = std::isfinite(thr) && thr > 0;
+  for (int a = 0; a < 3 && ok; ++a) {
+    if (a >= c->dim) {
+      r.box.lo[a] = 0;
+      r.box.len[a] = 1;
+      continue;
+    }
+    const long long N = c->n[a], s = N / 2;
+    double lo = INFINITY, hi = -INFINITY;
+    for (int q = 0; q < n; ++q) {
+      lo = std::min(lo, std::floor((k0[3 * q + a] - thr) / c->dk[a]) - 1.0);
+      hi = std::max(hi, std::ceil((k0[3 * q + a] + thr) / c->dk[a]) + 1.0);
+    }
+    if (!(lo <= hi)) {
+      ok = false;
+      break;
+    }
+    const long long mlo = (long long)std::max(lo, (double)-s), mhi = (long long)std::min(hi, (double)(N - s - 1));
+    if (mhi < mlo) {
+      // no grid point near any ball along this axis: the plain reduction
+      // (all sums zero)
+      ok = false;
+      break;
+    }
+    r.box.lo[a] = (int)mlo;
+    r.box.len[a] = (int)(mhi - mlo + 1);
+    vol *= r.box.len[a];
+  }
+  if (ok && vol > 0 && vol * 2 < c->npts && vol < (1LL << 31)) {
+    r.use_box = true;
+    r.box.psik = psik;
+    r.box.gr = r.full.gr;
+    r.box.nk0 = n;
+    for (int q = 0; q < n; ++q)
+      for (int a = 0; a < 3; ++a) r.box.k0[q][a] = r.full.k0[q][a];
+    r.box.thr = thr;
+    r.npts = vol;
+  }
+  return r;
+}
+
 int vkind_of(int kind) {
   switch (kind) {
     case ATUS_STEP_FREEPROP: return V_NL;
@@ -584,7 +699,8 @@ int ctx_create(atus_ctx** out, int dim, const long long n[3], const double dx[3]
     return bail(ATUS_ERR_NOMEM, "wavefunction allocation (" + std::to_string(sizeof(double2) * c->npts * ncomp) +
                                     " bytes)");
   if (hipMemset(c->psi, 0, sizeof(double2) * c->npts * ncomp) != hipSuccess) return bail(ATUS_ERR_HIP, "memset");
-  if (hipMalloc(&c->kin, sizeof(double2) * 2 * 3 * c->kin_nmax) != hipSuccess) return bail(ATUS_ERR_NOMEM, "tables");
+  if (hipMalloc(&c->kin, sizeof(double2) * atus_ctx::KIN_SLOTS_N * 3 * c->kin_nmax) != hipSuccess)
+    return bail(ATUS_ERR_NOMEM, "tables");
   if (c->fs && (hipMalloc(&c->kinz, sizeof(double2) * 2 * c->npts) != hipSuccess ||
                 ensure_twiddles(c, c->fs_n1) != ATUS_OK || ensure_twiddles(c, c->fs_n2) != ATUS_OK))
     return bail(ATUS_ERR_NOMEM, "four-step tables");
@@ -882,24 +998,32 @@ int step_vargs(const VArgs& base, int m, double tt, double dt, const StepFeed& f
 }
 
 // a 3D block with the y and z passes between two kinetic steps chained slab
-// by slab of c->slab x-planes (they only couple points of one x-plane):
-//   per slab [z fwd, y fwd]; kin; per step per slab [y inv, vfuse, y fwd],
-//   kin; per slab [y inv, z inv]
+// by slab of c->slab x-planes (they only couple points of one x-plane).
+// Between two kinetic steps the state is kept in x space along x and in k
+// space along y and z; a step is
+//   per slab [y inv, vfuse (z inv, V, z fwd), y fwd]; kin (x fwd, phase, x inv).
+// The half kinetic steps at the block's ends run per axis (the three axes'
+// phases commute), so that they fold into the first and last steps:
+//   kin_x/2; per slab [kin_y/2, vfuse with z's half step before V, y fwd]; kin
+//   ... per slab [y inv, vfuse with z's half step after V, kin_y/2]; kin_x/2
+// which is 4 Nk + 1 passes over the grid per block of Nk steps.
 // Each slab's passes follow each other while the slab's data is still in
-// the device's last-level cache.  Same kernels and arithmetic as the
-// unslabbed order.
+// the device's last-level cache.
 int single_block_slabs(atus_ctx* c, int vk, const VArgs& base, int Nk, double dt, double* t, const StepFeed& fd) {
-  const int S = c->slab_planes(), n0 = c->n[0], N0 = c->n[0];
+  // without slabs (a grid of at most one slab's size, or slabs switched
+  // off) the whole grid is one slab on the context's stream
+  const int n0 = c->n[0], N0 = c->n[0];
+  const int S = c->slab_planes() > 0 ? c->slab_planes() : n0;
   const long long inner0 = c->inner_of(0);
-  auto kin = [&](int f, const VArgs& va) -> int {
+  auto kin = [&](const KinTabs& kt, const VArgs& va) -> int {
     ProfScope ps(c, ATUS_KFAM_KIN);
-    HIP_TRY(launch_kin(N0, c->rm[ATUS_KFAM_KIN], false, V_NONE, 1, c->psi, c->npts, c->ncomp, inner0, c->tw[N0],
-                       kin_tabs(c, f), va, c->dx[0], lim(c, ATUS_KFAM_KIN), c->stream));
+    HIP_TRY(launch_kin(N0, c->rm[ATUS_KFAM_KIN], false, V_NONE, 1, c->psi, c->npts, c->ncomp, inner0, c->tw[N0], kt,
+                       va, c->dx[0], lim(c, ATUS_KFAM_KIN), c->stream));
     return ATUS_OK;
   };
   // odd slabs on the second stream (profiled blocks record their events on
   // each launch's stream)
-  const int K = std::max(1, std::min(c->slab_streams, atus_ctx::MAX_SLAB_STREAMS));
+  const int K = S >= n0 ? 1 : std::max(1, std::min(c->slab_streams, atus_ctx::MAX_SLAB_STREAMS));
   if (K > 1 && !c->fork_ev &&
       hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming) != hipSuccess)
     return fail(ATUS_ERR_HIP, "fork event of the slab chains");
@@ -927,36 +1051,25 @@ int single_block_slabs(atus_ctx* c, int vk, const VArgs& base, int Nk, double dt
     return ATUS_OK;
   };
   double tt = *t;
-  ATUS_TRY(fork());
-  for (int s0 = 0; s0 < n0; s0 += S) {
-    const int Ss = std::min(S, n0 - s0);
-    ATUS_TRY(pass_slab(c, 2, -1, s0, Ss, st_of(s0)));
-    ATUS_TRY(pass_slab(c, 1, -1, s0, Ss, st_of(s0)));
-  }
-  ATUS_TRY(join());
-  ATUS_TRY(kin(0, base));
+  ATUS_TRY(kin(axis_tabs(c, 0), base));
   tt += 0.5 * dt;
   for (int m = 1; m <= Nk; ++m) {
     VArgs va;
     ATUS_TRY(step_vargs(base, m, tt, dt, fd, va));
+    const bool first = m == 1, last = m == Nk;
     ATUS_TRY(fork());
     for (int s0 = 0; s0 < n0; s0 += S) {
       const int Ss = std::min(S, n0 - s0);
-      ATUS_TRY(pass_slab(c, 1, +1, s0, Ss, st_of(s0)));
-      ATUS_TRY(vfuse_slab(c, vk, va, s0, Ss, st_of(s0)));
-      ATUS_TRY(pass_slab(c, 1, -1, s0, Ss, st_of(s0)));
+      if (first) ATUS_TRY(kin_y_slab(c, va, s0, Ss, st_of(s0)));
+      else ATUS_TRY(pass_slab(c, 1, +1, s0, Ss, st_of(s0)));
+      ATUS_TRY(vfuse_slab(c, vk, va, s0, Ss, st_of(s0), (first ? 1 : 0) | (last ? 2 : 0)));
+      if (last) ATUS_TRY(kin_y_slab(c, va, s0, Ss, st_of(s0)));
+      else ATUS_TRY(pass_slab(c, 1, -1, s0, Ss, st_of(s0)));
     }
     ATUS_TRY(join());
-    ATUS_TRY(kin(m < Nk ? 1 : 0, va));
-    tt += (m < Nk) ? dt : 0.5 * dt;
+    ATUS_TRY(kin(last ? axis_tabs(c, 0) : kin_tabs(c, 1), va));
+    tt += last ? 0.5 * dt : dt;
   }
-  ATUS_TRY(fork());
-  for (int s0 = 0; s0 < n0; s0 += S) {
-    const int Ss = std::min(S, n0 - s0);
-    ATUS_TRY(pass_slab(c, 1, +1, s0, Ss, st_of(s0)));
-    ATUS_TRY(pass_slab(c, 2, +1, s0, Ss, st_of(s0)));
-  }
-  ATUS_TRY(join());
   *t = tt;
   return ATUS_OK;
 }
@@ -1037,24 +1150,25 @@ int single_block(atus_ctx* c, int vk, const VArgs& base, int Nk, double dt, doub
   // species (per-component kinetic tables, four-component steps) run the
   // V step and the kinetic step as separate passes
   const bool fused = c->ncomp <= 2 && !c->alpha2_set && (vk == V_NL || vk == V_NL_IF || vk == V_BRAGG);
-  if (fused && c->slab_planes() > 0) return single_block_slabs(c, vk, base, Nk, dt, t, fd);
+  if (fused && c->dim == 3) return single_block_slabs(c, vk, base, Nk, dt, t, fd);
   if (fused && !fd.ds && resident_ok(c)) return resident_block(c, vk, base, Nk, dt, t, fd, false);
   const int N0 = c->n[0], NL = c->n[c->dim - 1];
   const long long inner0 = c->inner_of(0);
   double tt = *t;
-  // half kinetic step
+  auto kin2d = [&](const KinTabs& kt, const VArgs& va) -> int {
+    ProfScope ps(c, ATUS_KFAM_KIN);
+    HIP_TRY(launch_kin(N0, c->rm[ATUS_KFAM_KIN], false, V_NONE, 1, c->psi, c->npts, c->ncomp, inner0, c->tw[N0], kt,
+                       va, c->dx[0], lim(c, ATUS_KFAM_KIN), c->stream));
+    return ATUS_OK;
+  };
+  // half kinetic step; a 2D grid keeps y in k space between two kinetic
+  // steps, and the half steps at the block's ends run per axis: x's share
+  // here and at the end, y's inside the first and last V steps' passes
   if (fused && c->fs) {
     ATUS_TRY(fs_col(c, FS_ENTER, c->psi, c->ncomp, c->npts, V_NONE, 1, false, base, 1.0));
     ATUS_TRY(fs_row_kin(c, 0, base));
-  } else if (fused && c->dim >= 2) {
-    ATUS_TRY(pass(c, c->dim - 1, -1, c->psi, c->psi, c->ncomp, 1.0));
-    if (c->dim == 3) ATUS_TRY(pass(c, 1, -1, c->psi, c->psi, c->ncomp, 1.0));
-    {
-      ProfScope ps(c, ATUS_KFAM_KIN);
-      HIP_TRY(launch_kin(N0, c->rm[ATUS_KFAM_KIN], false, V_NONE, 1, c->psi, c->npts, c->ncomp, inner0, c->tw[N0], kin_tabs(c, 0), base,
-                         c->dx[0], lim(c, ATUS_KFAM_KIN), c->stream));
-    }
-    if (c->dim == 3) ATUS_TRY(pass(c, 1, +1, c->psi, c->psi, c->ncomp, 1.0));
+  } else if (fused && c->dim == 2) {
+    ATUS_TRY(kin2d(axis_tabs(c, 0), base));
   } else {
     ATUS_TRY(kinetic_unfused(c, 0));
   }
@@ -1077,28 +1191,18 @@ int single_block(atus_ctx* c, int vk, const VArgs& base, int Nk, double dt, doub
       HIP_TRY(launch_kin(N0, c->rm[ATUS_KFAM_KIN], true, vk, c->ncomp, c->psi, c->npts, c->ncomp, 1, c->tw[N0], kin_tabs(c, f), va,
                          c->dx[0], lim(c, ATUS_KFAM_KIN), c->stream));
     } else {
-      const long long outer = c->npts / NL;
-      const long long ppx = c->dim == 3 ? c->n[1] : 1;
+      const int edge = (m == 1 ? 1 : 0) | (m == Nk ? 2 : 0);
       {
         ProfScope ps(c, ATUS_KFAM_VFUSE);
-        HIP_TRY(launch_vfuse(NL, c->rm[ATUS_KFAM_VFUSE], vk, c->ncomp, c->psi, c->npts, outer, c->tw[NL], va, ppx, c->n[0], c->dx[0],
-                             lim(c, ATUS_KFAM_VFUSE), c->stream));
+        HIP_TRY(launch_vfuse(NL, c->rm[ATUS_KFAM_VFUSE], vk, c->ncomp, c->psi, c->npts, c->npts / NL, c->tw[NL], va, 1,
+                             c->n[0], c->dx[0], lim(c, ATUS_KFAM_VFUSE), c->stream, edge ? axis_tabs(c, 1).e0 : nullptr,
+                             edge));
       }
-      if (c->dim == 3) ATUS_TRY(pass(c, 1, -1, c->psi, c->psi, c->ncomp, 1.0));
-      {
-        ProfScope ps(c, ATUS_KFAM_KIN);
-        HIP_TRY(launch_kin(N0, c->rm[ATUS_KFAM_KIN], false, V_NONE, 1, c->psi, c->npts, c->ncomp, inner0, c->tw[N0], kin_tabs(c, f), va,
-                           c->dx[0], lim(c, ATUS_KFAM_KIN), c->stream));
-      }
-      if (c->dim == 3) ATUS_TRY(pass(c, 1, +1, c->psi, c->psi, c->ncomp, 1.0));
+      ATUS_TRY(kin2d(m < Nk ? kin_tabs(c, 1) : axis_tabs(c, 0), va));
     }
     tt += (m < Nk) ? dt : 0.5 * dt;
   }
-  if (fused && c->fs) {
-    ATUS_TRY(fs_col(c, FS_EXIT, c->psi, c->ncomp, c->npts, V_NONE, 1, false, base, 1.0));
-  } else if (fused && c->dim >= 2) {
-    ATUS_TRY(pass(c, c->dim - 1, +1, c->psi, c->psi, c->ncomp, 1.0));
-  }
+  if (fused && c->fs) ATUS_TRY(fs_col(c, FS_EXIT, c->psi, c->ncomp, c->npts, V_NONE, 1, false, base, 1.0));
   *t = tt;
   return ATUS_OK;
 }
@@ -1117,18 +1221,20 @@ int batch_block(atus_ctx* c, int vk, const VArgs& base, int Nk, double dt, doubl
   const int N0 = c->n[0], NL = c->n[c->dim - 1];
   const long long inner0 = c->inner_of(0);
   double tt = *t;
+  auto kin = [&](const KinTabs& kt, const VArgs& va) -> int {
+    ProfScope ps(c, ATUS_KFAM_KIN);
+    HIP_TRY(launch_kin(N0, c->rm[ATUS_KFAM_KIN], false, V_NONE, 1, c->psi, c->npts, c->ncomp, inner0, c->tw[N0], kt,
+                       va, c->dx[0], lim(c, ATUS_KFAM_KIN), c->stream));
+    return ATUS_OK;
+  };
+  // 2D/3D: the order of a single run's block (x in x space, the other axes
+  // in k space between two kinetic steps; the half steps at the block's ends
+  // per axis, folded into the first and last steps)
   if (c->fs) {
     ATUS_TRY(fs_col(c, FS_ENTER, c->psi, c->ncomp, c->npts, V_NONE, 1, false, base, 1.0));
     ATUS_TRY(fs_row_kin(c, 0, base));
   } else if (c->dim >= 2) {
-    ATUS_TRY(pass(c, c->dim - 1, -1, c->psi, c->psi, c->ncomp, 1.0));
-    if (c->dim == 3) ATUS_TRY(pass(c, 1, -1, c->psi, c->psi, c->ncomp, 1.0));
-    {
-      ProfScope ps(c, ATUS_KFAM_KIN);
-      HIP_TRY(launch_kin(N0, c->rm[ATUS_KFAM_KIN], false, V_NONE, 1, c->psi, c->npts, c->ncomp, inner0, c->tw[N0],
-                         kin_tabs(c, 0), base, c->dx[0], lim(c, ATUS_KFAM_KIN), c->stream));
-    }
-    if (c->dim == 3) ATUS_TRY(pass(c, 1, +1, c->psi, c->psi, c->ncomp, 1.0));
+    ATUS_TRY(kin(axis_tabs(c, 0), base));
   } else {
     ATUS_TRY(kinetic_unfused(c, 0));
   }
@@ -1147,28 +1253,29 @@ int batch_block(atus_ctx* c, int vk, const VArgs& base, int Nk, double dt, doubl
       HIP_TRY(launch_vstep_batch(N0, c->rm[ATUS_KFAM_KIN], true, vk, c->psi, c->npts, B, c->tw[N0], kin_tabs(c, f), va,
                                  1, N0, c->dx[0], lim(c, ATUS_KFAM_KIN), c->stream));
     } else {
+      const bool first = m == 1, last = m == Nk;
+      const int edge = (first ? 1 : 0) | (last ? 2 : 0);
       const long long ppx = c->dim == 3 ? c->n[1] : 1;
+      if (c->dim == 3) {
+        if (first) ATUS_TRY(kin_y_slab(c, va, 0, c->n[0], c->stream));
+        else ATUS_TRY(pass(c, 1, +1, c->psi, c->psi, c->ncomp, 1.0));
+      }
       {
         ProfScope ps(c, ATUS_KFAM_VFUSE);
         HIP_TRY(launch_vstep_batch(NL, c->rm[ATUS_KFAM_VFUSE], false, vk, c->psi, c->npts,
                                    (long long)B * base.pencils_per_run, c->tw[NL], kin_tabs(c, f), va, ppx, c->n[0],
-                                   c->dx[0], lim(c, ATUS_KFAM_VFUSE), c->stream));
+                                   c->dx[0], lim(c, ATUS_KFAM_VFUSE), c->stream,
+                                   edge ? axis_tabs(c, c->dim - 1).e0 : nullptr, edge));
       }
-      if (c->dim == 3) ATUS_TRY(pass(c, 1, -1, c->psi, c->psi, c->ncomp, 1.0));
-      {
-        ProfScope ps(c, ATUS_KFAM_KIN);
-        HIP_TRY(launch_kin(N0, c->rm[ATUS_KFAM_KIN], false, V_NONE, 1, c->psi, c->npts, c->ncomp, inner0, c->tw[N0],
-                           kin_tabs(c, f), va, c->dx[0], lim(c, ATUS_KFAM_KIN), c->stream));
+      if (c->dim == 3) {
+        if (last) ATUS_TRY(kin_y_slab(c, va, 0, c->n[0], c->stream));
+        else ATUS_TRY(pass(c, 1, -1, c->psi, c->psi, c->ncomp, 1.0));
       }
-      if (c->dim == 3) ATUS_TRY(pass(c, 1, +1, c->psi, c->psi, c->ncomp, 1.0));
+      ATUS_TRY(kin(last ? axis_tabs(c, 0) : kin_tabs(c, 1), va));
     }
     tt += (m < Nk) ? dt : 0.5 * dt;
   }
-  if (c->fs) {
-    ATUS_TRY(fs_col(c, FS_EXIT, c->psi, c->ncomp, c->npts, V_NONE, 1, false, base, 1.0));
-  } else if (c->dim >= 2) {
-    ATUS_TRY(pass(c, c->dim - 1, +1, c->psi, c->psi, c->ncomp, 1.0));
-  }
+  if (c->fs) ATUS_TRY(fs_col(c, FS_EXIT, c->psi, c->ncomp, c->npts, V_NONE, 1, false, base, 1.0));
   *t = tt;
   return ATUS_OK;
 }
@@ -1397,15 +1504,13 @@ int atus_rabi_integrals(atus_ctx* c, int comp, const double* k0, int n, double t
   if (n < 1 || n > RED_MAX_NV || !k0 || !out) return fail(ATUS_ERR_ARG, "need 1..8 momentum points");
   ATUS_TRY(ensure_scratch(c));
   ATUS_TRY(full_ft(c, -1, c->psi + comp * c->npts, c->scratch, 1));
-  RedRabi fn{};
-  fn.psik = c->scratch;
-  fn.gr = c->grid();
-  fn.nk0 = n;
-  for (int q = 0; q < n; ++q)
-    for (int a = 0; a < 3; ++a) fn.k0[q][a] = k0[3 * q + a];
-  fn.thr = threshold;
-  double s[RED_MAX_NV];
-  ATUS_TRY(reduce(c, fn, c->npts, s));
+  const RabiReduction rr = rabi_reduction(c, c->scratch, k0, n, threshold);
+  double s[RED_MAX_NV] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (rr.use_box) {
+    ATUS_TRY(reduce(c, rr.box, rr.npts, s));
+  } else {
+    ATUS_TRY(reduce(c, rr.full, c->npts, s));
+  }
   for (int q = 0; q < n; ++q) out[q] = c->ar_k() * s[q];
   return ATUS_OK;
 }
@@ -1945,15 +2050,13 @@ int atus_batch_rabi_integrals(atus_ctx* c, int comp, const double* k0, int n, do
   const long long cs = (long long)c->nc_run * c->npts;
   double2* p = c->psi + (long long)comp * c->npts;
   ATUS_TRY(full_ft(c, -1, p, p, c->nbatch, cs));
-  RedRabi fn{};
-  fn.psik = p;
-  fn.gr = c->grid();
-  fn.nk0 = n;
-  for (int q = 0; q < n; ++q)
-    for (int a = 0; a < 3; ++a) fn.k0[q][a] = k0[3 * q + a];
-  fn.thr = threshold;
-  std::vector<double> s((size_t)c->nbatch * RED_MAX_NV);
-  ATUS_TRY(reduce_seg(c, fn, c->npts, cs, c->nbatch, s.data()));
+  const RabiReduction rr = rabi_reduction(c, p, k0, n, threshold);
+  std::vector<double> s((size_t)c->nbatch * RED_MAX_NV, 0.0);
+  if (rr.use_box) {
+    ATUS_TRY(reduce_seg(c, rr.box, rr.npts, cs, c->nbatch, s.data()));
+  } else {
+    ATUS_TRY(reduce_seg(c, rr.full, c->npts, cs, c->nbatch, s.data()));
+  }
   ATUS_TRY(full_ft(c, +1, p, p, c->nbatch, cs));
   HIP_TRY(hipStreamSynchronize(c->stream));
   for (int r = 0; r < c->nbatch; ++r)
@@ -2136,16 +2239,13 @@ int batch_rabi_launch(atus_ctx* c, const double* k0, int n, double threshold, do
   const long long cs = (long long)c->nc_run * c->npts;
   double2* p = c->psi;
   ATUS_TRY(full_ft(c, -1, p, p, c->nbatch, cs));
-  RedRabi fn{};
-  fn.psik = p;
-  fn.gr = c->grid();
-  fn.nk0 = n;
-  for (int q = 0; q < n; ++q)
-    for (int a = 0; a < 3; ++a) fn.k0[q][a] = k0[3 * q + a];
-  fn.thr = threshold;
-  const unsigned nb = seg_blocks(c->npts, c->nbatch);
+  const RabiReduction rr = rabi_reduction(c, p, k0, n, threshold);
+  const unsigned nb = seg_blocks(rr.npts, c->nbatch);
   if (c->sred_cap < (size_t)c->nbatch * nb * RED_MAX_NV) return fail(ATUS_ERR_STATE, "reduction buffers not sized");
-  ATUS_TRY(reduce_seg_launch(c, fn, c->npts, cs, c->nbatch, nb, c->sred_part, res));
+  if (rr.use_box)
+    ATUS_TRY(reduce_seg_launch(c, rr.box, rr.npts, cs, c->nbatch, nb, c->sred_part, res));
+  else
+    ATUS_TRY(reduce_seg_launch(c, rr.full, c->npts, cs, c->nbatch, nb, c->sred_part, res));
   ATUS_TRY(full_ft(c, +1, p, p, c->nbatch, cs));
   return ATUS_OK;
 }

@@ -68,7 +68,9 @@ struct atus_ctx {
   double2* scratch = nullptr;
   std::map<int, double2*> tw;
 
-  double2* kin = nullptr;   // [2 fracs][3 axes][nmax]
+  // kinetic phase tables: half step, full step, half step of one axis alone
+  static constexpr int KIN_SLOTS_N = 3;
+  double2* kin = nullptr;   // [KIN_SLOTS_N][3 axes][nmax]
   int kin_nmax = 0;
   double kin_dt = NAN;
   bool kin_valid = false;

@@ -106,7 +106,30 @@ struct Plan {
   static constexpr int S = A + (Q > 1 ? 1 : 0);        // number of stages
   template <int s> static constexpr int radix() { return s < A ? R : Q; }
   template <int s> static constexpr int span() { return ipow(R, s); }  // Ls of stage s
+  // start of stage s's twiddles in the stage-major table (stage 0 has none)
+  template <int s> static constexpr int tw_off() {
+    if constexpr (s <= 1) return 0;
+    else return tw_off<s - 1>() + (radix<s - 1>() - 1) * span<s - 1>();
+  }
 };
+
+// Twiddle tables hold, behind the natural table W^j (j < N), one
+// stage-major table per plan (values per thread 8, then 16): for every
+// stage s >= 1 of the plan, entry tw_off<s>() + (m - 1) Ls + k is
+// W^(k m N / (Ls r)), m = 1..r-1, k = 0..Ls-1 -- the values the stage reads,
+// in the order the threads of a wavefront read them (consecutive k), which
+// spreads them over the LDS banks and over fewer cache lines.  The entries
+// of a plan sum to N - 1.
+template <int RM>
+__host__ __device__ constexpr int tw_table_slot() {
+  static_assert(RM == 8 || RM == 16, "plans of 8 or 16 values per thread");
+  return RM == 8 ? 1 : 2;
+}
+constexpr int TW_TABLES = 3;  // natural, then the plans of 8 and 16
+template <int N, int RM>
+__device__ __forceinline__ const double2* fft_tw(const double2* tw) {
+  return tw + (size_t)N * tw_table_slot<RM>();
+}
 
 // LDS position of pencil element j: XOR swizzle inside rows of 16 elements
 // (16 B each) so the stride-16 scatter of the first stage hits distinct banks
@@ -123,9 +146,24 @@ __device__ __forceinline__ void lds_sync() {
   }
 }
 
-// compute of stage s (twiddle + r-point DFTs); twiddle table tw[j] = e^{-2 pi i j / N}
-template <int N, int RM, int s, int DIR>
-__device__ __forceinline__ void stage_compute(double2 (&v)[(Plan<N, RM>::R)], int t, const double2* __restrict__ tw) {
+// element i of a twiddle table in LDS or (address-space qualified) global
+// memory
+typedef double __attribute__((ext_vector_type(2))) tw_vec2;
+template <class TW>
+__device__ __forceinline__ double2 load_tw(TW p, int i) {
+  if constexpr (__is_same(TW, const double2*)) {
+    return p[i];
+  } else {
+    typedef const __attribute__((address_space(1))) tw_vec2* gvec;
+    const tw_vec2 w = ((gvec)p)[i];
+    return make_double2(w.x, w.y);
+  }
+}
+
+// compute of stage s (twiddle + r-point DFTs); tw: the plan's stage-major
+// twiddle table (fft_tw)
+template <int N, int RM, int s, int DIR, class TW>
+__device__ __forceinline__ void stage_compute(double2 (&v)[(Plan<N, RM>::R)], int t, TW tw) {
   using PL = Plan<N, RM>;
   constexpr int R = PL::R, P = PL::P;
   constexpr int r = PL::template radix<s>();
@@ -139,10 +177,10 @@ __device__ __forceinline__ void stage_compute(double2 (&v)[(Plan<N, RM>::R)], in
     if constexpr (Ls > 1) {
       const int b = t + q * P;
       const int k = b & (Ls - 1);
-      constexpr int step = N / (Ls * r);
+      const TW tws = tw + PL::template tw_off<s>() + k;
 #pragma unroll
       for (int m = 1; m < r; ++m) {
-        double2 w = tw[k * m * step];
+        double2 w = load_tw(tws, (m - 1) * Ls);
         x[m] = (DIR < 0) ? cmul(x[m], w) : cmulc(x[m], w);
       }
     }
@@ -175,22 +213,22 @@ __device__ __forceinline__ void stage_exchange(double2 (&v)[(Plan<N, RM>::R)], i
   for (int i = 0; i < R; ++i) v[i] = lds[lds_pos(t + P * i)];
 }
 
-template <int N, int RM, int s, int DIR, bool WAVE_ONLY>
-__device__ __forceinline__ void run_stages(double2 (&v)[(Plan<N, RM>::R)], int t, double2* lds,
-                                           const double2* __restrict__ tw) {
+template <int N, int RM, int s, int DIR, bool WAVE_ONLY, class TW>
+__device__ __forceinline__ void run_stages(double2 (&v)[(Plan<N, RM>::R)], int t, double2* lds, TW tw) {
   stage_compute<N, RM, s, DIR>(v, t, tw);
   if constexpr (s + 1 < Plan<N, RM>::S) {
     stage_exchange<N, RM, s, WAVE_ONLY>(v, t, lds);
-    run_stages<N, RM, s + 1, DIR, WAVE_ONLY>(v, t, lds, tw);
+    run_stages<N, RM, s + 1, DIR, WAVE_ONLY, TW>(v, t, lds, tw);
   }
 }
 
 // full unnormalised N-point DFT of the pencil held in registers (layout
-// t + P*i in and out).  lds: this pencil's N-element region.
-template <int N, int RM, int DIR, bool WAVE_ONLY>
-__device__ __forceinline__ void fft_pencil(double2 (&v)[(Plan<N, RM>::R)], int t, double2* lds,
-                                           const double2* __restrict__ tw) {
-  if constexpr (N > 1) run_stages<N, RM, 0, DIR, WAVE_ONLY>(v, t, lds, tw);
+// t + P*i in and out).  lds: this pencil's region of at least N elements;
+// tw: the plan's stage-major twiddle table (fft_tw), a pointer to LDS or
+// global memory.
+template <int N, int RM, int DIR, bool WAVE_ONLY, class TW>
+__device__ __forceinline__ void fft_pencil(double2 (&v)[(Plan<N, RM>::R)], int t, double2* lds, TW tw) {
+  if constexpr (N > 1) run_stages<N, RM, 0, DIR, WAVE_ONLY, TW>(v, t, lds, tw);
 }
 
 }  // namespace atus

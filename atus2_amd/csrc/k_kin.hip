@@ -28,8 +28,8 @@ static hipError_t kin_nr(bool contig, int vk, int nc, double2* psi, long long cs
     // columns of 2048 and more: workgroups of up to 512 threads hold twice
     // the columns, so each row is read in segments twice as long
     if constexpr (N >= 2048)
-      return kin_go<N, RM, false, V_NONE, 1, 512>(psi, cs, ncomp, inner, 1, tw, kt, va, dx0, max_lds, s);
-    return kin_go<N, RM, false, V_NONE, 1>(psi, cs, ncomp, inner, 1, tw, kt, va, dx0, max_lds, s);
+      return kin_go<N, RM, false, V_NONE, 1, 512>(psi, cs, ncomp, inner, rows, tw, kt, va, dx0, max_lds, s);
+    return kin_go<N, RM, false, V_NONE, 1>(psi, cs, ncomp, inner, rows, tw, kt, va, dx0, max_lds, s);
   }
   // 1D: the pencil is the whole grid
   if (vk == V_NONE) return kin_go<N, RM, true, V_NONE, 1>(psi, cs, ncomp, 1, rows, tw, kt, va, dx0, max_lds, s);

@@ -5,15 +5,35 @@ namespace atus {
 
 bool supported_len(int N) { return N >= 2 && N <= 4096 && (N & (N - 1)) == 0; }
 
+// LDS elements per pencil region.  LDS serves a 16-byte access of a
+// wavefront 16 lanes at a time from 64 banks of 4 bytes, i.e. 16 slots of
+// 16 bytes; within a pencil the exchange positions are XOR-swizzled in
+// rows of 16 elements, so 16 lanes of one pencil hit 16 distinct slots.
+// When 16 lanes span several pencils -- g pencils of 16/g lanes each (a
+// strided pass: lane = t*T + p, g = min(T, 16); a contiguous pass with P <
+// 16: g = 16/P) -- pencil regions 16/g slots apart (mod 16) keep them on
+// disjoint slots; with one pencil per 16 lanes one element of padding is
+// kept.
+static int pencil_lds_elems(int N, int P, bool contig, long long T) {
+  int g = 1;
+  if (contig) g = P < 16 ? 16 / P : 1;
+  else g = T < 16 ? (int)T : 16;
+  if (g > 16) g = 16;
+  const int pad = g > 1 ? 16 / g : 1;
+  return N + pad;
+}
+
 PassCfg make_pass_cfg(int N, int rm, long long inner, long long outer, int ycount, size_t max_lds, int ntables,
                       int max_threads) {
   PassCfg c{};
   const int P = threads_per_pencil(N, rm);
-  const size_t pencil_bytes = (size_t)(N + 1) * sizeof(double2);  // lds_stride<N>()
+  const bool contig = inner == 1;
   const size_t table_bytes = N <= 1024 ? (size_t)ntables * N * sizeof(double2) : 0;
   long long T = max_threads / P;
   if (T < 1) T = 1;
-  while (T > 1 && (size_t)T * pencil_bytes + table_bytes > max_lds) T >>= 1;
+  if (!contig && T > inner) T = inner;
+  auto pencil_bytes = [&](long long t) { return (size_t)pencil_lds_elems(N, P, contig, t) * sizeof(double2); };
+  while (T > 1 && (size_t)T * pencil_bytes(T) + table_bytes > max_lds) T >>= 1;
   if (inner == 1) {
     long long tiles = (outer + T - 1) / T;
     c.g.inner = 1;
@@ -22,7 +42,6 @@ PassCfg make_pass_cfg(int N, int rm, long long inner, long long outer, int ycoun
     c.g.ntiles = tiles;
     c.g.T = (int)T;
   } else {
-    if (T > inner) T = inner;
     c.g.inner = inner;
     c.g.outer = outer;
     c.g.col_tiles = (inner + T - 1) / T;
@@ -34,7 +53,8 @@ PassCfg make_pass_cfg(int N, int rm, long long inner, long long outer, int ycoun
   if (T * P > max_threads) T = 0;
   c.block = dim3((unsigned)(T * P));
   c.grid = dim3((unsigned)c.g.ntiles, (unsigned)ycount);
-  c.lds = (size_t)T * pencil_bytes + table_bytes;
+  c.g.ls = pencil_lds_elems(N, P, contig, T);
+  c.lds = (size_t)T * pencil_bytes(T) + table_bytes;
   if (c.lds > max_lds) c.block = dim3(0);  // rejected by the launchers
   return c;
 }

@@ -53,6 +53,7 @@ struct AxisGeo {
   long long col_tiles;  // strided: inner / T tiles per slab
   long long ntiles;     // gridDim.x
   int T;                // pencils per workgroup
+  int ls;               // LDS elements per pencil region (N plus padding)
 };
 
 // per-run parameters of a batch of independent two-component runs: the
@@ -129,14 +130,6 @@ __device__ __forceinline__ void sincos_fast(double x, double* s, double* c) {
   }
 }
 
-// LDS elements per pencil region: one element of padding shifts the
-// regions of neighbouring pencils by 16 B, so the pencils a wavefront
-// touches together do not map to the same banks (regions 4 KiB apart would)
-template <int N>
-__host__ __device__ constexpr int lds_stride() {
-  return N + 1;
-}
-
 // lookup tables (twiddles, kinetic phases) of length N are staged in LDS
 // behind the pencil regions when they are small enough
 template <int N>
@@ -152,6 +145,22 @@ __device__ __forceinline__ const double2* stage_table(double2* lds_tab, const do
     return lds_tab;
   } else {
     return src;
+  }
+}
+
+// The global twiddle table (N > 1024) as a pointer the compiler cannot
+// prove equal to earlier ones: the loads of a transform are not merged with
+// the identical loads of the transform before it, which would keep all of
+// its twiddles live in registers in between.  The table stays a global
+// (not a flat) address.
+typedef const __attribute__((address_space(1))) double2* global_double2_ptr;
+template <int N>
+__device__ __forceinline__ auto fresh_tw(const double2* p) {
+  if constexpr (!tables_in_lds<N>()) {
+    asm volatile("" : "+s"(p));
+    return (global_double2_ptr)p;
+  } else {
+    return p;
   }
 }
 
@@ -665,8 +674,8 @@ __global__ __launch_bounds__(256) void k_pass(const double2* src, double2* dst,
   extern __shared__ double2 lds_all[];
   constexpr int R = Plan<N, RM>::R;
   PencilMap<N, RM, CONTIG> m(g);
-  double2* lds = lds_all + m.p * lds_stride<N>();
-  const double2* twl = stage_table<N>(lds_all + (size_t)g.T * lds_stride<N>(), tw);
+  double2* lds = lds_all + m.p * g.ls;
+  const double2* twl = stage_table<N>(lds_all + (size_t)g.T * g.ls, fft_tw<N, RM>(tw));
   const long long coff = (long long)blockIdx.y * comp_stride;
   double2 v[R];
   load_pencil<N, RM>(v, src + coff, m.base, m.stride, m.t, m.valid);
@@ -709,15 +718,15 @@ __global__ __launch_bounds__(LB) void k_kin(double2* __restrict__ psi, long long
   constexpr int R = Plan<N, RM>::R, P = Plan<N, RM>::P;
   constexpr bool WAVE = CONTIG && (P <= 64);
   PencilMap<N, RM, CONTIG> m(g);
-  double2* lds = lds_all + m.p * lds_stride<N>();
-  const double2* twl = stage_table<N>(lds_all + (size_t)g.T * lds_stride<N>(), tw);
+  double2* lds = lds_all + m.p * g.ls;
+  const double2* twl = stage_table<N>(lds_all + (size_t)g.T * g.ls, fft_tw<N, RM>(tw));
   // the component of this block (one per blockIdx.y) may belong to the
   // second species, with its own tables (workgroup-uniform choice)
   const bool sp2 = NC == 1 && kt.f0 != nullptr && (int)blockIdx.y >= kt.split;
   const double2* t0 = sp2 ? kt.f0 : kt.e0;
   const double2* t1t = sp2 ? kt.f1 : kt.e1;
   const double2* t2t = sp2 ? kt.f2 : kt.e2;
-  const double2* e0l = stage_table<N>(lds_all + (size_t)g.T * lds_stride<N>() + N, t0);
+  const double2* e0l = stage_table<N>(lds_all + (size_t)g.T * g.ls + N, t0);
   // pencil constant of the phase (axes 1, 2)
   double2 cp = make_double2(1.0, 0.0);
   if constexpr (!CONTIG) {
@@ -758,7 +767,7 @@ __global__ __launch_bounds__(LB) void k_kin(double2* __restrict__ psi, long long
   if constexpr (tables_in_lds<N>()) __syncthreads();
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
-    fft_pencil<N, RM, -1, WAVE>(v[c], m.t, lds, twl);
+    fft_pencil<N, RM, -1, WAVE>(v[c], m.t, lds, fresh_tw<N>(twl));
     const double2* ezr = nullptr;
     if constexpr (CONTIG) {
       if (kt.ez) ezr = kt.ez + (m.pid % kt.ez_rows) * N;
@@ -773,7 +782,7 @@ __global__ __launch_bounds__(LB) void k_kin(double2* __restrict__ psi, long long
       }
       v[c][i] = cmul(v[c][i], e);
     }
-    fft_pencil<N, RM, +1, WAVE>(v[c], m.t, lds, twl);
+    fft_pencil<N, RM, +1, WAVE>(v[c], m.t, lds, fresh_tw<N>(twl));
     store_pencil<N, RM>(v[c], psi + c0 + c * comp_stride, m.base, m.stride, m.t, m.valid);
   }
 }
@@ -799,9 +808,9 @@ __global__ __launch_bounds__(LB) void k_block1d(double2* __restrict__ psi, long 
   constexpr int R = Plan<N, RM>::R, P = Plan<N, RM>::P;
   constexpr bool WAVE = P <= 64;
   PencilMap<N, RM, true> m(g);
-  double2* lds = lds_all + m.p * lds_stride<N>();
-  double2* tab = lds_all + (size_t)g.T * lds_stride<N>();
-  const double2* twl = stage_table<N>(tab, tw);
+  double2* lds = lds_all + m.p * g.ls;
+  double2* tab = lds_all + (size_t)g.T * g.ls;
+  const double2* twl = stage_table<N>(tab, fft_tw<N, RM>(tw));
   const double2* ehl = stage_table<N>(tab + N, eh);
   const double2* efl = stage_table<N>(tab + 2 * N, ef);
   long long c0 = 0;
@@ -819,10 +828,10 @@ __global__ __launch_bounds__(LB) void k_block1d(double2* __restrict__ psi, long 
   auto kinetic = [&](const double2* e) {
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
-      fft_pencil<N, RM, -1, WAVE>(v[c], m.t, lds, twl);
+      fft_pencil<N, RM, -1, WAVE>(v[c], m.t, lds, fresh_tw<N>(twl));
 #pragma unroll
       for (int i = 0; i < R; ++i) v[c][i] = cmul(v[c][i], e[m.t + P * i]);
-      fft_pencil<N, RM, +1, WAVE>(v[c], m.t, lds, twl);
+      fft_pencil<N, RM, +1, WAVE>(v[c], m.t, lds, fresh_tw<N>(twl));
     }
   };
   kinetic(ehl);
@@ -858,16 +867,20 @@ __global__ __launch_bounds__(LB) void k_block1d(double2* __restrict__ psi, long 
 // inverse DFT along the last axis, V step, forward DFT along the last axis.
 // The pencil (contiguous) has fixed coordinates on the other axes, so
 // x0 and the Bragg field are per-pencil constants.
-template <int N, int RM, int VK, int NC, bool BATCH = false>
+// EDGE: the first and last steps of a block also carry the last axis's
+// share of the half kinetic step (forward DFT, times kz[j], inverse DFT;
+// kz includes this axis's 1/N): before the V step when edge & 1 (the pencil
+// arrives in x space), after it when edge & 2 (the pencil leaves in x space).
+template <int N, int RM, int VK, int NC, bool BATCH = false, bool EDGE = false>
 __global__ __launch_bounds__(256) void k_vfuse(double2* __restrict__ psi, long long comp_stride, AxisGeo g,
                                                const double2* __restrict__ tw, VArgs va, long long pencils_per_x0,
-                                               int n0, double dx0) {
+                                               int n0, double dx0, const double2* __restrict__ kz, int edge) {
   extern __shared__ double2 lds_all[];
   constexpr int R = Plan<N, RM>::R, P = Plan<N, RM>::P;
   constexpr bool WAVE = P <= 64;
   PencilMap<N, RM, true> m(g);
-  double2* lds = lds_all + m.p * lds_stride<N>();
-  const double2* twl = stage_table<N>(lds_all + (size_t)g.T * lds_stride<N>(), tw);
+  double2* lds = lds_all + m.p * g.ls;
+  const double2* twl = stage_table<N>(lds_all + (size_t)g.T * g.ls, fft_tw<N, RM>(tw));
   long long pid = m.pid;
   if constexpr (BATCH) {
     const long long r = m.pid / va.pencils_per_run;
@@ -884,8 +897,22 @@ __global__ __launch_bounds__(256) void k_vfuse(double2* __restrict__ psi, long l
 #pragma unroll
   for (int c = 0; c < NC; ++c) load_pencil<N, RM>(v[c], psi + c * comp_stride, m.base, 1, m.t, m.valid);
   if constexpr (tables_in_lds<N>()) __syncthreads();
+  // the half kinetic phase along this axis, on the spectrum of component c
+  auto kinetic_phase = [&](int c) {
 #pragma unroll
-  for (int c = 0; c < NC; ++c) fft_pencil<N, RM, +1, WAVE>(v[c], m.t, lds, twl);
+    for (int i = 0; i < R; ++i) v[c][i] = cmul(v[c][i], kz[m.t + P * i]);
+  };
+  if constexpr (EDGE) {
+    if (edge & 1) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        fft_pencil<N, RM, -1, WAVE>(v[c], m.t, lds, fresh_tw<N>(twl));
+        kinetic_phase(c);
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < NC; ++c) fft_pencil<N, RM, +1, WAVE>(v[c], m.t, lds, fresh_tw<N>(twl));
 #pragma unroll
   for (int i = 0; i < R; ++i) {
     double2 a[NC];
@@ -897,7 +924,13 @@ __global__ __launch_bounds__(256) void k_vfuse(double2* __restrict__ psi, long l
   }
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
-    fft_pencil<N, RM, -1, WAVE>(v[c], m.t, lds, twl);
+    fft_pencil<N, RM, -1, WAVE>(v[c], m.t, lds, fresh_tw<N>(twl));
+    if constexpr (EDGE) {
+      if (edge & 2) {
+        kinetic_phase(c);
+        fft_pencil<N, RM, +1, WAVE>(v[c], m.t, lds, fresh_tw<N>(twl));
+      }
+    }
     store_pencil<N, RM>(v[c], psi + c * comp_stride, m.base, 1, m.t, m.valid);
   }
 }
@@ -923,8 +956,8 @@ __global__ __launch_bounds__(256) void k_fs_col(double2* __restrict__ psi, long 
   extern __shared__ double2 lds_all[];
   constexpr int R = Plan<N1, RM>::R, P = Plan<N1, RM>::P;
   PencilMap<N1, RM, false> m(g);
-  double2* lds = lds_all + m.p * lds_stride<N1>();
-  const double2* twl = stage_table<N1>(lds_all + (size_t)g.T * lds_stride<N1>(), tw1);
+  double2* lds = lds_all + m.p * g.ls;
+  const double2* twl = stage_table<N1>(lds_all + (size_t)g.T * g.ls, fft_tw<N1, RM>(tw1));
   const long long N2 = g.inner;
   const long long N = N2 * N1;
   const long long j2 = m.pid % N2;
@@ -1010,8 +1043,8 @@ __global__ __launch_bounds__(256) void k_sobf(const double2* src, double2* out_s
   constexpr int R = Plan<N, RM>::R, P = Plan<N, RM>::P;
   constexpr bool WAVE = CONTIG && (P <= 64);
   PencilMap<N, RM, CONTIG> m(g);
-  double2* lds = lds_all + m.p * lds_stride<N>();
-  const double2* twl = stage_table<N>(lds_all + (size_t)g.T * lds_stride<N>(), tw);
+  double2* lds = lds_all + m.p * g.ls;
+  const double2* twl = stage_table<N>(lds_all + (size_t)g.T * g.ls, fft_tw<N, RM>(tw));
   const long long coff = (long long)blockIdx.y * comp_stride;
   double k12 = 0.0;  // sum over axes 1, 2 of k^2 (k.scale(alpha).k with isotropic alpha)
   if constexpr (!CONTIG) {
@@ -1065,8 +1098,8 @@ __global__ __launch_bounds__(256) void k_sob_gradfuse(const double2* __restrict_
   constexpr int R = Plan<N, RM>::R, P = Plan<N, RM>::P;
   constexpr bool WAVE = P <= 64;
   PencilMap<N, RM, true> m(g);
-  double2* lds = lds_all + m.p * lds_stride<N>();
-  const double2* twl = stage_table<N>(lds_all + (size_t)g.T * lds_stride<N>(), tw);
+  double2* lds = lds_all + m.p * g.ls;
+  const double2* twl = stage_table<N>(lds_all + (size_t)g.T * g.ls, fft_tw<N, RM>(tw));
   const long long off = (long long)comp * npts;
   double2 v[R];
   load_pencil<N, RM>(v, lap + off, m.base, 1, m.t, m.valid);
@@ -1337,6 +1370,54 @@ struct RedRabi {
     const double d = a.x * a.x + a.y * a.y;
     double k[3];
     k_of(l, gr, k);
+    int j = 0;
+    for (int q = 0; q < nk0; ++q) {
+      double dd = 0.0;
+      for (int ax = 0; ax < gr.dim; ++ax) {
+        const double e = k0[q][ax] - k[ax];
+        dd += e * e;
+      }
+      if (sqrt(dd) < thr) {
+#pragma unroll
+        for (int s = 0; s < RED_MAX_NV; ++s)
+          if (s == j) acc[s] += d;
+        continue;
+      }
+      ++j;
+    }
+  }
+};
+
+// RedRabi over the index box that holds every ball |k - k0_q| < thr: box
+// point l has centred indices m_a = lo[a] + (its offset along axis a), so
+// k_a = dk_a m_a is kval's value at grid index m_a mod n_a; points outside
+// the box lie outside every ball and add nothing.  The membership test and
+// the accumulation are RedRabi's.
+struct RedRabiBox {
+  static constexpr int NV = RED_MAX_NV;
+  const double2* psik;
+  Grid gr;
+  int lo[3];
+  int len[3];  // box extent per axis (1 beyond the grid's dimension)
+  int nk0;
+  double k0[RED_MAX_NV][3];
+  double thr;
+  __device__ void shift(long long off) { psik += off; }
+  __device__ void operator()(long long l, double (&acc)[RED_MAX_NV]) const {
+    const unsigned u = (unsigned)l;  // boxes hold fewer than 2^31 points
+    const unsigned q12 = (unsigned)len[1] * (unsigned)len[2];
+    const unsigned b0 = u / q12, r = u - b0 * q12;
+    const unsigned b1 = r / (unsigned)len[2], b2 = r - b1 * (unsigned)len[2];
+    int m[3] = {lo[0] + (int)b0, lo[1] + (int)b1, lo[2] + (int)b2};
+    int i[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) i[a] = m[a] < 0 ? m[a] + gr.n[a] : m[a];
+    const double2 v = psik[((long long)i[0] * gr.n[1] + i[1]) * gr.n[2] + i[2]];
+    const double d = v.x * v.x + v.y * v.y;
+    double k[3];
+    k[0] = gr.dk[0] * (double)m[0];
+    k[1] = gr.dim > 1 ? gr.dk[1] * (double)m[1] : 0.0;
+    k[2] = gr.dim > 2 ? gr.dk[2] * (double)m[2] : 0.0;
     int j = 0;
     for (int q = 0; q < nk0; ++q) {
       double dd = 0.0;

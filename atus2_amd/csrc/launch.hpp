@@ -61,24 +61,30 @@ hipError_t launch_pass(int N, int rm, bool contig, int dir, const double2* src, 
                        int ncomp, long long inner, long long outer, const double2* tw, double scale,
                        const Limits& max_lds, hipStream_t s);
 
-// kinetic step along axis 0; vk/nc select the fused V prologue (1D only)
+// kinetic step along axis 0; vk/nc select the fused V prologue (1D only).
+// Strided pencils (contig false): `inner` points apart, `rows` blocks of
+// inner * N points (1 for axis 0, the planes of a slab for the middle axis)
 hipError_t launch_kin(int N, int rm, bool contig, int vk, int nc, double2* psi, long long comp_stride, int ncomp,
                       long long inner, const double2* tw, const KinTabs& kt, const VArgs& va, double dx0,
                       const Limits& max_lds, hipStream_t s, long long rows = 1);
 
-// inverse / V / forward along the last (contiguous) axis of a 2D/3D grid
+// inverse / V / forward along the last (contiguous) axis of a 2D/3D grid;
+// edge (bit 0: before, bit 1: after the V step) adds this axis's half
+// kinetic step with the phase table kz (k_vfuse's EDGE form)
 hipError_t launch_vfuse(int N, int rm, int vk, int nc, double2* psi, long long comp_stride, long long outer,
                         const double2* tw, const VArgs& va, long long pencils_per_x0, int n0, double dx0,
-                        const Limits& max_lds, hipStream_t s);
+                        const Limits& max_lds, hipStream_t s, const double2* kz = nullptr, int edge = 0);
 
 // V step of a batch of independent two-component runs (VArgs::runs, ::Fm,
 // ::pencils_per_run, ::run_shift set).  one_d: the fused 1D step over
 // `count` runs (the kinetic phase of slot kt follows the V step); otherwise
 // the contiguous-axis pass (inverse DFT, V, forward DFT) over `count`
 // pencils of all runs.
+// edge: as launch_vfuse (2D/3D only)
 hipError_t launch_vstep_batch(int N, int rm, bool one_d, int vk, double2* psi, long long comp_stride, long long count,
                               const double2* tw, const KinTabs& kt, const VArgs& va, long long pencils_per_x0, int n0,
-                              double dx0, const Limits& max_lds, hipStream_t s);
+                              double dx0, const Limits& max_lds, hipStream_t s, const double2* kz = nullptr,
+                              int edge = 0);
 
 // a whole block of a 1D grid (N <= 4096) in one launch: one run (nc
 // components, runs = 1) or a batch of `runs` two-component runs; eh / ef

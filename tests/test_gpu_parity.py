@@ -258,8 +258,11 @@ def test_reductions_match_oracle(hip, orc):
         assert np.allclose(ctx.expval_k(0), ek_ref, rtol=1e-12, atol=1e-13)
         K = 2 * np.pi / 16.0 * 3
         k0 = [(0, 0, 0), (K, 0, 0), (-K, 0, 0)]
-        rb_ref, _ = orc.rabi_integrals(g, psi, k0, 1.0)
-        assert np.allclose(ctx.rabi_integrals(0, k0, 1.0), rb_ref, rtol=1e-12, atol=1e-14)
+        # a small radius sums over the index box of the balls, a radius
+        # whose box covers more than half the grid over every point
+        for thr in (1.0, 6.0):
+            rb_ref, _ = orc.rabi_integrals(g, psi, k0, thr)
+            assert np.allclose(ctx.rabi_integrals(0, k0, thr), rb_ref, rtol=1e-12, atol=1e-14), thr
         E_ref = orc.energy(g, [psi, psi2], gs, [0.5, 0.5, 0.5], [V, V])
         assert abs(ctx.energy() / E_ref - 1) < 1e-12
         # psi is untouched by the reductions that transform it
