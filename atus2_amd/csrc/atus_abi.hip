@@ -569,6 +569,75 @@ RabiReduction rabi_reduction(const atus_ctx* c, const double2* psik, const doubl
   return r;
 }
 
+// compact-spectrum buffers of the 3D box Rabi integrals over nseg segments
+size_t rabi_box_elems(const atus_ctx* c, const RabiReduction& rr, int nseg) {
+  const long long n0 = c->n[0], n1 = c->n[1];
+  const long long l1 = rr.box.len[1], l2 = rr.box.len[2];
+  return (size_t)nseg * (size_t)(n0 * n1 * l2 + n0 * l1 * l2);
+}
+
+// whether the Rabi integrals of this context take the 3D box passes
+bool rabi_box3_ok(const atus_ctx* c, const RabiReduction& rr) { return c->dim == 3 && !c->fs && rr.use_box; }
+
+// the buffers of rabi_box3 (allocated outside any graph capture)
+int ensure_rabi_box(atus_ctx* c, const RabiReduction& rr, int nseg) {
+  const size_t need = rabi_box_elems(c, rr, nseg);
+  if (c->rbuf_cap < need) {
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    (void)hipFree(c->rbuf);
+    c->rbuf = nullptr;
+    c->rbuf_cap = 0;
+    if (hipMalloc(&c->rbuf, sizeof(double2) * need) != hipSuccess) return fail(ATUS_ERR_NOMEM, "Rabi spectrum buffers");
+    c->rbuf_cap = need;
+  }
+  for (int a = 0; a < 3; ++a) ATUS_TRY(ensure_twiddles(c, c->n[a]));
+  RabiBox rb{};
+  for (int a = 0; a < 3; ++a) rb.n[a] = c->n[a];
+  unsigned nb = 0;
+  rb.len[1] = rr.box.len[1];
+  rb.len[2] = rr.box.len[2];
+  HIP_TRY(launch_rabi_box(0, c->rm[ATUS_KFAM_PASS_STRIDED], nullptr, 0, nseg, nullptr, nullptr, c->tw[c->n[0]], rb,
+                          lim(c, ATUS_KFAM_PASS_STRIDED), nullptr, &nb, c->stream));
+  return ensure_sred(c, (size_t)nseg * nb * RED_MAX_NV, nseg);
+}
+
+// compute_rabi_integrals of nseg segments (`seg` elements apart from src)
+// of a 3D grid from the spectrum inside the balls' index box: the three
+// k_rabi passes and the segmented final sum into res (device, nseg *
+// RED_MAX_NV, before the factor ar_k); src is not modified.  The buffers
+// come from ensure_rabi_box.
+int rabi_box3(atus_ctx* c, const double2* src, long long seg, int nseg, const RabiReduction& rr, double* res) {
+  if (c->rbuf_cap < rabi_box_elems(c, rr, nseg)) return fail(ATUS_ERR_STATE, "Rabi spectrum buffers not sized");
+  RabiBox rb{};
+  for (int a = 0; a < 3; ++a) {
+    rb.n[a] = c->n[a];
+    rb.lo[a] = rr.box.lo[a];
+    rb.len[a] = rr.box.len[a];
+    rb.dk[a] = c->dk[a];
+  }
+  rb.nk0 = rr.box.nk0;
+  for (int q = 0; q < RED_MAX_NV; ++q)
+    for (int a = 0; a < 3; ++a) rb.k0[q][a] = rr.box.k0[q][a];
+  rb.thr = rr.box.thr;
+  rb.scale = ft_factor(c->dim, c->dx);
+  double2* zb = c->rbuf;
+  double2* yb = c->rbuf + (size_t)nseg * c->n[0] * c->n[1] * rb.len[2];
+  unsigned nb = 0;
+  HIP_TRY(launch_rabi_box(2, c->rm[ATUS_KFAM_PASS_CONTIG], src, seg, nseg, zb, yb, c->tw[c->n[2]], rb,
+                          lim(c, ATUS_KFAM_PASS_CONTIG), nullptr, nullptr, c->stream));
+  HIP_TRY(launch_rabi_box(1, c->rm[ATUS_KFAM_PASS_STRIDED], src, seg, nseg, zb, yb, c->tw[c->n[1]], rb,
+                          lim(c, ATUS_KFAM_PASS_STRIDED), nullptr, nullptr, c->stream));
+  HIP_TRY(launch_rabi_box(0, c->rm[ATUS_KFAM_PASS_STRIDED], src, seg, nseg, zb, yb, c->tw[c->n[0]], rb,
+                          lim(c, ATUS_KFAM_PASS_STRIDED), nullptr, &nb, c->stream));
+  if (c->sred_cap < (size_t)nseg * nb * RED_MAX_NV) return fail(ATUS_ERR_STATE, "reduction buffers not sized");
+  HIP_TRY(launch_rabi_box(0, c->rm[ATUS_KFAM_PASS_STRIDED], src, seg, nseg, zb, yb, c->tw[c->n[0]], rb,
+                          lim(c, ATUS_KFAM_PASS_STRIDED), c->sred_part, &nb, c->stream));
+  hipLaunchKernelGGL((k_reduce_seg_final<RED_MAX_NV>), dim3((unsigned)nseg), dim3(RED_THREADS), 0, c->stream,
+                     c->sred_part, (int)nb, res);
+  HIP_TRY(hipGetLastError());
+  return ATUS_OK;
+}
+
 int vkind_of(int kind) {
   switch (kind) {
     case ATUS_STEP_FREEPROP: return V_NL;
@@ -757,6 +826,7 @@ int atus_ctx_destroy(atus_ctx* c) {
   (void)hipFree(c->kin2);
   (void)hipFree(c->sred_part);
   (void)hipFree(c->sred_out);
+  (void)hipFree(c->rbuf);
   (void)hipFree(c->blk_dev);
   if (c->blk_host) (void)hipHostFree(c->blk_host);
   if (c->blk_ev) (void)hipEventDestroy(c->blk_ev);
@@ -1502,6 +1572,17 @@ int atus_rabi_integrals(atus_ctx* c, int comp, const double* k0, int n, double t
   ATUS_TRY(check_ctx(c));
   ATUS_TRY(check_comp(c, comp));
   if (n < 1 || n > RED_MAX_NV || !k0 || !out) return fail(ATUS_ERR_ARG, "need 1..8 momentum points");
+  {
+    const RabiReduction rr = rabi_reduction(c, c->psi + comp * c->npts, k0, n, threshold);
+    if (rabi_box3_ok(c, rr)) {
+      ATUS_TRY(ensure_rabi_box(c, rr, 1));
+      ATUS_TRY(rabi_box3(c, c->psi + comp * c->npts, c->npts, 1, rr, c->sred_out));
+      HIP_TRY(hipMemcpyAsync(c->red_host, c->sred_out, sizeof(double) * RED_MAX_NV, hipMemcpyDeviceToHost, c->stream));
+      HIP_TRY(hipStreamSynchronize(c->stream));
+      for (int q = 0; q < n; ++q) out[q] = c->ar_k() * c->red_host[q];
+      return ATUS_OK;
+    }
+  }
   ATUS_TRY(ensure_scratch(c));
   ATUS_TRY(full_ft(c, -1, c->psi + comp * c->npts, c->scratch, 1));
   const RabiReduction rr = rabi_reduction(c, c->scratch, k0, n, threshold);
@@ -2049,6 +2130,19 @@ int atus_batch_rabi_integrals(atus_ctx* c, int comp, const double* k0, int n, do
   if (n < 1 || n > RED_MAX_NV || !k0 || !out) return fail(ATUS_ERR_ARG, "need 1..8 momentum points");
   const long long cs = (long long)c->nc_run * c->npts;
   double2* p = c->psi + (long long)comp * c->npts;
+  {
+    const RabiReduction rr = rabi_reduction(c, p, k0, n, threshold);
+    if (rabi_box3_ok(c, rr)) {
+      ATUS_TRY(ensure_rabi_box(c, rr, c->nbatch));
+      ATUS_TRY(rabi_box3(c, p, cs, c->nbatch, rr, c->sred_out));
+      std::vector<double> s((size_t)c->nbatch * RED_MAX_NV);
+      HIP_TRY(hipMemcpyAsync(s.data(), c->sred_out, sizeof(double) * s.size(), hipMemcpyDeviceToHost, c->stream));
+      HIP_TRY(hipStreamSynchronize(c->stream));
+      for (int r = 0; r < c->nbatch; ++r)
+        for (int q = 0; q < n; ++q) out[(size_t)r * n + q] = c->ar_k() * s[(size_t)r * RED_MAX_NV + q];
+      return ATUS_OK;
+    }
+  }
   ATUS_TRY(full_ft(c, -1, p, p, c->nbatch, cs));
   const RabiReduction rr = rabi_reduction(c, p, k0, n, threshold);
   std::vector<double> s((size_t)c->nbatch * RED_MAX_NV, 0.0);
@@ -2238,6 +2332,10 @@ namespace {
 int batch_rabi_launch(atus_ctx* c, const double* k0, int n, double threshold, double* res) {
   const long long cs = (long long)c->nc_run * c->npts;
   double2* p = c->psi;
+  {
+    const RabiReduction rr = rabi_reduction(c, p, k0, n, threshold);
+    if (rabi_box3_ok(c, rr)) return rabi_box3(c, p, cs, c->nbatch, rr, res);
+  }
   ATUS_TRY(full_ft(c, -1, p, p, c->nbatch, cs));
   const RabiReduction rr = rabi_reduction(c, p, k0, n, threshold);
   const unsigned nb = seg_blocks(rr.npts, c->nbatch);
@@ -2317,6 +2415,10 @@ int atus_scan_graph_create(atus_ctx* const* ctxs, int nctx, int kind, double dt,
     if (rc == ATUS_OK && c->fs) rc = ensure_tbuf(c);
     if (rc == ATUS_OK && nk0 > 0)
       rc = ensure_sred(c, (size_t)c->nbatch * seg_blocks(c->npts, c->nbatch) * RED_MAX_NV, c->nbatch);
+    if (rc == ATUS_OK && nk0 > 0) {
+      const RabiReduction rr = rabi_reduction(c, c->psi, k0, nk0, threshold);
+      if (rabi_box3_ok(c, rr)) rc = ensure_rabi_box(c, rr, c->nbatch);
+    }
     if (rc != ATUS_OK) {
       const std::string m = g_err;
       return bail(rc, m);

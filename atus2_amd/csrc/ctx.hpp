@@ -95,6 +95,9 @@ struct atus_ctx {
   double* sred_out = nullptr;
   size_t sred_cap = 0;
   int sred_nseg = 0;
+  // compact spectra of the Rabi integrals of 3D grids (k_rabi_z / _y)
+  double2* rbuf = nullptr;
+  size_t rbuf_cap = 0;
 
   bool sob_ready = false;
   double sob_alpha[2] = {0, 0};

@@ -17,32 +17,31 @@ static hipError_t kinb_go(double2* psi, long long cs, long long nbatch, const do
   return hipGetLastError();
 }
 
-template <int N, int RM, int VK, bool EDGE>
+template <int N, int RM, int VK, int EDGE>
 static hipError_t vfb_go(double2* psi, long long cs, long long outer, const double2* tw, const VArgs& va,
-                         long long ppx, int n0, double dx0, const double2* kz, int edge, const Limits& max_lds,
-                         hipStream_t s) {
-  PassCfg c = make_pass_cfg(N, RM, 1, outer, 1, max_lds);
+                         long long ppx, int n0, double dx0, const double2* kz, const Limits& max_lds, hipStream_t s) {
+  PassCfg c = make_pass_cfg(N, RM, 1, outer, 1, max_lds, EDGE ? 2 : 1);
   if (c.block.x == 0 || c.block.x > 256 || c.grid.x == 0) return hipErrorInvalidConfiguration;
   static bool ok = false;
   hipError_t e = allow_lds(k_vfuse<N, RM, VK, 2, true, EDGE>, c.lds, ok);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL((k_vfuse<N, RM, VK, 2, true, EDGE>), c.grid, c.block, c.lds, s, psi, cs, c.g, tw, va, ppx, n0,
-                     dx0, kz, edge);
+                     dx0, kz);
   return hipGetLastError();
 }
 
-template <int N, int RM, bool EDGE>
+template <int N, int RM, int EDGE>
 static hipError_t batch_nr(bool one_d, int vk, double2* psi, long long cs, long long count, const double2* tw,
                            const KinTabs& kt, const VArgs& va, long long ppx, int n0, double dx0, const double2* kz,
-                           int edge, const Limits& max_lds, hipStream_t s) {
+                           const Limits& max_lds, hipStream_t s) {
   if (one_d) {
-    if constexpr (!EDGE) {
+    if constexpr (EDGE == 0) {
       if (vk == V_BRAGG) return kinb_go<N, RM, V_BRAGG>(psi, cs, count, tw, kt, va, dx0, max_lds, s);
       if (vk == V_NL_IF) return kinb_go<N, RM, V_NL_IF>(psi, cs, count, tw, kt, va, dx0, max_lds, s);
     }
   } else {
-    if (vk == V_BRAGG) return vfb_go<N, RM, V_BRAGG, EDGE>(psi, cs, count, tw, va, ppx, n0, dx0, kz, edge, max_lds, s);
-    if (vk == V_NL_IF) return vfb_go<N, RM, V_NL_IF, EDGE>(psi, cs, count, tw, va, ppx, n0, dx0, kz, edge, max_lds, s);
+    if (vk == V_BRAGG) return vfb_go<N, RM, V_BRAGG, EDGE>(psi, cs, count, tw, va, ppx, n0, dx0, kz, max_lds, s);
+    if (vk == V_NL_IF) return vfb_go<N, RM, V_NL_IF, EDGE>(psi, cs, count, tw, va, ppx, n0, dx0, kz, max_lds, s);
   }
   return hipErrorInvalidValue;
 }
@@ -54,13 +53,13 @@ static hipError_t batch_n(int rm, bool one_d, int vk, double2* psi, long long cs
   constexpr bool r8 = N > 8 && N / 8 <= 256;
   // block edges: the plan of 8 values per thread where it exists (as for
   // single runs)
-  if (edge) {
-    if constexpr (r8) return batch_nr<N, 8, true>(one_d, vk, psi, cs, count, tw, kt, va, ppx, n0, dx0, kz, edge, max_lds, s);
-    else return batch_nr<N, 16, true>(one_d, vk, psi, cs, count, tw, kt, va, ppx, n0, dx0, kz, edge, max_lds, s);
-  }
+  constexpr int RE = r8 ? 8 : 16;
+  if (edge == 1) return batch_nr<N, RE, 1>(one_d, vk, psi, cs, count, tw, kt, va, ppx, n0, dx0, kz, max_lds, s);
+  if (edge == 2) return batch_nr<N, RE, 2>(one_d, vk, psi, cs, count, tw, kt, va, ppx, n0, dx0, kz, max_lds, s);
+  if (edge == 3) return batch_nr<N, RE, 3>(one_d, vk, psi, cs, count, tw, kt, va, ppx, n0, dx0, kz, max_lds, s);
   if (rm == 8 && r8)
-    return batch_nr<N, 8, false>(one_d, vk, psi, cs, count, tw, kt, va, ppx, n0, dx0, nullptr, 0, max_lds, s);
-  return batch_nr<N, 16, false>(one_d, vk, psi, cs, count, tw, kt, va, ppx, n0, dx0, nullptr, 0, max_lds, s);
+    return batch_nr<N, 8, 0>(one_d, vk, psi, cs, count, tw, kt, va, ppx, n0, dx0, nullptr, max_lds, s);
+  return batch_nr<N, 16, 0>(one_d, vk, psi, cs, count, tw, kt, va, ppx, n0, dx0, nullptr, max_lds, s);
 }
 
 hipError_t launch_vstep_batch(int N, int rm, bool one_d, int vk, double2* psi, long long comp_stride, long long count,

@@ -113,4 +113,56 @@ hipError_t launch_pass(int N, int rm, bool contig, int dir, const double2* src, 
   }
 }
 
+// ------------------------------------------------ Rabi integrals of a 3D box
+
+template <int N, int RM>
+static hipError_t rabi_nr(int axis, const double2* src, long long seg, int nseg, double2* zb, double2* yb,
+                          const double2* tw, const RabiBox& rb, const Limits& lim, double* partials,
+                          unsigned* nblocks, hipStream_t s) {
+  PassCfg c;
+  if (axis == 2) c = make_pass_cfg(N, RM, 1, (long long)rb.n[0] * rb.n[1], nseg, lim);
+  else if (axis == 1) c = make_pass_cfg(N, RM, rb.len[2], rb.n[0], nseg, lim);
+  else c = make_pass_cfg(N, RM, (long long)rb.len[1] * rb.len[2], 1, nseg, lim);
+  if (c.block.x == 0 || c.block.x > 256 || c.grid.x == 0) return hipErrorInvalidConfiguration;
+  static bool ok[3] = {false, false, false};
+  hipError_t e;
+  if (axis == 2) {
+    e = allow_lds(k_rabi_z<N, RM>, c.lds, ok[0]);
+    if (e == hipSuccess) hipLaunchKernelGGL((k_rabi_z<N, RM>), c.grid, c.block, c.lds, s, src, seg, zb, c.g, tw, rb);
+  } else if (axis == 1) {
+    e = allow_lds(k_rabi_y<N, RM>, c.lds, ok[1]);
+    if (e == hipSuccess) hipLaunchKernelGGL((k_rabi_y<N, RM>), c.grid, c.block, c.lds, s, zb, yb, c.g, tw, rb);
+  } else {
+    if (nblocks) *nblocks = c.grid.x;
+    e = allow_lds(k_rabi_x<N, RM>, c.lds, ok[2]);
+    if (e == hipSuccess && partials)
+      hipLaunchKernelGGL((k_rabi_x<N, RM>), c.grid, c.block, c.lds, s, yb, c.g, tw, rb, partials);
+  }
+  if (e != hipSuccess) return e;
+  return hipGetLastError();
+}
+
+template <int N>
+static hipError_t rabi_n(int rm, int axis, const double2* src, long long seg, int nseg, double2* zb, double2* yb,
+                         const double2* tw, const RabiBox& rb, const Limits& lim, double* partials, unsigned* nblocks,
+                         hipStream_t s) {
+  if (rm == 8 && N > 8 && N / 8 <= 256)
+    return rabi_nr<N, 8>(axis, src, seg, nseg, zb, yb, tw, rb, lim, partials, nblocks, s);
+  return rabi_nr<N, 16>(axis, src, seg, nseg, zb, yb, tw, rb, lim, partials, nblocks, s);
+}
+
+hipError_t launch_rabi_box(int axis, int rm, const double2* src, long long seg, int nseg, double2* zb, double2* yb,
+                           const double2* tw, const RabiBox& rb, const Limits& lim, double* partials,
+                           unsigned* nblocks, hipStream_t s) {
+  if (axis < 0 || axis > 2) return hipErrorInvalidValue;
+  switch (rb.n[axis]) {
+#define ATUS_CASE(n) \
+  case n: return rabi_n<n>(rm, axis, src, seg, nseg, zb, yb, tw, rb, lim, partials, nblocks, s);
+    ATUS_CASE(2) ATUS_CASE(4) ATUS_CASE(8) ATUS_CASE(16) ATUS_CASE(32) ATUS_CASE(64) ATUS_CASE(128)
+    ATUS_CASE(256) ATUS_CASE(512) ATUS_CASE(1024) ATUS_CASE(2048) ATUS_CASE(4096)
+#undef ATUS_CASE
+    default: return hipErrorInvalidValue;
+  }
+}
+
 }  // namespace atus

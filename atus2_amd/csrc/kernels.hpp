@@ -869,12 +869,13 @@ __global__ __launch_bounds__(LB) void k_block1d(double2* __restrict__ psi, long 
 // x0 and the Bragg field are per-pencil constants.
 // EDGE: the first and last steps of a block also carry the last axis's
 // share of the half kinetic step (forward DFT, times kz[j], inverse DFT;
-// kz includes this axis's 1/N): before the V step when edge & 1 (the pencil
-// arrives in x space), after it when edge & 2 (the pencil leaves in x space).
-template <int N, int RM, int VK, int NC, bool BATCH = false, bool EDGE = false>
+// kz includes this axis's 1/N): before the V step when EDGE & 1 (the pencil
+// arrives in x space), after it when EDGE & 2 (the pencil leaves in x
+// space).  kz is staged in LDS behind the twiddles when N <= 1024.
+template <int N, int RM, int VK, int NC, bool BATCH = false, int EDGE = 0>
 __global__ __launch_bounds__(256) void k_vfuse(double2* __restrict__ psi, long long comp_stride, AxisGeo g,
                                                const double2* __restrict__ tw, VArgs va, long long pencils_per_x0,
-                                               int n0, double dx0, const double2* __restrict__ kz, int edge) {
+                                               int n0, double dx0, const double2* __restrict__ kz) {
   extern __shared__ double2 lds_all[];
   constexpr int R = Plan<N, RM>::R, P = Plan<N, RM>::P;
   constexpr bool WAVE = P <= 64;
@@ -893,6 +894,8 @@ __global__ __launch_bounds__(256) void k_vfuse(double2* __restrict__ psi, long l
   double Om = 0.0;
   double2 eta = make_double2(1.0, 0.0);
   if constexpr (VK == V_BRAGG) bragg_field(x0, va, Om, eta);
+  const double2* kzl = nullptr;
+  if constexpr (EDGE != 0) kzl = stage_table<N>(lds_all + (size_t)g.T * g.ls + N, kz);
   double2 v[NC][R];
 #pragma unroll
   for (int c = 0; c < NC; ++c) load_pencil<N, RM>(v[c], psi + c * comp_stride, m.base, 1, m.t, m.valid);
@@ -900,15 +903,13 @@ __global__ __launch_bounds__(256) void k_vfuse(double2* __restrict__ psi, long l
   // the half kinetic phase along this axis, on the spectrum of component c
   auto kinetic_phase = [&](int c) {
 #pragma unroll
-    for (int i = 0; i < R; ++i) v[c][i] = cmul(v[c][i], kz[m.t + P * i]);
+    for (int i = 0; i < R; ++i) v[c][i] = cmul(v[c][i], kzl[m.t + P * i]);
   };
-  if constexpr (EDGE) {
-    if (edge & 1) {
+  if constexpr ((EDGE & 1) != 0) {
 #pragma unroll
-      for (int c = 0; c < NC; ++c) {
-        fft_pencil<N, RM, -1, WAVE>(v[c], m.t, lds, fresh_tw<N>(twl));
-        kinetic_phase(c);
-      }
+    for (int c = 0; c < NC; ++c) {
+      fft_pencil<N, RM, -1, WAVE>(v[c], m.t, lds, fresh_tw<N>(twl));
+      kinetic_phase(c);
     }
   }
 #pragma unroll
@@ -925,11 +926,9 @@ __global__ __launch_bounds__(256) void k_vfuse(double2* __restrict__ psi, long l
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     fft_pencil<N, RM, -1, WAVE>(v[c], m.t, lds, fresh_tw<N>(twl));
-    if constexpr (EDGE) {
-      if (edge & 2) {
-        kinetic_phase(c);
-        fft_pencil<N, RM, +1, WAVE>(v[c], m.t, lds, fresh_tw<N>(twl));
-      }
+    if constexpr ((EDGE & 2) != 0) {
+      kinetic_phase(c);
+      fft_pencil<N, RM, +1, WAVE>(v[c], m.t, lds, fresh_tw<N>(twl));
     }
     store_pencil<N, RM>(v[c], psi + c * comp_stride, m.base, 1, m.t, m.valid);
   }
@@ -1357,6 +1356,29 @@ struct RedMom2 {
 
 // Rabi momentum bins (CRT_Base_IF::compute_rabi_integrals, CRT_Base_IF.h:
 // 240-256), including the slot counter that only advances on a miss
+// one point of compute_rabi_integrals (the reference's accumulation): |psi^|^2
+// = d at wave vector k goes to acc[j], j counting the balls k0_q (radius
+// thr) passed over before the first one that holds k
+__device__ __forceinline__ void rabi_accumulate(const double (&k)[3], double d, int dim, int nk0,
+                                                const double (&k0)[RED_MAX_NV][3], double thr,
+                                                double (&acc)[RED_MAX_NV]) {
+  int j = 0;
+  for (int q = 0; q < nk0; ++q) {
+    double dd = 0.0;
+    for (int ax = 0; ax < dim; ++ax) {
+      const double e = k0[q][ax] - k[ax];
+      dd += e * e;
+    }
+    if (sqrt(dd) < thr) {
+#pragma unroll
+      for (int s = 0; s < RED_MAX_NV; ++s)
+        if (s == j) acc[s] += d;
+      continue;
+    }
+    ++j;
+  }
+}
+
 struct RedRabi {
   static constexpr int NV = RED_MAX_NV;
   const double2* psik;
@@ -1370,21 +1392,7 @@ struct RedRabi {
     const double d = a.x * a.x + a.y * a.y;
     double k[3];
     k_of(l, gr, k);
-    int j = 0;
-    for (int q = 0; q < nk0; ++q) {
-      double dd = 0.0;
-      for (int ax = 0; ax < gr.dim; ++ax) {
-        const double e = k0[q][ax] - k[ax];
-        dd += e * e;
-      }
-      if (sqrt(dd) < thr) {
-#pragma unroll
-        for (int s = 0; s < RED_MAX_NV; ++s)
-          if (s == j) acc[s] += d;
-        continue;
-      }
-      ++j;
-    }
+    rabi_accumulate(k, d, gr.dim, nk0, k0, thr, acc);
   }
 };
 
@@ -1418,23 +1426,123 @@ struct RedRabiBox {
     k[0] = gr.dk[0] * (double)m[0];
     k[1] = gr.dim > 1 ? gr.dk[1] * (double)m[1] : 0.0;
     k[2] = gr.dim > 2 ? gr.dk[2] * (double)m[2] : 0.0;
-    int j = 0;
-    for (int q = 0; q < nk0; ++q) {
-      double dd = 0.0;
-      for (int ax = 0; ax < gr.dim; ++ax) {
-        const double e = k0[q][ax] - k[ax];
-        dd += e * e;
-      }
-      if (sqrt(dd) < thr) {
-#pragma unroll
-        for (int s = 0; s < RED_MAX_NV; ++s)
-          if (s == j) acc[s] += d;
-        continue;
-      }
-      ++j;
-    }
+    rabi_accumulate(k, d, gr.dim, nk0, k0, thr, acc);
   }
 };
+
+// ---------------------------------------------- Rabi integrals of a 3D box
+// compute_rabi_integrals on a 3D grid only needs the spectrum inside the
+// index box of the balls (RedRabiBox's box, len[a] of the n[a] indices
+// along axis a).  Three passes transform only what reaches the box:
+//   k_rabi_z: forward DFT along z of every (x, y) pencil of the grid,
+//             keeping the len2 outputs inside the box: zb [n0][n1][len2]
+//   k_rabi_y: forward DFT along y of zb's (x, kz) pencils, keeping the len1
+//             outputs inside the box: yb [n0][len1][len2]
+//   k_rabi_x: forward DFT along x of yb's pencils times the transform's
+//             scale, then RedRabi's sums over the whole kx range: one
+//             partial per workgroup and segment
+// blockIdx.y is the segment (component or run), `seg` elements apart in
+// the input and compact in the buffers.  Each pencil's transform is the
+// full pass's (same plan, twiddles and operation order).
+struct RabiBox {
+  int n[3];
+  int lo[3];
+  int len[3];
+  double dk[3];
+  int nk0;
+  double k0[RED_MAX_NV][3];
+  double thr;
+  double scale;
+};
+
+// grid index of box index b along axis a (centred index lo + b, wrapped)
+__device__ __forceinline__ int rabi_box_index(const RabiBox& rb, int a, int b) {
+  const int m = rb.lo[a] + b;
+  return m < 0 ? m + rb.n[a] : m;
+}
+
+// box index of grid index i along axis a, or -1 outside the box
+__device__ __forceinline__ int rabi_box_of(const RabiBox& rb, int a, int i) {
+  const int m = i < rb.n[a] / 2 ? i : i - rb.n[a];
+  const int b = m - rb.lo[a];
+  return (b >= 0 && b < rb.len[a]) ? b : -1;
+}
+
+template <int N2, int RM>
+__global__ __launch_bounds__(256) void k_rabi_z(const double2* __restrict__ src, long long seg, double2* zb,
+                                                AxisGeo g, const double2* __restrict__ tw, RabiBox rb) {
+  extern __shared__ double2 lds_all[];
+  constexpr int R = Plan<N2, RM>::R, P = Plan<N2, RM>::P;
+  PencilMap<N2, RM, true> m(g);
+  double2* lds = lds_all + m.p * g.ls;
+  const double2* twl = stage_table<N2>(lds_all + (size_t)g.T * g.ls, fft_tw<N2, RM>(tw));
+  double2 v[R];
+  load_pencil<N2, RM>(v, src + (long long)blockIdx.y * seg, m.base, 1, m.t, m.valid);
+  if constexpr (tables_in_lds<N2>()) __syncthreads();
+  fft_pencil<N2, RM, -1, (P <= 64)>(v, m.t, lds, twl);
+  if (!m.valid) return;
+  double2* out = zb + ((long long)blockIdx.y * g.outer + m.pid) * rb.len[2];
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    const int b = rabi_box_of(rb, 2, m.t + P * i);
+    if (b >= 0) out[b] = v[i];
+  }
+}
+
+template <int N1, int RM>
+__global__ __launch_bounds__(256) void k_rabi_y(const double2* __restrict__ zb, double2* yb, AxisGeo g,
+                                                const double2* __restrict__ tw, RabiBox rb) {
+  extern __shared__ double2 lds_all[];
+  constexpr int R = Plan<N1, RM>::R, P = Plan<N1, RM>::P;
+  PencilMap<N1, RM, false> m(g);  // columns: kz box index; slabs: x
+  double2* lds = lds_all + m.p * g.ls;
+  const double2* twl = stage_table<N1>(lds_all + (size_t)g.T * g.ls, fft_tw<N1, RM>(tw));
+  const long long len12 = (long long)rb.len[1] * rb.len[2];
+  double2 v[R];
+  load_pencil<N1, RM>(v, zb + (long long)blockIdx.y * g.outer * N1 * g.inner, m.base, m.stride, m.t, m.valid);
+  if constexpr (tables_in_lds<N1>()) __syncthreads();
+  fft_pencil<N1, RM, -1, false>(v, m.t, lds, twl);
+  if (!m.valid) return;
+  const long long o = m.pid / g.inner, b2 = m.pid - o * g.inner;
+  double2* out = yb + (long long)blockIdx.y * g.outer * len12 + o * len12 + b2;
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    const int b = rabi_box_of(rb, 1, m.t + P * i);
+    if (b >= 0) out[(long long)b * rb.len[2]] = v[i];
+  }
+}
+
+template <int N0, int RM>
+__global__ __launch_bounds__(256) void k_rabi_x(const double2* __restrict__ yb, AxisGeo g,
+                                                const double2* __restrict__ tw, RabiBox rb,
+                                                double* __restrict__ partials) {
+  extern __shared__ double2 lds_all[];
+  constexpr int R = Plan<N0, RM>::R, P = Plan<N0, RM>::P;
+  PencilMap<N0, RM, false> m(g);  // columns: (ky, kz) box index pairs
+  double2* lds = lds_all + m.p * g.ls;
+  const double2* twl = stage_table<N0>(lds_all + (size_t)g.T * g.ls, fft_tw<N0, RM>(tw));
+  double2 v[R];
+  load_pencil<N0, RM>(v, yb + (long long)blockIdx.y * N0 * g.inner, m.base, m.stride, m.t, m.valid);
+  if constexpr (tables_in_lds<N0>()) __syncthreads();
+  fft_pencil<N0, RM, -1, false>(v, m.t, lds, twl);
+  double acc[RED_MAX_NV];
+#pragma unroll
+  for (int q = 0; q < RED_MAX_NV; ++q) acc[q] = 0.0;
+  if (m.valid) {
+    const int b1 = (int)(m.pid / rb.len[2]), b2 = (int)(m.pid - (long long)b1 * rb.len[2]);
+    double k[3];
+    k[1] = rb.dk[1] * (double)(rb.lo[1] + b1);
+    k[2] = rb.dk[2] * (double)(rb.lo[2] + b2);
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      const int j = m.t + P * i;
+      k[0] = rb.dk[0] * (double)(j < N0 / 2 ? j : j - N0);
+      const double2 a = cscale(v[i], rb.scale);
+      rabi_accumulate(k, a.x * a.x + a.y * a.y, 3, rb.nk0, rb.k0, rb.thr, acc);
+    }
+  }
+  block_reduce_store<RED_MAX_NV>(acc, partials + ((long long)blockIdx.y * gridDim.x + blockIdx.x) * RED_MAX_NV);
+}
 
 // kinetic energy density sum_a alpha_a k_a^2 |psi^|^2 on a transformed buffer
 struct RedEkin {

@@ -102,6 +102,15 @@ hipError_t launch_fs_col(int n1, int rm, int mode, int vk, int nc, bool batch, d
                          int ycount, long long n2, const double2* tw1, const double2* wN, const VArgs& va, double dx,
                          double scale, const Limits& max_lds, hipStream_t s);
 
+// one pass of the Rabi integrals of a 3D box (k_rabi_z / _y / _x for axis
+// 2 / 1 / 0) over nseg segments: src (axis 2) `seg` elements apart, the
+// compact buffers zb [nseg][n0][n1][len2] and yb [nseg][n0][len1][len2];
+// axis 0 writes nseg * (*nblocks) * RED_MAX_NV partials (partials NULL:
+// only *nblocks is set)
+hipError_t launch_rabi_box(int axis, int rm, const double2* src, long long seg, int nseg, double2* zb, double2* yb,
+                           const double2* tw, const RabiBox& rb, const Limits& lim, double* partials,
+                           unsigned* nblocks, hipStream_t s);
+
 // Sobolev filters along axis 0
 hipError_t launch_sobf(int N, int rm, bool contig, const double2* src, double2* out_sob, double2* out_lap,
                        long long comp_stride, int ncomp, long long inner, const double2* tw, const SobArgs& sa,
