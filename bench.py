@@ -45,14 +45,18 @@ def gpu_run(args, wl):
     t = 0.0
     choice = {}
     if args.tuning:
-        # fixed choices, e.g. "pass_strided=8,vfuse=8,slab=-1"
+        # fixed choices, e.g. "pass_strided=8,vfuse=8,slab=-1,threads.vfuse=64"
         for item in args.tuning.split(","):
             fam, v = (w.strip() for w in item.split("="))
             if fam == "slab":
                 ctx.set_slab(int(v))
+                choice[fam] = int(v)
+            elif fam.startswith("threads."):
+                ctx.tune_threads(fam.split(".", 1)[1], int(v))
+                choice.setdefault("threads", {})[fam.split(".", 1)[1]] = int(v)
             else:
                 ctx.tune(fam, int(v))
-            choice[fam] = int(v)
+                choice[fam] = int(v)
         log(f"[bench] fixed choices: {choice}")
     elif not args.no_autotune:
         from atus2_amd.autotune import autotune
@@ -297,7 +301,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--tuning", default="", help="fixed values per thread, e.g. pass_strided=8,vfuse=8")
+    ap.add_argument("--tuning", default="",
+                    help="fixed choices, e.g. pass_strided=8,vfuse=8,slab=-1,threads.vfuse=64")
     ap.add_argument("--gs-cases", type=int, default=4, help="ground states timed cold and warm (0: none)")
     ap.add_argument("--no-other-configs", action="store_true",
                     help="skip the 1D 4096 and 2D 2048^2 rt_solver workloads")
