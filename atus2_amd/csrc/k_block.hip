@@ -14,7 +14,9 @@ static hipError_t block_go(double2* psi, long long cs, long long runs, const dou
   // the workgroup holds
   const int max_threads = BATCH ? LB : threads_per_pencil(N, RM);
   PassCfg c = make_pass_cfg(N, RM, 1, runs, 1, max_lds, 3, max_threads);
-  if (c.block.x == 0 || c.block.x > LB || c.grid.x == 0) return hipErrorInvalidConfiguration;
+  // above 1024 points the twiddle table alone is staged in LDS
+  if constexpr (!tables_in_lds<N>()) c.lds += (size_t)N * sizeof(double2);
+  if (c.block.x == 0 || c.block.x > LB || c.grid.x == 0 || c.lds > max_lds.lds) return hipErrorInvalidConfiguration;
   static bool ok = false;
   hipError_t e = allow_lds(k_block1d<N, RM, VK, NC, BATCH, LB>, c.lds, ok);
   if (e != hipSuccess) return e;

@@ -8,7 +8,7 @@ template <int N, int RM, bool CONTIG, int VK, int NC, int LB = 256>
 static hipError_t kin_go(double2* psi, long long cs, int ycount, long long inner, long long outer,
                          const double2* tw, const KinTabs& kt, const VArgs& va, double dx0, const Limits& max_lds,
                          hipStream_t s) {
-  PassCfg c = make_pass_cfg(N, RM, inner, outer, ycount, max_lds, 2, LB);
+  PassCfg c = make_pass_cfg(N, RM, inner, outer, ycount, max_lds, kin_tables_in_lds<N, CONTIG>() ? 2 : 0, LB);
   if (c.block.x == 0 || c.block.x > LB || c.grid.x == 0) return hipErrorInvalidConfiguration;
   static bool ok = false;
   hipError_t e = allow_lds(k_kin<N, RM, CONTIG, VK, NC, false, LB>, c.lds, ok);

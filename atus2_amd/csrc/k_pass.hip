@@ -5,21 +5,25 @@ namespace atus {
 
 bool supported_len(int N) { return N >= 2 && N <= 4096 && (N & (N - 1)) == 0; }
 
-// LDS elements per pencil region.  LDS serves a 16-byte access of a
-// wavefront 16 lanes at a time from 64 banks of 4 bytes, i.e. 16 slots of
-// 16 bytes; within a pencil the exchange positions are XOR-swizzled in
-// rows of 16 elements, so 16 lanes of one pencil hit 16 distinct slots.
-// When 16 lanes span several pencils -- g pencils of 16/g lanes each (a
-// strided pass: lane = t*T + p, g = min(T, 16); a contiguous pass with P <
-// 16: g = 16/P) -- pencil regions 16/g slots apart (mod 16) keep them on
-// disjoint slots; with one pencil per 16 lanes one element of padding is
-// kept.
+// LDS elements per pencil region.  The bank-conflict counters of the
+// passes fit a model where LDS serves a 16-byte read of a wavefront 16
+// lanes at a time from 64 banks of 4 bytes (16 slots of 16 B), and a
+// 16-byte write 8 lanes at a time from 32 banks (8 slots).  Within a pencil
+// the exchange positions are XOR-swizzled in rows of 16 elements.  When the
+// lanes of a group span several pencils, the regions of those pencils are
+// padded so that they start on different slots:
+//   * strided passes (lane = t*T + p): an 8-lane write group holds
+//     g = min(T, 8) pencils, padding 8/g elements (1 for T = 1 or T >= 8);
+//   * contiguous passes with P < 16 threads per pencil: a 16-lane read
+//     group holds 16/P pencils, padding P elements;
+//   * otherwise one element.
 static int pencil_lds_elems(int N, int P, bool contig, long long T) {
-  int g = 1;
-  if (contig) g = P < 16 ? 16 / P : 1;
-  else g = T < 16 ? (int)T : 16;
-  if (g > 16) g = 16;
-  const int pad = g > 1 ? 16 / g : 1;
+  int pad = 1;
+  if (contig) {
+    if (P < 16) pad = P;
+  } else if (T > 1 && T < 8) {
+    pad = 8 / (int)T;
+  }
   return N + pad;
 }
 

@@ -136,11 +136,20 @@ template <int N>
 constexpr bool tables_in_lds() {
   return N <= 1024;
 }
+// k_kin's twiddle and phase tables: in LDS when N <= 1024, except for the
+// strided form at 512 and 1024 points, whose 16 (RM = 16) or 8 pencils per
+// workgroup already fill 66 KiB of LDS: with 16-32 KiB of tables on top, one
+// workgroup fits a CU, without them two
+template <int N, bool CONTIG>
+constexpr bool kin_tables_in_lds() {
+  return CONTIG ? tables_in_lds<N>() : N <= 256;
+}
+
 // copies src[0..N) to lds_tab (then a workgroup barrier) and returns the
 // pointer the kernel should read; the global table is returned otherwise
-template <int N>
+template <int N, bool IN_LDS = tables_in_lds<N>()>
 __device__ __forceinline__ const double2* stage_table(double2* lds_tab, const double2* src) {
-  if constexpr (tables_in_lds<N>()) {
+  if constexpr (IN_LDS) {
     for (int j = threadIdx.x; j < N; j += blockDim.x) lds_tab[j] = src[j];
     return lds_tab;
   } else {
@@ -154,9 +163,9 @@ __device__ __forceinline__ const double2* stage_table(double2* lds_tab, const do
 // its twiddles live in registers in between.  The table stays a global
 // (not a flat) address.
 typedef const __attribute__((address_space(1))) double2* global_double2_ptr;
-template <int N>
+template <int N, bool IN_LDS = tables_in_lds<N>()>
 __device__ __forceinline__ auto fresh_tw(const double2* p) {
-  if constexpr (!tables_in_lds<N>()) {
+  if constexpr (!IN_LDS) {
     asm volatile("" : "+s"(p));
     return (global_double2_ptr)p;
   } else {
@@ -717,16 +726,17 @@ __global__ __launch_bounds__(LB) void k_kin(double2* __restrict__ psi, long long
   extern __shared__ double2 lds_all[];
   constexpr int R = Plan<N, RM>::R, P = Plan<N, RM>::P;
   constexpr bool WAVE = CONTIG && (P <= 64);
+  constexpr bool KT = kin_tables_in_lds<N, CONTIG>();
   PencilMap<N, RM, CONTIG> m(g);
   double2* lds = lds_all + m.p * g.ls;
-  const double2* twl = stage_table<N>(lds_all + (size_t)g.T * g.ls, fft_tw<N, RM>(tw));
+  const double2* twl = stage_table<N, KT>(lds_all + (size_t)g.T * g.ls, fft_tw<N, RM>(tw));
   // the component of this block (one per blockIdx.y) may belong to the
   // second species, with its own tables (workgroup-uniform choice)
   const bool sp2 = NC == 1 && kt.f0 != nullptr && (int)blockIdx.y >= kt.split;
   const double2* t0 = sp2 ? kt.f0 : kt.e0;
   const double2* t1t = sp2 ? kt.f1 : kt.e1;
   const double2* t2t = sp2 ? kt.f2 : kt.e2;
-  const double2* e0l = stage_table<N>(lds_all + (size_t)g.T * g.ls + N, t0);
+  const double2* e0l = stage_table<N, KT>(lds_all + (size_t)g.T * g.ls + N, t0);
   // pencil constant of the phase (axes 1, 2)
   double2 cp = make_double2(1.0, 0.0);
   if constexpr (!CONTIG) {
@@ -764,10 +774,10 @@ __global__ __launch_bounds__(LB) void k_kin(double2* __restrict__ psi, long long
       for (int c = 0; c < NC; ++c) v[c][i] = a[c];
     }
   }
-  if constexpr (tables_in_lds<N>()) __syncthreads();
+  if constexpr (KT) __syncthreads();
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
-    fft_pencil<N, RM, -1, WAVE>(v[c], m.t, lds, fresh_tw<N>(twl));
+    fft_pencil<N, RM, -1, WAVE>(v[c], m.t, lds, fresh_tw<N, KT>(twl));
     const double2* ezr = nullptr;
     if constexpr (CONTIG) {
       if (kt.ez) ezr = kt.ez + (m.pid % kt.ez_rows) * N;
@@ -782,7 +792,7 @@ __global__ __launch_bounds__(LB) void k_kin(double2* __restrict__ psi, long long
       }
       v[c][i] = cmul(v[c][i], e);
     }
-    fft_pencil<N, RM, +1, WAVE>(v[c], m.t, lds, fresh_tw<N>(twl));
+    fft_pencil<N, RM, +1, WAVE>(v[c], m.t, lds, fresh_tw<N, KT>(twl));
     store_pencil<N, RM>(v[c], psi + c0 + c * comp_stride, m.base, m.stride, m.t, m.valid);
   }
 }
@@ -810,7 +820,13 @@ __global__ __launch_bounds__(LB) void k_block1d(double2* __restrict__ psi, long 
   PencilMap<N, RM, true> m(g);
   double2* lds = lds_all + m.p * g.ls;
   double2* tab = lds_all + (size_t)g.T * g.ls;
-  const double2* twl = stage_table<N>(tab, fft_tw<N, RM>(tw));
+  // the twiddles live in LDS at every length (the block reads them
+  // 2 (Nk + 1) times); the phase tables when N <= 1024
+  {
+    const double2* src = fft_tw<N, RM>(tw);
+    for (int j = threadIdx.x; j < N; j += blockDim.x) tab[j] = src[j];
+  }
+  const double2* twl = tab;
   const double2* ehl = stage_table<N>(tab + N, eh);
   const double2* efl = stage_table<N>(tab + 2 * N, ef);
   long long c0 = 0;
@@ -824,14 +840,14 @@ __global__ __launch_bounds__(LB) void k_block1d(double2* __restrict__ psi, long 
   double2 v[NC][R];
 #pragma unroll
   for (int c = 0; c < NC; ++c) load_pencil<N, RM>(v[c], psi + c0 + c * comp_stride, m.base, 1, m.t, m.valid);
-  if constexpr (tables_in_lds<N>()) __syncthreads();
+  __syncthreads();
   auto kinetic = [&](const double2* e) {
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
-      fft_pencil<N, RM, -1, WAVE>(v[c], m.t, lds, fresh_tw<N>(twl));
+      fft_pencil<N, RM, -1, WAVE>(v[c], m.t, lds, twl);
 #pragma unroll
       for (int i = 0; i < R; ++i) v[c][i] = cmul(v[c][i], e[m.t + P * i]);
-      fft_pencil<N, RM, +1, WAVE>(v[c], m.t, lds, fresh_tw<N>(twl));
+      fft_pencil<N, RM, +1, WAVE>(v[c], m.t, lds, twl);
     }
   };
   kinetic(ehl);
