@@ -8,7 +8,7 @@ namespace atus {
 template <int N, int RM, int VK>
 static hipError_t kinb_go(double2* psi, long long cs, long long nbatch, const double2* tw, const KinTabs& kt,
                           const VArgs& va, double dx0, const Limits& max_lds, hipStream_t s) {
-  PassCfg c = make_pass_cfg(N, RM, 1, nbatch, 1, max_lds, 2);
+  PassCfg c = make_pass_cfg(N, RM, 1, nbatch, 1, max_lds, 0, 256, kin_lds_table_elems<N, RM, true>());
   if (c.block.x == 0 || c.block.x > 256 || c.grid.x == 0) return hipErrorInvalidConfiguration;
   static bool ok = false;
   hipError_t e = allow_lds(k_kin<N, RM, true, VK, 2, true>, c.lds, ok);

@@ -8,7 +8,7 @@ template <int N, int RM, bool CONTIG, int VK, int NC, int LB = 256>
 static hipError_t kin_go(double2* psi, long long cs, int ycount, long long inner, long long outer,
                          const double2* tw, const KinTabs& kt, const VArgs& va, double dx0, const Limits& max_lds,
                          hipStream_t s) {
-  PassCfg c = make_pass_cfg(N, RM, inner, outer, ycount, max_lds, kin_tables_in_lds<N, CONTIG>() ? 2 : 0, LB);
+  PassCfg c = make_pass_cfg(N, RM, inner, outer, ycount, max_lds, 0, LB, kin_lds_table_elems<N, RM, CONTIG>());
   if (c.block.x == 0 || c.block.x > LB || c.grid.x == 0) return hipErrorInvalidConfiguration;
   static bool ok = false;
   hipError_t e = allow_lds(k_kin<N, RM, CONTIG, VK, NC, false, LB>, c.lds, ok);
@@ -26,7 +26,8 @@ static hipError_t kin_nr(bool contig, int vk, int nc, double2* psi, long long cs
     // 2D/3D: axis 0 is strided, one component per blockIdx.y, no V prologue
     if (vk != V_NONE) return hipErrorInvalidValue;
     // columns of 2048 and more: workgroups of up to 512 threads hold twice
-    // the columns, so each row is read in segments twice as long
+    // the columns, so each row is read in segments twice as long (at 1024
+    // points, 512 threads leave one workgroup per CU: slower)
     if constexpr (N >= 2048)
       return kin_go<N, RM, false, V_NONE, 1, 512>(psi, cs, ncomp, inner, rows, tw, kt, va, dx0, max_lds, s);
     return kin_go<N, RM, false, V_NONE, 1>(psi, cs, ncomp, inner, rows, tw, kt, va, dx0, max_lds, s);

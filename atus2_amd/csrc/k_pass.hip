@@ -28,11 +28,13 @@ static int pencil_lds_elems(int N, int P, bool contig, long long T) {
 }
 
 PassCfg make_pass_cfg(int N, int rm, long long inner, long long outer, int ycount, size_t max_lds, int ntables,
-                      int max_threads) {
+                      int max_threads, long long table_elems) {
   PassCfg c{};
   const int P = threads_per_pencil(N, rm);
   const bool contig = inner == 1;
-  const size_t table_bytes = N <= 1024 ? (size_t)ntables * N * sizeof(double2) : 0;
+  const size_t table_bytes = table_elems >= 0   ? (size_t)table_elems * sizeof(double2)
+                             : N <= 1024 ? (size_t)ntables * N * sizeof(double2)
+                                         : 0;
   long long T = max_threads / P;
   if (T < 1) T = 1;
   if (!contig && T > inner) T = inner;

@@ -136,13 +136,28 @@ template <int N>
 constexpr bool tables_in_lds() {
   return N <= 1024;
 }
-// k_kin's twiddle and phase tables: in LDS when N <= 1024, except for the
-// strided form at 512 and 1024 points, whose 16 (RM = 16) or 8 pencils per
-// workgroup already fill 66 KiB of LDS: with 16-32 KiB of tables on top, one
-// workgroup fits a CU, without them two
+// k_kin's tables in LDS: the twiddles its plan reads (plan_tw_elems, at
+// most N - 1 entries) when N <= 1024; the phase table e0 (N entries) as
+// well, except for strided columns of 1024 points.  There the pencils of a
+// workgroup already take 64 KiB of LDS (4 pencils at 16 values per thread,
+// 4 at 8 values and 512 threads): with the twiddles (16128 or 16256 B) two
+// workgroups still fit a CU's 160 KiB, with e0 on top only one would.
 template <int N, bool CONTIG>
-constexpr bool kin_tables_in_lds() {
-  return CONTIG ? tables_in_lds<N>() : N <= 256;
+constexpr bool kin_tw_in_lds() {
+  return tables_in_lds<N>();
+}
+template <int N, bool CONTIG>
+constexpr bool kin_phase_in_lds() {
+  return CONTIG ? tables_in_lds<N>() : N <= 512;
+}
+template <int N, int RM>
+__host__ __device__ constexpr int plan_tw_elems() {
+  return Plan<N, RM>::template tw_off<Plan<N, RM>::S>();
+}
+// LDS elements of k_kin's tables
+template <int N, int RM, bool CONTIG>
+__host__ __device__ constexpr int kin_lds_table_elems() {
+  return (kin_tw_in_lds<N, CONTIG>() ? plan_tw_elems<N, RM>() : 0) + (kin_phase_in_lds<N, CONTIG>() ? N : 0);
 }
 
 // copies src[0..N) to lds_tab (then a workgroup barrier) and returns the
@@ -726,17 +741,18 @@ __global__ __launch_bounds__(LB) void k_kin(double2* __restrict__ psi, long long
   extern __shared__ double2 lds_all[];
   constexpr int R = Plan<N, RM>::R, P = Plan<N, RM>::P;
   constexpr bool WAVE = CONTIG && (P <= 64);
-  constexpr bool KT = kin_tables_in_lds<N, CONTIG>();
+  constexpr bool KT = kin_tw_in_lds<N, CONTIG>(), KE = kin_phase_in_lds<N, CONTIG>();
+  constexpr int TWN = KT ? plan_tw_elems<N, RM>() : 0;
   PencilMap<N, RM, CONTIG> m(g);
   double2* lds = lds_all + m.p * g.ls;
-  const double2* twl = stage_table<N, KT>(lds_all + (size_t)g.T * g.ls, fft_tw<N, RM>(tw));
+  const double2* twl = stage_table<TWN, KT>(lds_all + (size_t)g.T * g.ls, fft_tw<N, RM>(tw));
   // the component of this block (one per blockIdx.y) may belong to the
   // second species, with its own tables (workgroup-uniform choice)
   const bool sp2 = NC == 1 && kt.f0 != nullptr && (int)blockIdx.y >= kt.split;
   const double2* t0 = sp2 ? kt.f0 : kt.e0;
   const double2* t1t = sp2 ? kt.f1 : kt.e1;
   const double2* t2t = sp2 ? kt.f2 : kt.e2;
-  const double2* e0l = stage_table<N, KT>(lds_all + (size_t)g.T * g.ls + N, t0);
+  const double2* e0l = stage_table<N, KE>(lds_all + (size_t)g.T * g.ls + TWN, t0);
   // pencil constant of the phase (axes 1, 2)
   double2 cp = make_double2(1.0, 0.0);
   if constexpr (!CONTIG) {
@@ -774,7 +790,7 @@ __global__ __launch_bounds__(LB) void k_kin(double2* __restrict__ psi, long long
       for (int c = 0; c < NC; ++c) v[c][i] = a[c];
     }
   }
-  if constexpr (KT) __syncthreads();
+  if constexpr (KT || KE) __syncthreads();
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     fft_pencil<N, RM, -1, WAVE>(v[c], m.t, lds, fresh_tw<N, KT>(twl));

@@ -28,19 +28,20 @@ struct Limits {
 
 // contiguous pencils if inner == 1; the pass covers `outer` slabs of
 // [N][inner]; ycount = gridDim.y (components)
-// ntables: lookup tables of length N the kernel stages in LDS (N <= 1024)
+// ntables: lookup tables of length N the kernel stages in LDS (N <= 1024);
+// table_elems >= 0 gives the LDS table elements exactly instead
 PassCfg make_pass_cfg(int N, int rm, long long inner, long long outer, int ycount, size_t max_lds,
-                      int ntables = 1, int max_threads = 256);
+                      int ntables = 1, int max_threads = 256, long long table_elems = -1);
 // the same within Limits (max_threads: the kernel's launch bounds)
 inline PassCfg make_pass_cfg(int N, int rm, long long inner, long long outer, int ycount, const Limits& lim,
-                             int ntables = 1, int max_threads = 256) {
+                             int ntables = 1, int max_threads = 256, long long table_elems = -1) {
   int t = max_threads;
   if (lim.threads > 0) {
     const int P = threads_per_pencil(N, rm);
     t = lim.threads < max_threads ? lim.threads : max_threads;
     if (t < P) t = P < max_threads ? P : max_threads;
   }
-  return make_pass_cfg(N, rm, inner, outer, ycount, lim.lds, ntables, t);
+  return make_pass_cfg(N, rm, inner, outer, ycount, lim.lds, ntables, t, table_elems);
 }
 
 bool supported_len(int N);  // power of two in [2, 4096]

@@ -52,9 +52,11 @@ def test_ft_matches_oracle(hip, orc, shape, rm):
     assert rel_l2(back, psi) < 1e-13
 
 
-# (2048, 4), (4096, 2): strided kinetic passes in 512-thread workgroups
+# (1024, 8), (2048, 4), (4096, 2): strided kinetic passes in workgroups of
+# up to 512 threads; (512, 8), (1024, 8): twiddles and phases in LDS, and
+# twiddles alone
 @pytest.mark.parametrize("shape", [(64,), (4096,), (8192,), (65536,), (32, 64), (8, 16, 32), (64, 64, 64),
-                                   (1024, 8), (2048, 4), (4096, 2)])
+                                   (512, 8), (1024, 8), (2048, 4), (4096, 2)])
 @pytest.mark.parametrize("frac", [0.5, 1.0])
 @pytest.mark.parametrize("rm", [16, 8])
 def test_kinetic_matches_oracle(hip, orc, shape, frac, rm):
