@@ -16,7 +16,9 @@ autotuner's values per thread and threads per workgroup.
 
 `python -m atus2_amd.scan` times the 512-run scan three ways (the graph,
 the shape groups one after another with eager launches, and every run
-one after another through the single-run path) and prints one JSON line.
+one after another through the single-run path) and prints one JSON line;
+with --family-times it times each kernel family of each shape group
+instead, one group at a time.
 """
 from __future__ import annotations
 
@@ -240,6 +242,38 @@ def run_single(runs, dt=2e-4, Nk=100, steps=3000, log=None):
     return out, elapsed
 
 
+def family_times(runs, dt=2e-4, Nk=20, reps=3, tune=True, log=None):
+    """Per shape group, alone on the device: device time of each kernel
+    family per whole-grid pass (the best of `reps` profiled blocks, HIP
+    events around every launch) and its rate in algorithmic bytes (a pass
+    reads and writes both components of every run: 64 B per point).
+    Returns a list of dicts, one per shape."""
+    out = []
+    for shape, idx, B in make_contexts(runs, dt, 100, tune, log):
+        try:
+            rs = [runs[i] for i in idx]
+            sig = np.array([r.sigma for r in rs])
+            t0s = np.array([r.t0 for r in rs])
+            t = 0.0
+            best = {}
+            for rep in range(reps + 1):
+                t1 = np.array(B.step_times(t, dt, Nk))[:, None]
+                F = np.exp(-0.5 * ((t1 - t0s[None, :]) / sig[None, :]) ** 2)
+                t, prof = B.run_batch_block_profiled(hip.STEP_BRAGG_AD, dt, Nk, t, F)
+                if rep == 0:
+                    continue  # first block: kernel loading and LDS set-up
+                for f, (ms, n, grid) in prof.items():
+                    if n:
+                        best[f] = min(best.get(f, float("inf")), ms / grid)
+            gb = 64.0 * len(idx) * math.prod(shape) / 1e9
+            out.append({"shape": list(shape), "runs": len(idx),
+                        "families": {f: {"us_per_grid": 1e3 * ms, "GBps": gb / (ms * 1e-3)}
+                                     for f, ms in sorted(best.items())}})
+        finally:
+            B.close()
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--steps", type=int, default=3000)
@@ -248,12 +282,19 @@ def main(argv=None):
     ap.add_argument("--no-single", action="store_true", help="skip the one-after-another baseline")
     ap.add_argument("--no-batched", action="store_true", help="skip the eager group-by-group batched run")
     ap.add_argument("--no-tune", action="store_true", help="library defaults instead of the autotuner's choices")
+    ap.add_argument("--family-times", action="store_true",
+                    help="only time each kernel family per shape group, one group at a time (JSON line per shape)")
     a = ap.parse_args(argv)
     runs = scan_runs(n1d=a.n1d, n2d=a.n2d)
     pts = sum(math.prod(r.shape) for r in runs) * a.steps
 
     def log(m):
         print(f"[scan] {m}", file=sys.stderr, flush=True)
+
+    if a.family_times:
+        for row in family_times(runs, tune=not a.no_tune, log=log):
+            print(json.dumps(row), flush=True)
+        return
 
     log(f"{len(runs)} runs, {a.steps} steps each")
     rg, tg, gpu = run_graph(runs, steps=a.steps, tune=not a.no_tune, log=log)

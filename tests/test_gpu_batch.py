@@ -203,3 +203,17 @@ def test_scan_graph_equals_eager_batches(hip, orc):
         for _, pair in made:
             for B in pair:
                 B.close()
+
+
+def test_scan_family_times_cover_every_group(hip):
+    """`python -m atus2_amd.scan --family-times`: one row per shape group
+    with a positive time and rate for each kernel family it launched."""
+    from atus2_amd.scan import family_times, scan_runs
+    runs = scan_runs(seed=7, n1d=3, n2d=1)
+    rows = family_times(runs, Nk=4, reps=1, tune=False)
+    assert sorted(tuple(r["shape"]) for r in rows) == sorted({r.shape for r in runs})
+    for r in rows:
+        assert r["runs"] == sum(1 for x in runs if list(x.shape) == r["shape"])
+        assert r["families"], r
+        for f, v in r["families"].items():
+            assert v["us_per_grid"] > 0 and v["GBps"] > 0, (r["shape"], f, v)
