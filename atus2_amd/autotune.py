@@ -53,18 +53,44 @@ def autotune(ctx: "hip.Context", kind: int, dt: float, t: float, las=None, amp=N
     if len(ctx.shape) == 3 and not isinstance(ctx, hip.BatchContext):
         # slab-chained passes on two streams against whole-grid passes: wall
         # time of ordinary blocks (profiled blocks run on one stream)
-        total = {}
-        for slab in (0, -1):
-            ctx.set_slab(slab)
+        def wall(t):
             t = ctx.run_block(kind, dt, Nk, t, las, amp)
             ctx.synchronize()
-            total[slab] = float("inf")
+            best_s = float("inf")
             for _ in range(reps):
                 tic = time.perf_counter()
                 t = ctx.run_block(kind, dt, Nk, t, las, amp)
                 ctx.synchronize()
-                total[slab] = min(total[slab], time.perf_counter() - tic)
+                best_s = min(best_s, time.perf_counter() - tic)
+            return t, best_s
+
+        total = {}
+        for slab in (0, -1):
+            ctx.set_slab(slab)
+            t, total[slab] = wall(t)
         choice["slab"] = min(total, key=total.get)
+        if choice["slab"] != 0:
+            # on the slab schedule a pass shares the device with the other
+            # stream's launches, so the fastest pass alone need not give the
+            # fastest step: each family of the slab chains in turn takes the
+            # (values per thread, threads) that minimises the block's wall
+            # time with the others fixed
+            ctx.set_slab(choice["slab"])
+            cur = total[choice["slab"]]
+            for f in ("pass_strided", "vfuse"):
+                for rm in (16, 8):
+                    for th in THREADS:
+                        if (rm, th) == (choice[f], choice["threads"][f]):
+                            continue
+                        ctx.tune(f, rm)
+                        ctx.tune_threads(f, th)
+                        t, w = wall(t)
+                        if w < cur:
+                            cur = w
+                            choice[f] = rm
+                            choice["threads"][f] = th
+                ctx.tune(f, choice[f])
+                ctx.tune_threads(f, choice["threads"][f])
     _apply(ctx, choice)
     _cache[key] = _copy(choice)
     return t, choice
