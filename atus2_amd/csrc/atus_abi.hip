@@ -310,14 +310,17 @@ int vfuse_slab(atus_ctx* c, int vk, const VArgs& va, int s0, int S, hipStream_t 
 }
 
 // y's share of the half kinetic step (forward DFT, phase, inverse DFT along
-// axis 1) over the x-planes [s0, s0 + S) of all components
+// axis 1) over the x-planes [s0, s0 + S) of all components.  It takes the
+// place of a strided y pass in the slab chain, beside the other stream's
+// launches, so it runs (and is timed and tuned) as that family.
 int kin_y_slab(atus_ctx* c, const VArgs& va, int s0, int S, hipStream_t st) {
   const int N = c->n[1];
   ATUS_TRY(ensure_twiddles(c, N));
   const long long plane = (long long)c->n[1] * c->n[2];
-  ProfScope ps(c, ATUS_KFAM_KIN, (double)S / c->n[0], st);
-  HIP_TRY(launch_kin(N, c->rm[ATUS_KFAM_KIN], false, V_NONE, 1, c->psi + (long long)s0 * plane, c->npts, c->ncomp,
-                     c->n[2], c->tw[N], axis_tabs(c, 1), va, c->dx[0], lim(c, ATUS_KFAM_KIN), st, S));
+  ProfScope ps(c, ATUS_KFAM_PASS_STRIDED, (double)S / c->n[0], st);
+  HIP_TRY(launch_kin(N, c->rm[ATUS_KFAM_PASS_STRIDED], false, V_NONE, 1, c->psi + (long long)s0 * plane, c->npts,
+                     c->ncomp, c->n[2], c->tw[N], axis_tabs(c, 1), va, c->dx[0], lim(c, ATUS_KFAM_PASS_STRIDED), st,
+                     S));
   return ATUS_OK;
 }
 
