@@ -3,7 +3,10 @@ values each thread holds (8 or 16: the radix of the transform stages) and
 the threads per workgroup (the pencils, or tile of columns, a workgroup
 takes), and for 3D grids whether the y/z passes run slab by slab
 (atus_set_slab), by timing profiled blocks of the actual workload on the
-actual context (HIP events on the context's stream).
+actual context (HIP events on the context's stream).  On the slab schedule
+the families of the slab chains are then re-timed by the wall time of
+ordinary blocks, where their launches share the device with the other
+stream's.
 
 The blocks it runs advance the context's state like any other block, so it
 belongs in the warm-up of a run.  Choices are cached per (grid shape,
