@@ -946,14 +946,37 @@ __global__ __launch_bounds__(256) void k_vfuse(double2* __restrict__ psi, long l
   }
 #pragma unroll
   for (int c = 0; c < NC; ++c) fft_pencil<N, RM, +1, WAVE>(v[c], m.t, lds, fresh_tw<N>(twl));
+  if constexpr (NC == 2) {
+    // the second component waits in this pencil's LDS region (each thread
+    // at its own positions) while the V step runs point by point, so only
+    // one component's values stay in registers through the V step's
+    // temporaries
+    lds_sync<WAVE>();  // the last exchange's reads of the region are done
 #pragma unroll
-  for (int i = 0; i < R; ++i) {
-    double2 a[NC];
+    for (int i = 0; i < R; ++i) lds[lds_pos(m.t + P * i)] = v[1][i];
+    asm volatile("" ::: "memory");
 #pragma unroll
-    for (int c = 0; c < NC; ++c) a[c] = v[c][i];
-    if (m.valid) v_point<VK, NC>(a, x0, m.base + m.t + P * i, va, Om, eta);
+    for (int i = 0; i < R; ++i) {
+      double2 a[2];
+      a[0] = v[0][i];
+      a[1] = lds[lds_pos(m.t + P * i)];
+      if (m.valid) v_point<VK, 2>(a, x0, m.base + m.t + P * i, va, Om, eta);
+      v[0][i] = a[0];
+      lds[lds_pos(m.t + P * i)] = a[1];
+    }
+    asm volatile("" ::: "memory");
 #pragma unroll
-    for (int c = 0; c < NC; ++c) v[c][i] = a[c];
+    for (int i = 0; i < R; ++i) v[1][i] = lds[lds_pos(m.t + P * i)];
+  } else {
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      double2 a[NC];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) a[c] = v[c][i];
+      if (m.valid) v_point<VK, NC>(a, x0, m.base + m.t + P * i, va, Om, eta);
+#pragma unroll
+      for (int c = 0; c < NC; ++c) v[c][i] = a[c];
+    }
   }
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
