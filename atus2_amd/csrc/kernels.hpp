@@ -866,6 +866,17 @@ __global__ __launch_bounds__(LB) void k_block1d(double2* __restrict__ psi, long 
       fft_pencil<N, RM, +1, WAVE>(v[c], m.t, lds, twl);
     }
   };
+  // the potential is the same for every step of the block: read once into
+  // registers rather than after every step's last barrier
+  double potv[VK == V_NL ? NC : 1][VK == V_NL ? R : 1];
+  if constexpr (VK == V_NL) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+#pragma unroll
+      for (int i = 0; i < R; ++i)
+        potv[c][i] = (va.pot && m.valid) ? va.pot[c * va.pot_stride + m.base + m.t + P * i] : 0.0;
+    }
+  }
   kinetic(ehl);
   for (int s = 1; s <= Nk; ++s) {
     va.t1p = t1tab + (s - 1);
@@ -885,7 +896,15 @@ __global__ __launch_bounds__(LB) void k_block1d(double2* __restrict__ psi, long 
       double Om = 0.0;
       double2 eta = make_double2(1.0, 0.0);
       if constexpr (VK == V_BRAGG) bragg_field(x0, va, Om, eta);
-      if (m.valid) v_point<VK, NC>(a, x0, m.base + j, va, Om, eta);
+      if constexpr (VK == V_NL) {
+        // v_point's V_NL branch with the preloaded potential
+        double extra[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) extra[c] = potv[c][i];
+        if (m.valid) nl_point<NC, true>(a, extra, va);
+      } else {
+        if (m.valid) v_point<VK, NC>(a, x0, m.base + j, va, Om, eta);
+      }
 #pragma unroll
       for (int c = 0; c < NC; ++c) v[c][i] = a[c];
     }
