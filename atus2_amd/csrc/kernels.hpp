@@ -858,12 +858,24 @@ __global__ __launch_bounds__(LB) void k_block1d(double2* __restrict__ psi, long 
   for (int c = 0; c < NC; ++c) load_pencil<N, RM>(v[c], psi + c0 + c * comp_stride, m.base, 1, m.t, m.valid);
   __syncthreads();
   auto kinetic = [&](const double2* e) {
+    if constexpr (NC == 1 && !tables_in_lds<N>()) {
+      // phases from global memory: loaded before the forward transform,
+      // whose barriers the loads could not otherwise move across
+      double2 ev[R];
 #pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      fft_pencil<N, RM, -1, WAVE>(v[c], m.t, lds, twl);
+      for (int i = 0; i < R; ++i) ev[i] = e[m.t + P * i];
+      fft_pencil<N, RM, -1, WAVE>(v[0], m.t, lds, twl);
 #pragma unroll
-      for (int i = 0; i < R; ++i) v[c][i] = cmul(v[c][i], e[m.t + P * i]);
-      fft_pencil<N, RM, +1, WAVE>(v[c], m.t, lds, twl);
+      for (int i = 0; i < R; ++i) v[0][i] = cmul(v[0][i], ev[i]);
+      fft_pencil<N, RM, +1, WAVE>(v[0], m.t, lds, twl);
+    } else {
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        fft_pencil<N, RM, -1, WAVE>(v[c], m.t, lds, twl);
+#pragma unroll
+        for (int i = 0; i < R; ++i) v[c][i] = cmul(v[c][i], e[m.t + P * i]);
+        fft_pencil<N, RM, +1, WAVE>(v[c], m.t, lds, twl);
+      }
     }
   };
   // the potential is the same for every step of the block: read once into
