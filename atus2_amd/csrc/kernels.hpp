@@ -791,6 +791,15 @@ __global__ __launch_bounds__(LB) void k_kin(double2* __restrict__ psi, long long
     }
   }
   if constexpr (KT || KE) __syncthreads();
+  // strided columns whose phase table stays in global memory, at 8 values
+  // per thread: the phases are loaded before the forward transform, whose
+  // barriers the loads could not otherwise move across
+  constexpr bool E_PRE = !CONTIG && !KE && NC == 1 && R <= 8;
+  double2 epre[E_PRE ? R : 1];
+  if constexpr (E_PRE) {
+#pragma unroll
+    for (int i = 0; i < R; ++i) epre[i] = e0l[m.t + P * i];
+  }
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     fft_pencil<N, RM, -1, WAVE>(v[c], m.t, lds, fresh_tw<N, KT>(twl));
@@ -803,6 +812,8 @@ __global__ __launch_bounds__(LB) void k_kin(double2* __restrict__ psi, long long
       double2 e;
       if constexpr (CONTIG) {
         e = ezr ? ezr[m.t + P * i] : e0l[m.t + P * i];
+      } else if constexpr (E_PRE) {
+        e = cmul(epre[i], cp);
       } else {
         e = cmul(e0l[m.t + P * i], cp);
       }
@@ -962,6 +973,14 @@ __global__ __launch_bounds__(256) void k_vfuse(double2* __restrict__ psi, long l
   double2 v[NC][R];
 #pragma unroll
   for (int c = 0; c < NC; ++c) load_pencil<N, RM>(v[c], psi + c * comp_stride, m.base, 1, m.t, m.valid);
+  // one component's potential is loaded with the pencil, before the
+  // transforms' barriers, which the loads could not otherwise move across
+  constexpr bool POT_PRE = VK == V_NL && NC == 1;
+  double potv[POT_PRE ? R : 1];
+  if constexpr (POT_PRE) {
+#pragma unroll
+    for (int i = 0; i < R; ++i) potv[i] = (va.pot && m.valid) ? va.pot[m.base + m.t + P * i] : 0.0;
+  }
   if constexpr (tables_in_lds<N>()) __syncthreads();
   // the half kinetic phase along this axis, on the spectrum of component c
   auto kinetic_phase = [&](int c) {
@@ -1004,7 +1023,13 @@ __global__ __launch_bounds__(256) void k_vfuse(double2* __restrict__ psi, long l
       double2 a[NC];
 #pragma unroll
       for (int c = 0; c < NC; ++c) a[c] = v[c][i];
-      if (m.valid) v_point<VK, NC>(a, x0, m.base + m.t + P * i, va, Om, eta);
+      if constexpr (POT_PRE) {
+        // v_point's V_NL branch with the preloaded potential
+        double extra[1] = {potv[i]};
+        if (m.valid) nl_point<NC, true>(a, extra, va);
+      } else {
+        if (m.valid) v_point<VK, NC>(a, x0, m.base + m.t + P * i, va, Om, eta);
+      }
 #pragma unroll
       for (int c = 0; c < NC; ++c) v[c][i] = a[c];
     }
