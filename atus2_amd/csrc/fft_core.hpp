@@ -222,13 +222,81 @@ __device__ __forceinline__ void run_stages(double2 (&v)[(Plan<N, RM>::R)], int t
   }
 }
 
+// Twiddles from global memory (N > 1024) at up to 8 values per thread: a
+// stage's twiddles (at most R - 1 per thread) are loaded before the
+// previous stage's exchange, so their latency hides behind its barriers
+// instead of following them.  Same values, same arithmetic.
+template <int N, int RM, int s, class TW>
+__device__ __forceinline__ void load_stage_tw(double2 (&w)[8], int t, TW tw) {
+  using PL = Plan<N, RM>;
+  constexpr int R = PL::R, P = PL::P;
+  constexpr int r = PL::template radix<s>();
+  constexpr int Ls = PL::template span<s>();
+  constexpr int Qn = R / r;
+  static_assert(Qn * (r - 1) <= 8, "prefetched twiddles of up to 8 values per thread");
+  if constexpr (Ls > 1) {
+#pragma unroll
+    for (int q = 0; q < Qn; ++q) {
+      const int k = (t + q * P) & (Ls - 1);
+      const TW tws = tw + PL::template tw_off<s>() + k;
+#pragma unroll
+      for (int m = 1; m < r; ++m) w[q * (r - 1) + m - 1] = load_tw(tws, (m - 1) * Ls);
+    }
+  }
+}
+
+template <int N, int RM, int s, int DIR>
+__device__ __forceinline__ void stage_compute_w(double2 (&v)[(Plan<N, RM>::R)], const double2 (&w)[8]) {
+  using PL = Plan<N, RM>;
+  constexpr int R = PL::R;
+  constexpr int r = PL::template radix<s>();
+  constexpr int Ls = PL::template span<s>();
+  constexpr int Qn = R / r;
+#pragma unroll
+  for (int q = 0; q < Qn; ++q) {
+    double2 x[r];
+#pragma unroll
+    for (int m = 0; m < r; ++m) x[m] = v[q + m * Qn];
+    if constexpr (Ls > 1) {
+#pragma unroll
+      for (int m = 1; m < r; ++m) {
+        const double2 wm = w[q * (r - 1) + m - 1];
+        x[m] = (DIR < 0) ? cmul(x[m], wm) : cmulc(x[m], wm);
+      }
+    }
+    dft<r, DIR>(x);
+#pragma unroll
+    for (int m = 0; m < r; ++m) v[q + m * Qn] = x[m];
+  }
+}
+
+template <int N, int RM, int s, int DIR, bool WAVE_ONLY, class TW>
+__device__ __forceinline__ void run_stages_pf(double2 (&v)[(Plan<N, RM>::R)], int t, double2* lds, TW tw,
+                                              const double2 (&w)[8]) {
+  stage_compute_w<N, RM, s, DIR>(v, w);
+  if constexpr (s + 1 < Plan<N, RM>::S) {
+    double2 wn[8];
+    load_stage_tw<N, RM, s + 1>(wn, t, tw);
+    stage_exchange<N, RM, s, WAVE_ONLY>(v, t, lds);
+    run_stages_pf<N, RM, s + 1, DIR, WAVE_ONLY, TW>(v, t, lds, tw, wn);
+  }
+}
+
 // full unnormalised N-point DFT of the pencil held in registers (layout
 // t + P*i in and out).  lds: this pencil's region of at least N elements;
 // tw: the plan's stage-major twiddle table (fft_tw), a pointer to LDS or
 // global memory.
 template <int N, int RM, int DIR, bool WAVE_ONLY, class TW>
 __device__ __forceinline__ void fft_pencil(double2 (&v)[(Plan<N, RM>::R)], int t, double2* lds, TW tw) {
-  if constexpr (N > 1) run_stages<N, RM, 0, DIR, WAVE_ONLY, TW>(v, t, lds, tw);
+  if constexpr (N > 1) {
+    if constexpr (!__is_same(TW, const double2*) && Plan<N, RM>::R <= 8) {
+      double2 w0[8];
+      load_stage_tw<N, RM, 0>(w0, t, tw);  // stage 0 reads no twiddles
+      run_stages_pf<N, RM, 0, DIR, WAVE_ONLY, TW>(v, t, lds, tw, w0);
+    } else {
+      run_stages<N, RM, 0, DIR, WAVE_ONLY, TW>(v, t, lds, tw);
+    }
+  }
 }
 
 }  // namespace atus
