@@ -791,15 +791,6 @@ __global__ __launch_bounds__(LB) void k_kin(double2* __restrict__ psi, long long
     }
   }
   if constexpr (KT || KE) __syncthreads();
-  // strided columns whose phase table stays in global memory, at 8 values
-  // per thread: the phases are loaded before the forward transform, whose
-  // barriers the loads could not otherwise move across
-  constexpr bool E_PRE = !CONTIG && !KE && NC == 1 && R <= 8;
-  double2 epre[E_PRE ? R : 1];
-  if constexpr (E_PRE) {
-#pragma unroll
-    for (int i = 0; i < R; ++i) epre[i] = e0l[m.t + P * i];
-  }
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     fft_pencil<N, RM, -1, WAVE>(v[c], m.t, lds, fresh_tw<N, KT>(twl));
@@ -812,8 +803,6 @@ __global__ __launch_bounds__(LB) void k_kin(double2* __restrict__ psi, long long
       double2 e;
       if constexpr (CONTIG) {
         e = ezr ? ezr[m.t + P * i] : e0l[m.t + P * i];
-      } else if constexpr (E_PRE) {
-        e = cmul(epre[i], cp);
       } else {
         e = cmul(e0l[m.t + P * i], cp);
       }
