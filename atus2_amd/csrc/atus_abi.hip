@@ -1088,10 +1088,15 @@ int single_block_slabs(atus_ctx* c, int vk, const VArgs& base, int Nk, double dt
   const int n0 = c->n[0], N0 = c->n[0];
   const int S = c->slab_planes() > 0 ? c->slab_planes() : n0;
   const long long inner0 = c->inner_of(0);
-  auto kin = [&](const KinTabs& kt, const VArgs& va) -> int {
+  // comp0_last: the components in reverse order (component 0's workgroups
+  // dispatched last), so that the block's final pass leaves component 0 in
+  // the last-level cache for Rabi integrals that follow; one species only
+  auto kin = [&](const KinTabs& kt, const VArgs& va, bool comp0_last = false) -> int {
     ProfScope ps(c, ATUS_KFAM_KIN);
-    HIP_TRY(launch_kin(N0, c->rm[ATUS_KFAM_KIN], false, V_NONE, 1, c->psi, c->npts, c->ncomp, inner0, c->tw[N0], kt,
-                       va, c->dx[0], lim(c, ATUS_KFAM_KIN), c->stream));
+    const bool rev = comp0_last && kt.f0 == nullptr && c->ncomp > 1;
+    double2* base = rev ? c->psi + (long long)(c->ncomp - 1) * c->npts : c->psi;
+    HIP_TRY(launch_kin(N0, c->rm[ATUS_KFAM_KIN], false, V_NONE, 1, base, rev ? -c->npts : c->npts, c->ncomp, inner0,
+                       c->tw[N0], kt, va, c->dx[0], lim(c, ATUS_KFAM_KIN), c->stream));
     return ATUS_OK;
   };
   // odd slabs on the second stream (profiled blocks record their events on
@@ -1140,7 +1145,7 @@ int single_block_slabs(atus_ctx* c, int vk, const VArgs& base, int Nk, double dt
       else ATUS_TRY(pass_slab(c, 1, -1, s0, Ss, st_of(s0)));
     }
     ATUS_TRY(join());
-    ATUS_TRY(kin(last ? axis_tabs(c, 0) : kin_tabs(c, 1), va));
+    ATUS_TRY(kin(last ? axis_tabs(c, 0) : kin_tabs(c, 1), va, last));
     tt += last ? 0.5 * dt : dt;
   }
   *t = tt;
