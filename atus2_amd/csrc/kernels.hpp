@@ -295,9 +295,15 @@ __device__ __forceinline__ void bragg_field(double x0, const VArgs& p, double& O
   const double F = p.Fp ? *p.Fp : p.F;
   Om = F * p.amp0 *
        cos(p.laser_k * x0 - (p.laser_domh + p.chirp * t1 + p.chirp_rate0 * t1) * t1 + p.phase0 / 2);
-  double s, c;
-  sincos(-0.5 * (p.laser_dk * x0), &s, &c);
-  eta = make_double2(c, s);
+  const double arg = -0.5 * (p.laser_dk * x0);
+  if (p.laser_dk == 0.0) {
+    // arg is +-0: cos = 1 and sin = arg exactly (the sign of zero kept)
+    eta = make_double2(1.0, arg);
+  } else {
+    double s, c;
+    sincos(arg, &s, &c);
+    eta = make_double2(c, s);
+  }
 }
 
 // per-point analytic 2x2 exponential of Do_Bragg_ad (bragg.cpp:226-301),
