@@ -138,10 +138,10 @@ constexpr bool tables_in_lds() {
 }
 // k_kin's tables in LDS: the twiddles its plan reads (plan_tw_elems, at
 // most N - 1 entries) when N <= 1024; the phase table e0 (N entries) as
-// well, except for strided columns of 1024 points.  There the pencils of a
-// workgroup already take 64 KiB of LDS (4 pencils at 16 values per thread,
-// 4 at 8 values and 512 threads): with the twiddles (16128 or 16256 B) two
-// workgroups still fit a CU's 160 KiB, with e0 on top only one would.
+// well, except for strided columns of 1024 points.  There the 4 pencils of
+// a 256-thread workgroup at 16 values per thread already take 64 KiB of
+// LDS: with the twiddles (16128 B) two workgroups still fit a CU's 160 KiB,
+// with e0 on top only one would.
 template <int N, bool CONTIG>
 constexpr bool kin_tw_in_lds() {
   return tables_in_lds<N>();
